@@ -1,0 +1,162 @@
+/* libsvdimager — MI355X (gfx950) snapshot imager behind the egbdfX/SVDimager host API.
+ *
+ * Pipeline: SVD rotation of uvw -> flag / WEIGHT_SPECTRUM polarisation collapse with MFS channel
+ * folding -> exponential-of-semicircle gridding of Chebyshev w-moments -> batched 2D inverse
+ * FFT -> image-plane Chebyshev w-phase recombination with ES grid correction -> SIN
+ * reprojection splat -> dirty image.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, never throws, and returns
+ * an svdi_status.  svdi_last_error() gives the message of the last failure on the calling
+ * thread.  Device pointers are HIP device memory of the context's device.
+ *
+ * The C++-linkage symbol FIpipe (archive/pipe_interface.cpp:8) is exported too, with the
+ * exact signature the reference's archive CLI links against; it is declared at the end of
+ * this header for C++ callers.
+ */
+#ifndef SVDIMAGER_H
+#define SVDIMAGER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVDI_ABI_VERSION 1
+
+typedef enum svdi_status {
+    SVDI_OK = 0,
+    SVDI_ERR_INVALID_ARGUMENT = 1,
+    SVDI_ERR_HIP = 2,
+    SVDI_ERR_OUT_OF_MEMORY = 3,
+    SVDI_ERR_UNSUPPORTED = 4,
+    SVDI_ERR_IO = 5,
+    SVDI_ERR_INTERNAL = 6
+} svdi_status;
+
+typedef struct svdi_context svdi_context;
+
+/* One TIME snapshot, the fields of HostMeasurementSetData (SVDIntegratedImager.cu:39-54) with
+ * uvw and channel frequencies kept in FP64.  Sample index = row + chan * num_rows
+ * (SVDIntegratedImager.cu:90-92).  weight0 / weight3 may be NULL when WEIGHT_SPECTRUM is
+ * absent (weights 1, SVDIntegratedImager.cu:252-253).  All pointers are device pointers. */
+typedef struct svdi_snapshot {
+    size_t num_rows;
+    size_t num_channels;
+    const double* uvw;          /* num_rows x 3, metres */
+    const double* freq_hz;      /* num_channels */
+    const float* vis0_re;
+    const float* vis0_im;
+    const float* vis3_re;
+    const float* vis3_im;
+    const uint8_t* flag0;
+    const uint8_t* flag3;
+    const float* weight0;
+    const float* weight3;
+    double array_center_x;      /* OBSERVATION/ARRAY_CENTER x, metres (0 = unknown) */
+} svdi_snapshot;
+
+typedef enum svdi_gridder {
+    SVDI_GRIDDER_AUTO = 0,      /* fastest available */
+    SVDI_GRIDDER_ATOMIC = 1,    /* one thread per sample, device atomics (the reference's scheme) */
+    SVDI_GRIDDER_TILED = 2      /* samples binned into LDS-resident grid tiles, FP32 */
+} svdi_gridder;
+
+typedef enum svdi_output {
+    SVDI_OUTPUT_DIRTY = 0,      /* |.| of each slab batch, bilinearly splatted (the reference's image) */
+    SVDI_OUTPUT_SIGNED = 1      /* signed, grid-corrected image before |.| and reprojection, x 1/S */
+} svdi_output;
+
+typedef enum svdi_semantics {
+    SVDI_SEMANTICS_INTEGRATED = 0,  /* SVDIntegratedImager.cu: slabs, K in {8,16,24,32} */
+    SVDI_SEMANTICS_ARCHIVE = 1      /* archive/FIPkernels.cu: one slab, smallest K in 1..32 */
+} svdi_semantics;
+
+typedef struct svdi_options {
+    int gridder;            /* svdi_gridder */
+    int output;             /* svdi_output */
+    int semantics;          /* svdi_semantics */
+    int num_terms;          /* 0 = the reference's selection, else a fixed K in 1..32 */
+    float target_alpha;     /* <= 0: SVD_CHEB_TARGET_ALPHA from the environment, else 10 */
+    size_t max_batch_slabs; /* 0: SVDIMAGER_MAX_BATCH_SLABS from the environment, else 25% of free HBM */
+    int verbose;            /* 1: print the reference's Chebyshev report and "Time elapsed" to stdout */
+} svdi_options;
+
+/* Planning results of one imaging call. */
+typedef struct svdi_plan_info {
+    float r3_min, r3_max, max_abs_t3;
+    uint32_t valid_pixels;
+    float alpha_max;        /* Phi_max */
+    float target_alpha;
+    int num_slabs;
+    float slab_width, slab_half_range, slab_alpha_max;
+    int num_terms;          /* selected K */
+    float term_error;       /* sampled Chebyshev error of the selected K */
+    float bucket_errors[4]; /* integrated semantics: errors at K = 8, 16, 24, 32 */
+    size_t batch_slabs;
+    size_t grid_size;
+    size_t num_planes;      /* slabs x K */
+    size_t records;         /* (sample, tile) records gridded by the tiled gridder */
+    float elapsed_ms;       /* device time from planning to the finished image */
+} svdi_plan_info;
+
+int svdi_version(void);
+const char* svdi_last_error(void);
+void svdi_default_options(svdi_options* opts);
+
+/* Context: one HIP device, one stream, a workspace reused across calls.  Calls on one context
+ * are serialised; use one context per host thread for concurrency. */
+int svdi_create(svdi_context** ctx, int device);
+int svdi_destroy(svdi_context* ctx);
+/* Run subsequent work on a caller stream (hipStream_t); NULL restores the context's stream. */
+int svdi_set_stream(svdi_context* ctx, void* hip_stream);
+/* Release the workspace (it regrows on the next call). */
+int svdi_trim(svdi_context* ctx);
+
+/* preprocess_measurement_set_gpu (SVDIntegratedImager.cu:635-853): device snapshot ->
+ * Bin (S x 3, rotated, uncentred, wavelengths), Vin (3 x 3 row-major), collapsed vis (S each).
+ * Output buffers are caller-allocated device memory.  The rotation is computed in FP64. */
+int svdi_preprocess(svdi_context* ctx, const svdi_snapshot* snapshot, float* bin, float* vin,
+                    float* vis_re, float* vis_im);
+
+/* FIpipeDevice (SVDIntegratedImager.cu:1498-1761): preprocessed device buffers -> image_size^2
+ * float image in device memory (row-major, x fastest).  opts / info may be NULL. */
+int svdi_image_preprocessed(svdi_context* ctx, const float* bin, const float* vin,
+                            const float* vis_re, const float* vis_im, size_t num_samples,
+                            float* image, size_t image_size, float cell_size,
+                            const svdi_options* opts, svdi_plan_info* info);
+
+/* preprocess_measurement_set_gpu followed by FIpipeDevice (SVDIntegratedImager.cu:1822-1828)
+ * without materialising Bin: positions are rotated in FP64 and visibilities collapsed inside
+ * the gridding pass. */
+int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snapshot, float* image,
+                        size_t image_size, float cell_size, const svdi_options* opts,
+                        svdi_plan_info* info);
+
+/* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
+ * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
+int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,
+                     const float* bin, const float* vin, float* dirty_image,
+                     size_t num_baselines, size_t image_size, float cell_size,
+                     const svdi_options* opts, svdi_plan_info* info);
+
+/* FITS primary-HDU images, BITPIX -32 (write_fits_2d, SVDIntegratedImager.cu:153-176; the
+ * archive's read_fits_image / write_fits_image, pipe_interface.cpp:12-137).  write replaces
+ * an existing file.  read returns 1-D (naxes[1] = 1) or 2-D images as float; free with
+ * svdi_free. */
+int svdi_fits_write(const char* path, const float* data, long naxis1, long naxis2);
+int svdi_fits_read(const char* path, float** data, long naxes[2]);
+void svdi_free(void* p);
+
+#ifdef __cplusplus
+}  /* extern "C" */
+
+/* archive/pipe_interface.cpp:8 — C++ linkage, mangled _Z6FIpipePfS_S_S_S_mmf.  Host buffers;
+ * prints the archive's Chebyshev range check and "Time elapsed" like FIPkernels.cu:733-757,888;
+ * returns 0 on success, nonzero on failure. */
+int FIpipe(float* Visreal, float* Visimag, float* Bin, float* Vin, float* dirty_image,
+           size_t num_baselines, size_t image_size, float cell_size);
+#endif
+
+#endif /* SVDIMAGER_H */

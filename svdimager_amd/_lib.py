@@ -1,0 +1,171 @@
+"""ctypes binding of libsvdimager (the C-ABI declared in include/svdimager.h).
+
+There is no CPU fallback: importing a function that needs the library raises when
+libsvdimager.so is missing, and every call that fails raises SvdiError with the library's
+message.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libsvdimager.so")
+
+SVDI_OK = 0
+GRIDDER_AUTO, GRIDDER_ATOMIC, GRIDDER_TILED = 0, 1, 2
+OUTPUT_DIRTY, OUTPUT_SIGNED = 0, 1
+SEMANTICS_INTEGRATED, SEMANTICS_ARCHIVE = 0, 1
+
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "HIP", 3: "OUT_OF_MEMORY", 4: "UNSUPPORTED",
+                5: "IO", 6: "INTERNAL"}
+
+# Every symbol include/svdimager.h declares.
+EXPORTED_SYMBOLS = [
+    "svdi_version", "svdi_last_error", "svdi_default_options", "svdi_create", "svdi_destroy",
+    "svdi_set_stream", "svdi_trim", "svdi_preprocess", "svdi_image_preprocessed",
+    "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
+    "_Z6FIpipePfS_S_S_S_mmf",
+]
+
+
+class SvdiError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"svdimager: {STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Snapshot(ctypes.Structure):
+    _fields_ = [
+        ("num_rows", ctypes.c_size_t), ("num_channels", ctypes.c_size_t),
+        ("uvw", ctypes.c_void_p), ("freq_hz", ctypes.c_void_p),
+        ("vis0_re", ctypes.c_void_p), ("vis0_im", ctypes.c_void_p),
+        ("vis3_re", ctypes.c_void_p), ("vis3_im", ctypes.c_void_p),
+        ("flag0", ctypes.c_void_p), ("flag3", ctypes.c_void_p),
+        ("weight0", ctypes.c_void_p), ("weight3", ctypes.c_void_p),
+        ("array_center_x", ctypes.c_double),
+    ]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [
+        ("gridder", ctypes.c_int), ("output", ctypes.c_int), ("semantics", ctypes.c_int),
+        ("num_terms", ctypes.c_int), ("target_alpha", ctypes.c_float),
+        ("max_batch_slabs", ctypes.c_size_t), ("verbose", ctypes.c_int),
+    ]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("r3_min", ctypes.c_float), ("r3_max", ctypes.c_float), ("max_abs_t3", ctypes.c_float),
+        ("valid_pixels", ctypes.c_uint32), ("alpha_max", ctypes.c_float),
+        ("target_alpha", ctypes.c_float), ("num_slabs", ctypes.c_int),
+        ("slab_width", ctypes.c_float), ("slab_half_range", ctypes.c_float),
+        ("slab_alpha_max", ctypes.c_float), ("num_terms", ctypes.c_int),
+        ("term_error", ctypes.c_float), ("bucket_errors", ctypes.c_float * 4),
+        ("batch_slabs", ctypes.c_size_t), ("grid_size", ctypes.c_size_t),
+        ("num_planes", ctypes.c_size_t), ("records", ctypes.c_size_t),
+        ("elapsed_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name == "bucket_errors" else v
+        return d
+
+
+_lib = None
+
+
+def lib():
+    """Load libsvdimager.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run svdimager_amd.build.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, f = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float
+        sigs = {
+            "svdi_version": (ctypes.c_int, []),
+            "svdi_last_error": (ctypes.c_char_p, []),
+            "svdi_default_options": (None, [ctypes.POINTER(Options)]),
+            "svdi_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int]),
+            "svdi_destroy": (ctypes.c_int, [vp]),
+            "svdi_set_stream": (ctypes.c_int, [vp, vp]),
+            "svdi_trim": (ctypes.c_int, [vp]),
+            "svdi_preprocess": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, vp, vp, vp]),
+            "svdi_image_preprocessed": (ctypes.c_int, [vp, vp, vp, vp, vp, sz, vp, sz, f,
+                                                       ctypes.POINTER(Options),
+                                                       ctypes.POINTER(PlanInfo)]),
+            "svdi_image_snapshot": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, sz, f,
+                                                   ctypes.POINTER(Options),
+                                                   ctypes.POINTER(PlanInfo)]),
+            "svdi_fipipe_host": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, sz, sz, f,
+                                                ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
+            "svdi_fits_write": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_long, ctypes.c_long]),
+            "svdi_fits_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
+                                              ctypes.POINTER(ctypes.c_long)]),
+            "svdi_free": (None, [vp]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        fip = getattr(L, "_Z6FIpipePfS_S_S_S_mmf")
+        fip.restype = ctypes.c_int
+        fip.argtypes = [vp, vp, vp, vp, vp, sz, sz, f]
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != SVDI_OK:
+        raise SvdiError(status, lib().svdi_last_error().decode(errors="replace"))
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().svdi_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One libsvdimager context (device, stream, workspace)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().svdi_create(ctypes.byref(self._h), device))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_stream(self, stream_ptr: Optional[int]) -> None:
+        check(lib().svdi_set_stream(self._h, stream_ptr))
+
+    def trim(self) -> None:
+        check(lib().svdi_trim(self._h))
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().svdi_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

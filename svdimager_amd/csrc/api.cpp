@@ -1,0 +1,570 @@
+// C-ABI of libsvdimager and the host orchestration of one imaging call.
+//
+// Sequencing follows FIpipeDevice (SVDIntegratedImager.cu:1498-1761): plan (r3 range, t3
+// stats, slab rule, Chebyshev K) -> per slab batch { grid -> 2D inverse FFT -> image pass }.
+// Differences: the plan is finalised on the device and read back in one copy; buffers, the
+// stream and FFT twiddles live in the context and are reused across calls; errors are status
+// codes, never exceptions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "svdi_internal.hpp"
+
+namespace svdi {
+
+namespace {
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return SVDI_OK;
+    } catch (const ArgError& e) {
+        set_error(e.what());
+        return SVDI_ERR_INVALID_ARGUMENT;
+    } catch (const UnsupportedError& e) {
+        set_error(e.what());
+        return SVDI_ERR_UNSUPPORTED;
+    } catch (const HipError& e) {
+        set_error(e.what());
+        return e.code == hipErrorOutOfMemory ? SVDI_ERR_OUT_OF_MEMORY : SVDI_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return SVDI_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return SVDI_ERR_INTERNAL;
+    } catch (...) {
+        set_error("unknown error");
+        return SVDI_ERR_INTERNAL;
+    }
+}
+
+// SVD_CHEB_TARGET_ALPHA, ignored unless positive (SVDIntegratedImager.cu:963-971).
+float env_target_alpha() {
+    if (const char* s = std::getenv("SVD_CHEB_TARGET_ALPHA")) {
+        const float v = static_cast<float>(std::atof(s));
+        if (v > 0.0f) return v;
+    }
+    return kDefaultTargetAlpha;
+}
+
+// choose_slab_batch_size (SVDIntegratedImager.cu:1470-1496).
+size_t choose_batch_slabs(size_t total_slabs, int K, size_t plane_elems, size_t opt_max) {
+    if (opt_max > 0) return std::min(total_slabs, opt_max);
+    const char* env = std::getenv("SVDIMAGER_MAX_BATCH_SLABS");
+    if (env != nullptr && env[0] != '\0') {
+        size_t requested = 0;
+        try {
+            requested = std::stoull(env);
+        } catch (const std::exception&) {
+            throw ArgError("SVDIMAGER_MAX_BATCH_SLABS is not an unsigned integer");
+        }
+        if (requested == 0) throw ArgError("SVDIMAGER_MAX_BATCH_SLABS must be greater than zero");
+        return std::min(total_slabs, requested);
+    }
+    size_t free_b = 0, total_b = 0;
+    SVDI_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per_slab = static_cast<size_t>(K) * plane_elems * sizeof(float2);
+    const size_t budget = free_b * 25 / 100;
+    return std::min(total_slabs, std::max<size_t>(1, budget / per_slab));
+}
+
+float bits_to_float(uint32_t b) {
+    float f;
+    std::memcpy(&f, &b, sizeof f);
+    return f;
+}
+
+void print_report(const HostPlan& hp, std::ostream& os) {
+    const DevPlan& p = hp.dev;
+    if (hp.semantics == Semantics::Integrated) {
+        os << "Chebyshev summary for paper:\n";
+        os << "  Phi_max                           = " << p.alpha_max << "\n";
+        os << "  alpha_target                      = " << p.target_alpha << "\n";
+        os << "  N_slab                            = " << p.num_slabs << "\n";
+        os << "  alpha_slab                        = " << p.slab_alpha_max << "\n";
+        os << "  selected K                        = " << hp.num_terms << "\n";
+        os << "  N_slab * K moment reconstructions = "
+           << static_cast<size_t>(p.num_slabs) * static_cast<size_t>(hp.num_terms) << "\n";
+        os << "  target approximation error         = " << kTargetError << "\n";
+    } else {
+        os << "Chebyshev range check:\n";
+        os << "  r3 interval = [" << p.r3_min << ", " << p.r3_max << "] wavelengths\n";
+        os << "  r3 centre   = " << p.r3_centre << " wavelengths\n";
+        os << "  r3 halfspan = " << p.r3_half_range << " wavelengths\n";
+        os << "  max |t3|    = " << p.max_abs_t3 << " over " << p.valid_pixels << " valid pixels\n";
+        os << "  alpha_max   = 2*pi*max|t3|*r3_halfspan = " << p.alpha_max << "\n";
+        os << "  target sampled max error            = " << kTargetError << "\n";
+        os << "  selected kept terms                 = " << hp.num_terms << " (keep T0..T"
+           << (hp.num_terms - 1) << ")\n";
+        os << "  selected sampled max error          = " << hp.term_error << "\n";
+        if (hp.term_error < kTargetError) {
+            os << "  Status: workable (sampled error < target)\n";
+        } else if (hp.term_error < 5.0f * kTargetError) {
+            os << "  Status: marginal (sampled error between target and 5x target)\n";
+        } else {
+            os << "  Status: not safe within configured max terms on this dataset\n";
+        }
+    }
+}
+
+void check_device_pointer(const void* p, const char* what) {
+    if (p == nullptr) throw ArgError(std::string(what) + " is null");
+}
+
+// The imaging core shared by every entry point.  `in` describes the samples; the image
+// (N x N floats, device) is overwritten.
+void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
+                const svdi_options& o, svdi_plan_info* info, float* image) {
+    if (N == 0) throw ArgError("Image_Size must be greater than zero");
+    if (!(cell > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
+    if (in.num_samples == 0) throw ArgError("Measurement set is empty");
+    if (o.num_terms < 0 || o.num_terms > kMaxTerms) throw ArgError("num_terms must be in 0..32");
+    if (N > 46340) throw ArgError("Image_Size too large");
+    const size_t G = (N * 3 + 1) / 2;                      // ceiling_divide(image_size * 3, 2)
+    HostPlan hp;
+    hp.semantics = o.semantics == SVDI_SEMANTICS_ARCHIVE ? Semantics::Archive : Semantics::Integrated;
+    hp.image_size = N;
+    hp.grid_size = G;
+    hp.cell = cell;
+    const float target_alpha = o.target_alpha > 0.0f ? o.target_alpha : env_target_alpha();
+
+    DevPlan* dplan = static_cast<DevPlan*>(ctx->ws.get(Workspace::kPlan, sizeof(DevPlan)));
+    SVDI_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    launch_plan(ctx, in, hp.semantics, N, cell, target_alpha, dplan);
+    SVDI_HIP(hipMemcpyAsync(ctx->pinned_plan, dplan, sizeof(DevPlan), hipMemcpyDeviceToHost, ctx->stream));
+    SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    hp.dev = *ctx->pinned_plan;
+
+    // K: bucket rule (SVDIntegratedImager.cu:1577-1603) or smallest passing K (FIPkernels.cu:301-318).
+    if (hp.semantics == Semantics::Integrated) {
+        const int buckets[4] = {8, 16, 24, 32};
+        int K = kMaxTerms;
+        float err = bits_to_float(hp.dev.bucket_err_bits[3]);
+        for (int i = 0; i < 4; ++i) {
+            const float e = bits_to_float(hp.dev.bucket_err_bits[i]);
+            if (e <= kTargetError) {
+                K = buckets[i];
+                err = e;
+                break;
+            }
+        }
+        hp.num_terms = K;
+        hp.term_error = err;
+    } else {
+        int K = -1;
+        for (int k = 1; k <= kMaxTerms; ++k)
+            if (bits_to_float(hp.dev.term_err_bits[k - 1]) <= kTargetError) {
+                K = k;
+                break;
+            }
+        if (K < 0) K = kMaxTerms;
+        hp.num_terms = K;
+        hp.term_error = bits_to_float(hp.dev.term_err_bits[K - 1]);
+    }
+    if (o.num_terms > 0) {
+        hp.num_terms = o.num_terms;
+        hp.term_error = hp.semantics == Semantics::Archive ? bits_to_float(hp.dev.term_err_bits[o.num_terms - 1])
+                                                           : hp.term_error;
+    }
+    const int K = hp.num_terms;
+    const size_t total_slabs = static_cast<size_t>(hp.dev.num_slabs);
+    hp.batch_slabs = choose_batch_slabs(total_slabs, K, G * G, o.max_batch_slabs);
+
+    if (G % 2 == 0) {
+        hp.row_lo = G / 2 - N / 2;
+        hp.col_lo = G / 2 - N / 2;
+        hp.rows = hp.cols = N;
+    } else {                                               // odd G: keep the whole plane
+        hp.row_lo = hp.col_lo = 0;
+        hp.rows = hp.cols = G;
+    }
+    const size_t P = hp.batch_slabs * static_cast<size_t>(K);
+    float2* planes = static_cast<float2*>(ctx->ws.get(Workspace::kPlanes, P * G * G * sizeof(float2)));
+    float2* rowfft = static_cast<float2*>(ctx->ws.get(Workspace::kRowFft, P * G * hp.cols * sizeof(float2)));
+    float2* colfft = static_cast<float2*>(ctx->ws.get(Workspace::kColFft, P * hp.rows * hp.cols * sizeof(float2)));
+    float* conv = static_cast<float*>(ctx->ws.get(Workspace::kConv, (N / 2 + 1) * sizeof(float)));
+
+    if (o.verbose) print_report(hp, std::cout);
+
+    launch_fill(ctx, image, N * N, 0.0f);
+    launch_conv_corr(ctx, N, G, hp.semantics == Semantics::Archive, conv);
+
+    GridParams gp = {};
+    gp.G = static_cast<int>(G);
+    gp.K = K;
+    gp.total_slabs = static_cast<int>(total_slabs);
+    gp.archive = hp.semantics == Semantics::Archive;
+    gp.r3_min = hp.dev.r3_min;
+    gp.slab_width = hp.dev.slab_width;
+    gp.inv_half = hp.dev.inv_slab_half_range;
+    gp.r3_centre = hp.dev.r3_centre;
+    gp.scale_f32 = cell * static_cast<float>(G);
+    gp.scale_f64 = static_cast<double>(cell) * static_cast<double>(G);
+    float det;
+    if (in.from_snapshot) {
+        DevGeometry g;
+        SVDI_HIP(hipMemcpyAsync(&g, in.geo, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        det = g.det;
+    } else {
+        float v[9];
+        SVDI_HIP(hipMemcpyAsync(v, in.vin_dev, sizeof v, hipMemcpyDeviceToHost, ctx->stream));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        det = std::fabs(v[0] * v[4] - v[1] * v[3]);
+    }
+    gp.det = det;
+    gp.inv_det = 1.0f / det;
+
+    ImageParams ip = {};
+    ip.N = static_cast<int>(N);
+    ip.G = static_cast<int>(G);
+    ip.K = K;
+    ip.archive = gp.archive;
+    ip.cell = cell;
+    ip.r3_min = hp.dev.r3_min;
+    ip.slab_width = hp.dev.slab_width;
+    ip.slab_half = hp.dev.slab_half_range;
+    ip.r3_centre = hp.dev.r3_centre;
+    ip.inv_num_samples = 1.0f / static_cast<float>(in.num_samples);
+    ip.rows = static_cast<int>(hp.rows);
+    ip.cols = static_cast<int>(hp.cols);
+    ip.row_lo = static_cast<int>(hp.row_lo);
+    ip.col_lo = static_cast<int>(hp.col_lo);
+    ip.signed_out = o.output == SVDI_OUTPUT_SIGNED;
+    const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
+
+    size_t records = 0;
+    const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_ATOMIC : o.gridder;
+    for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
+        const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
+        gp.slab_start = static_cast<int>(start);
+        gp.nslabs = static_cast<int>(nb);
+        if (gridder == SVDI_GRIDDER_TILED) {
+            records += launch_grid_tiled(ctx, in, gp, planes, 32);
+        } else {
+            launch_grid_atomic(ctx, in, gp, planes);
+        }
+        const size_t Pb = nb * static_cast<size_t>(K);
+        launch_fft2_inverse(ctx, planes, Pb, G, hp.row_lo, hp.rows, hp.col_lo, hp.cols, rowfft, colfft);
+        ip.P = static_cast<int>(Pb);
+        ip.slab_start = static_cast<int>(start);
+        ip.nslabs = static_cast<int>(nb);
+        launch_image(ctx, colfft, vin_dev, conv, ip, image);
+    }
+    SVDI_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+    SVDI_HIP(hipEventSynchronize(ctx->ev_stop));
+    float ms = 0.0f;
+    SVDI_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    if (o.verbose) std::cout << "Time elapsed: " << ms << " ms" << std::endl;
+
+    if (info) {
+        info->r3_min = hp.dev.r3_min;
+        info->r3_max = hp.dev.r3_max;
+        info->max_abs_t3 = hp.dev.max_abs_t3;
+        info->valid_pixels = hp.dev.valid_pixels;
+        info->alpha_max = hp.dev.alpha_max;
+        info->target_alpha = hp.dev.target_alpha;
+        info->num_slabs = hp.dev.num_slabs;
+        info->slab_width = hp.dev.slab_width;
+        info->slab_half_range = hp.dev.slab_half_range;
+        info->slab_alpha_max = hp.dev.slab_alpha_max;
+        info->num_terms = K;
+        info->term_error = hp.term_error;
+        for (int i = 0; i < 4; ++i) info->bucket_errors[i] = bits_to_float(hp.dev.bucket_err_bits[i]);
+        info->batch_slabs = hp.batch_slabs;
+        info->grid_size = G;
+        info->num_planes = total_slabs * static_cast<size_t>(K);
+        info->records = records;
+        info->elapsed_ms = ms;
+    }
+}
+
+svdi_options resolve(const svdi_options* o, int default_semantics) {
+    svdi_options r;
+    svdi_default_options(&r);
+    r.semantics = default_semantics;
+    if (o) r = *o;
+    return r;
+}
+
+struct DeviceGuard {
+    explicit DeviceGuard(int dev) { SVDI_HIP(hipSetDevice(dev)); }
+};
+
+SampleInput snapshot_input(svdi_context* ctx, const svdi_snapshot& s) {
+    if (s.num_rows == 0 || s.num_channels == 0) throw ArgError("Measurement set is empty");
+    check_device_pointer(s.uvw, "uvw");
+    check_device_pointer(s.freq_hz, "freq_hz");
+    check_device_pointer(s.vis0_re, "vis0_re");
+    check_device_pointer(s.vis0_im, "vis0_im");
+    check_device_pointer(s.vis3_re, "vis3_re");
+    check_device_pointer(s.vis3_im, "vis3_im");
+    check_device_pointer(s.flag0, "flag0");
+    check_device_pointer(s.flag3, "flag3");
+    if (s.num_rows > 0xffffffffull || s.num_channels > 65535)
+        throw ArgError("snapshot too large: rows must fit 32 bits and channels 65535");
+    DevGeometry* geo = static_cast<DevGeometry*>(ctx->ws.get(Workspace::kGeometry, sizeof(DevGeometry)));
+    double* rot = static_cast<double*>(ctx->ws.get(Workspace::kRot, s.num_rows * 3 * sizeof(double)));
+    const size_t r3_blocks = (s.num_rows + 255) / 256;
+    float* r3p = static_cast<float*>(ctx->ws.get(Workspace::kR3Partials, r3_blocks * 2 * sizeof(float)));
+    int nb = 0;
+    launch_preprocess_geometry(ctx, s, geo, rot, r3p, &nb);
+    SampleInput in;
+    in.from_snapshot = true;
+    in.num_rows = s.num_rows;
+    in.num_channels = s.num_channels;
+    in.num_samples = s.num_rows * s.num_channels;
+    in.rot = rot;
+    in.freq = s.freq_hz;
+    in.v0r = s.vis0_re;
+    in.v0i = s.vis0_im;
+    in.v3r = s.vis3_re;
+    in.v3i = s.vis3_im;
+    in.f0 = s.flag0;
+    in.f3 = s.flag3;
+    in.w0 = s.weight0;
+    in.w3 = s.weight3;
+    in.geo = geo;
+    in.r3_partials = r3p;
+    in.r3_blocks = nb;
+    return in;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------- Workspace
+void* Workspace::get(Slot s, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    bytes = (bytes + 255) & ~static_cast<size_t>(255);
+    if (size_[s] >= bytes) return ptr_[s];
+    if (ptr_[s] != nullptr) {
+        SVDI_HIP(hipDeviceSynchronize());
+        SVDI_HIP(hipFree(ptr_[s]));
+        ptr_[s] = nullptr;
+        size_[s] = 0;
+    }
+    void* p = nullptr;
+    SVDI_HIP(hipMalloc(&p, bytes));
+    ptr_[s] = p;
+    size_[s] = bytes;
+    return p;
+}
+
+void Workspace::release() {
+    for (int i = 0; i < kSlots; ++i) {
+        if (ptr_[i] != nullptr) (void)hipFree(ptr_[i]);
+        ptr_[i] = nullptr;
+        size_[i] = 0;
+    }
+}
+
+size_t Workspace::bytes_held() const {
+    size_t t = 0;
+    for (int i = 0; i < kSlots; ++i) t += size_[i];
+    return t;
+}
+
+}  // namespace svdi
+
+using namespace svdi;
+
+extern "C" {
+
+int svdi_version(void) { return SVDI_ABI_VERSION; }
+
+const char* svdi_last_error(void) { return g_last_error.c_str(); }
+
+void svdi_default_options(svdi_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->gridder = SVDI_GRIDDER_AUTO;
+    o->output = SVDI_OUTPUT_DIRTY;
+    o->semantics = SVDI_SEMANTICS_INTEGRATED;
+}
+
+int svdi_create(svdi_context** out, int device) {
+    return guarded([&] {
+        if (!out) throw ArgError("ctx is null");
+        *out = nullptr;
+        int count = 0;
+        SVDI_HIP(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) throw ArgError("no such HIP device");
+        SVDI_HIP(hipSetDevice(device));
+        std::unique_ptr<svdi_context> c(new svdi_context());
+        c->device = device;
+        SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
+        SVDI_HIP(hipEventCreate(&c->ev_start));
+        SVDI_HIP(hipEventCreate(&c->ev_stop));
+        SVDI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned_plan), sizeof(DevPlan), hipHostMallocDefault));
+        hipDeviceProp_t prop;
+        SVDI_HIP(hipGetDeviceProperties(&prop, device));
+        c->cu_count = prop.multiProcessorCount;
+        *out = c.release();
+    });
+}
+
+int svdi_destroy(svdi_context* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->ws.release();
+        if (ctx->pinned_plan) (void)hipHostFree(ctx->pinned_plan);
+        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+    });
+}
+
+int svdi_set_stream(svdi_context* ctx, void* stream) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    });
+}
+
+int svdi_trim(svdi_context* ctx) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->ws.release();
+        ctx->twiddle_n = 0;
+    });
+}
+
+int svdi_preprocess(svdi_context* ctx, const svdi_snapshot* snap, float* bin, float* vin,
+                    float* vis_re, float* vis_im) {
+    return guarded([&] {
+        if (!ctx || !snap) throw ArgError("ctx / snapshot is null");
+        check_device_pointer(bin, "bin");
+        check_device_pointer(vin, "vin");
+        check_device_pointer(vis_re, "vis_re");
+        check_device_pointer(vis_im, "vis_im");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        SampleInput in = snapshot_input(ctx, *snap);
+        launch_materialize(ctx, *snap, in.geo, bin, vin, vis_re, vis_im);
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int svdi_image_preprocessed(svdi_context* ctx, const float* bin, const float* vin,
+                            const float* vis_re, const float* vis_im, size_t num_samples,
+                            float* image, size_t image_size, float cell_size,
+                            const svdi_options* opts, svdi_plan_info* info) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        check_device_pointer(bin, "bin");
+        check_device_pointer(vin, "vin");
+        check_device_pointer(vis_re, "vis_re");
+        check_device_pointer(vis_im, "vis_im");
+        check_device_pointer(image, "image");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        SampleInput in;
+        in.num_samples = num_samples;
+        in.bin = bin;
+        in.vis_re = vis_re;
+        in.vis_im = vis_im;
+        in.vin_dev = vin;
+        image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_INTEGRATED), info, image);
+    });
+}
+
+int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snap, float* image,
+                        size_t image_size, float cell_size, const svdi_options* opts,
+                        svdi_plan_info* info) {
+    return guarded([&] {
+        if (!ctx || !snap) throw ArgError("ctx / snapshot is null");
+        check_device_pointer(image, "image");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        SampleInput in = snapshot_input(ctx, *snap);
+        image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_INTEGRATED), info, image);
+    });
+}
+
+int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,
+                     const float* bin, const float* vin, float* dirty_image,
+                     size_t num_baselines, size_t image_size, float cell_size,
+                     const svdi_options* opts, svdi_plan_info* info) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        if (!vis_re || !vis_im || !bin || !vin || !dirty_image) throw ArgError("null host buffer");
+        if (num_baselines == 0) throw ArgError("num_baselines must be greater than zero");
+        if (image_size == 0) throw ArgError("Image_Size must be greater than zero");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        const size_t S = num_baselines;
+        float* d_bin = static_cast<float*>(ctx->ws.get(Workspace::kInBin, S * 3 * sizeof(float)));
+        float* d_vre = static_cast<float*>(ctx->ws.get(Workspace::kInVre, S * sizeof(float)));
+        float* d_vim = static_cast<float*>(ctx->ws.get(Workspace::kInVim, S * sizeof(float)));
+        float* d_vin = static_cast<float*>(ctx->ws.get(Workspace::kInVin, 9 * sizeof(float)));
+        float* d_img = static_cast<float*>(ctx->ws.get(Workspace::kImage, image_size * image_size * sizeof(float)));
+        SVDI_HIP(hipMemcpyAsync(d_bin, bin, S * 3 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        SVDI_HIP(hipMemcpyAsync(d_vre, vis_re, S * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        SVDI_HIP(hipMemcpyAsync(d_vim, vis_im, S * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        SVDI_HIP(hipMemcpyAsync(d_vin, vin, 9 * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+        SampleInput in;
+        in.num_samples = S;
+        in.bin = d_bin;
+        in.vis_re = d_vre;
+        in.vis_im = d_vim;
+        in.vin_dev = d_vin;
+        image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_ARCHIVE), info, d_img);
+        SVDI_HIP(hipMemcpyAsync(dirty_image, d_img, image_size * image_size * sizeof(float),
+                                hipMemcpyDeviceToHost, ctx->stream));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void svdi_free(void* p) { std::free(p); }
+
+}  // extern "C"
+
+// archive/pipe_interface.cpp:8 — the C++-linkage entry point the archive CLI links against.
+// One process-wide context, created on first use, serialises calls.
+int FIpipe(float* Visreal, float* Visimag, float* Bin, float* Vin, float* dirty_image,
+           size_t num_baselines, size_t image_size, float cell_size) {
+    static std::mutex mu;
+    static svdi_context* ctx = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (ctx == nullptr) {
+        int dev = 0;
+        if (const char* e = std::getenv("SVDIMAGER_DEVICE")) dev = std::atoi(e);
+        if (svdi_create(&ctx, dev) != SVDI_OK) {
+            std::fprintf(stderr, "FIpipe: %s\n", svdi_last_error());
+            ctx = nullptr;
+            return 1;
+        }
+    }
+    svdi_options o;
+    svdi_default_options(&o);
+    o.semantics = SVDI_SEMANTICS_ARCHIVE;
+    o.verbose = 1;
+    const int rc = svdi_fipipe_host(ctx, Visreal, Visimag, Bin, Vin, dirty_image, num_baselines,
+                                    image_size, cell_size, &o, nullptr);
+    if (rc != SVDI_OK) {
+        std::fprintf(stderr, "Error sync : %s\n", svdi_last_error());
+        return rc;
+    }
+    return 0;
+}

@@ -1,0 +1,149 @@
+// Image-plane pass (fused_interpolation, SVDIntegratedImager.cu:1295-1463; accumulation /
+// scaling / coordschange / p2p / finalinterp, archive/FIPkernels.cu:463-677) and the ES grid
+// correction c(t) (convolveKernel, SVDIntegratedImager.cu:1204-1216).
+//
+// Per pixel (x fastest; idxr = x - N/2, idyr = y - N/2):
+//   t1 = cell idyr, t2 = cell idxr, t3 = -v33 + sgn(v33) sqrt(1 - (t1+v13)^2 - (t2+v23)^2),
+//   sum = Re sum_slabs sum_k e^{i 2 pi t3 r3c} (k ? 2 : 1) i^k J_k(2 pi t3 h) M_{slab,k}(y, x),
+//   sum *= (-1)^(idxr+idyr) / (c(|idxr|) c(|idyr|) 2.4937^2),
+// then |sum| / S is splatted bilinearly at the SIN re-projection of the pixel.  t3, the
+// w-phase and the Bessel sequence are FP64 (the reference evaluates them in FP32).
+#include "svdi_internal.hpp"
+
+namespace svdi {
+namespace {
+
+struct GL28Table {
+    double node[14];
+    double weight[14];
+};
+
+GL28Table make_gl28() {
+    GL28Table g;
+    const int n = 28;
+    for (int i = 0; i < 14; ++i) {
+        double x = std::cos(kPi * (i + 0.75) / (n + 0.5));
+        for (int it = 0; it < 100; ++it) {
+            double p0 = 1.0, p1 = x;
+            for (int k = 2; k <= n; ++k) {
+                const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            const double dp = n * (x * p1 - p0) / (x * x - 1.0);
+            const double dx = p1 / dp;
+            x -= dx;
+            if (std::fabs(dx) < 1e-16) break;
+        }
+        double p0 = 1.0, p1 = x;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        const double dp = n * (x * p1 - p0) / (x * x - 1.0);
+        g.node[i] = x;
+        g.weight[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    }
+    return g;
+}
+
+// c(t) = 8 / 2.4937 * sum_i ES(x_i) w_i cos(pi t 8 x_i / G), t = 0 .. N/2: the Fourier
+// transform of the ES kernel by the positive half of a 28-point Gauss-Legendre rule.
+__global__ void k_conv_corr(int N, int G, GL28Table gl, float* __restrict__ conv) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > N / 2) return;
+    const double tn = static_cast<double>(t) / static_cast<double>(G);
+    double acc = 0.0;
+    for (int i = 0; i < 14; ++i)
+        acc += es_kernel_f64(gl.node[i]) * gl.weight[i] * cos(kPi * tn * kSupport * gl.node[i]);
+    conv[t] = static_cast<float>(acc * kSupport / 2.4937047051153827);
+}
+
+__global__ void __launch_bounds__(256)
+k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restrict__ vin,
+        const float* __restrict__ conv, float* __restrict__ image) {
+    const int x = blockIdx.x * 16 + threadIdx.x;
+    const int y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= ip.N || y >= ip.N) return;
+    const long half = ip.N / 2;
+    const long idxr = x - half, idyr = y - half;
+    float V[9];
+    for (int i = 0; i < 9; ++i) V[i] = vin[i];
+    const double t1 = static_cast<double>(ip.cell) * static_cast<double>(idyr);
+    const double t2 = static_cast<double>(ip.cell) * static_cast<double>(idxr);
+    const double v13 = V[2], v23 = V[5], v33 = V[8];
+    const double ra = 1.0 - (t1 + v13) * (t1 + v13) - (t2 + v23) * (t2 + v23);
+    if (ra < 0.0) return;
+    const double t3 = -v33 + (v33 >= 0.0 ? 1.0 : -1.0) * sqrt(ra);
+    const double alpha = 2.0 * kPi * t3 * static_cast<double>(ip.slab_half);
+    double jn[kMaxTerms];
+    bessel_sequence_f64(alpha, ip.K, jn);
+    // Plane element read by this pixel: linear index G*G/2 + G/2 + idyr*G + idxr of a G x G
+    // plane (SVDIntegratedImager.cu:1397), located in the transformed window.
+    const long G = ip.G;
+    const long lin = (G * G) / 2 + G / 2 + idyr * G + idxr;
+    const long gr = lin / G - ip.row_lo, gc = lin % G - ip.col_lo;
+    const size_t plane_stride = static_cast<size_t>(ip.rows) * ip.cols;
+    const float2* px = planes + static_cast<size_t>(gr) * ip.cols + gc;
+    double sum = 0.0;
+    for (int ls = 0; ls < ip.nslabs; ++ls) {
+        const float centre = ip.archive ? ip.r3_centre
+                                        : ip.r3_min + (static_cast<float>(ip.slab_start + ls) + 0.5f) * ip.slab_width;
+        double sb, cb;
+        sincos(2.0 * kPi * t3 * static_cast<double>(centre), &sb, &cb);
+        for (int k = 0; k < ip.K; ++k) {
+            const float2 m = px[static_cast<size_t>(ls * ip.K + k) * plane_stride];
+            const double a = (k == 0 ? 1.0 : 2.0) * jn[k];
+            double cr, ci;
+            switch (k & 3) {                    // a i^k e^{i beta}
+                case 0: cr = a * cb; ci = a * sb; break;
+                case 1: cr = -a * sb; ci = a * cb; break;
+                case 2: cr = -a * cb; ci = -a * sb; break;
+                default: cr = a * sb; ci = -a * cb; break;
+            }
+            sum += cr * m.x - ci * m.y;
+        }
+    }
+    float ps = static_cast<float>(sum);
+    if ((idxr + idyr) & 1) ps = -ps;
+    ps *= 1 / (conv[labs(idxr)] * conv[labs(idyr)] * kConvCorrNorm * kConvCorrNorm);
+    if (ip.signed_out) {
+        image[static_cast<size_t>(y) * ip.N + x] += ps * ip.inv_num_samples;
+        return;
+    }
+    const float value = fabsf(ps) * ip.inv_num_samples;
+    float oi0, oi1;
+    sin_reproject(ip.N, ip.cell, V, x, y, &oi0, &oi1);
+    const float LL = oi0 - half;
+    const float MM = oi1 - half;
+    if (fabsf(LL) < half - 1 && fabsf(MM) < half - 1) {
+        const float LLf = floorf(LL), MMf = floorf(MM);
+        const float LLc = ceilf(LL), MMc = ceilf(MM);
+        float* d = image + half * ip.N + half;
+        const long n = ip.N;
+        unsafeAtomicAdd(&d[static_cast<long>(MMf) * n + static_cast<long>(LLf)], (1 - LL + LLf) * (1 - MM + MMf) * value);
+        unsafeAtomicAdd(&d[static_cast<long>(MMc) * n + static_cast<long>(LLf)], (1 - LL + LLf) * (0 + MM - MMf) * value);
+        unsafeAtomicAdd(&d[static_cast<long>(MMf) * n + static_cast<long>(LLc)], (0 + LL - LLf) * (1 - MM + MMf) * value);
+        unsafeAtomicAdd(&d[static_cast<long>(MMc) * n + static_cast<long>(LLc)], (0 + LL - LLf) * (0 + MM - MMf) * value);
+    }
+}
+
+}  // namespace
+
+void launch_conv_corr(svdi_context* ctx, size_t N, size_t G, int /*archive_variant*/, float* conv) {
+    static const GL28Table gl = make_gl28();
+    const int count = static_cast<int>(N / 2 + 1);
+    k_conv_corr<<<(count + 255) / 256, 256, 0, ctx->stream>>>(static_cast<int>(N), static_cast<int>(G), gl, conv);
+    SVDI_LAUNCH_CHECK();
+}
+
+void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev, const float* conv,
+                  const ImageParams& ip, float* image) {
+    dim3 block(16, 16);
+    dim3 grid(static_cast<unsigned>((ip.N + 15) / 16), static_cast<unsigned>((ip.N + 15) / 16));
+    k_image<<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
+    SVDI_LAUNCH_CHECK();
+}
+
+}  // namespace svdi

@@ -1,0 +1,188 @@
+// Internal declarations of libsvdimager: the per-handle context, the device-side geometry and
+// plan records, and the stage launchers implemented in the .hip files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "svdi_math.hpp"
+#include "svdimager.h"
+
+namespace svdi {
+
+struct HipError : std::runtime_error {
+    hipError_t code;
+    HipError(hipError_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define SVDI_HIP(call)                                                                   \
+    do {                                                                                 \
+        const hipError_t e__ = (call);                                                   \
+        if (e__ != hipSuccess)                                                           \
+            throw ::svdi::HipError(e__, std::string(#call) + ": " + hipGetErrorString(e__)); \
+    } while (0)
+
+#define SVDI_LAUNCH_CHECK() SVDI_HIP(hipGetLastError())
+
+// Rotation of the snapshot, written on the device by the preprocess finalise kernel.
+struct DevGeometry {
+    double R[9];          // rows = principal axes, oriented (FP64)
+    float vin[9];         // the same rows in FP32 (the reference's Vin)
+    double evals[3];      // ascending
+    double mean[3];       // mean baseline in wavelengths
+    double fmin, fmax;    // channel frequency range, Hz
+    float det;            // |V00 V11 - V01 V10|
+};
+
+// Chebyshev / slab plan, filled on the device, read back by the host in one copy.
+struct DevPlan {
+    float r3_min, r3_max, max_abs_t3;
+    uint32_t valid_pixels;
+    float r3_centre, r3_half_range, alpha_max, target_alpha;
+    int32_t num_slabs;
+    float r3_range, slab_width, slab_half_range, inv_slab_half_range, slab_alpha_max;
+    uint32_t bucket_err_bits[4];     // integrated-imager buckets K = 8, 16, 24, 32
+    uint32_t term_err_bits[32];      // archive search, K = 1 .. 32
+    uint32_t vis_absmax_bits;        // max |vis| / |det| over gridded samples (for MFMA scaling)
+};
+
+enum class Semantics { Integrated = SVDI_SEMANTICS_INTEGRATED, Archive = SVDI_SEMANTICS_ARCHIVE };
+
+// Per-sample input of the imager: either the reference's preprocessed Bin / Vin / vis
+// (FIpipe and FIpipeDevice) or a raw snapshot that is rotated and collapsed on the fly.
+struct SampleInput {
+    bool from_snapshot = false;
+    size_t num_samples = 0;
+    // preprocessed form
+    const float* bin = nullptr;      // S x 3
+    const float* vis_re = nullptr;
+    const float* vis_im = nullptr;
+    const float* vin_dev = nullptr;  // 9 floats on the device
+    // snapshot form
+    size_t num_rows = 0, num_channels = 0;
+    const double* rot = nullptr;     // R x 3 rotated uvw (metres), device
+    const double* freq = nullptr;
+    const float *v0r = nullptr, *v0i = nullptr, *v3r = nullptr, *v3i = nullptr;
+    const uint8_t *f0 = nullptr, *f3 = nullptr;
+    const float *w0 = nullptr, *w3 = nullptr;
+    const DevGeometry* geo = nullptr;
+    // r3 range partials (min, max pairs) already reduced per block, or null to reduce Bin
+    const float* r3_partials = nullptr;
+    int r3_blocks = 0;
+};
+
+// Host copy of the plan plus the derived launch sizes.
+struct HostPlan {
+    DevPlan dev;
+    Semantics semantics;
+    int num_terms = 0;
+    float term_error = 0.0f;
+    size_t image_size = 0, grid_size = 0;
+    float cell = 0.0f;
+    size_t batch_slabs = 0;
+    // FFT windows: the image pass reads rows [row_lo, row_lo + rows) and columns
+    // [col_lo, col_lo + cols) of each transformed plane.
+    size_t row_lo = 0, rows = 0, col_lo = 0, cols = 0;
+};
+
+struct GridParams {
+    int G;
+    int K;
+    int slab_start, nslabs, total_slabs;
+    int archive;            // 1: one slab centred at r3_centre
+    float r3_min, slab_width, inv_half, r3_centre;
+    float scale_f32;        // cell * G in FP32 (preprocessed input)
+    double scale_f64;       // cell * G / c (snapshot input)
+    float inv_det;          // 1 / |det| (vis weighting)
+    float det;
+};
+
+// Grown-on-demand device buffers owned by a context.
+class Workspace {
+  public:
+    enum Slot {
+        kPartials, kPlan, kGeometry, kRot, kConv, kPlanes, kRowFft, kColFft, kTwiddles,
+        kBinCounts, kBinOffsets, kBinCursor, kRecPos, kRecVim, kItems, kMisc, kImage,
+        kInBin, kInVre, kInVim, kInVin, kR3Partials, kSlots
+    };
+    void* get(Slot s, size_t bytes);
+    void release();
+    size_t bytes_held() const;
+    ~Workspace() { release(); }
+
+  private:
+    void* ptr_[kSlots] = {};
+    size_t size_[kSlots] = {};
+};
+
+}  // namespace svdi
+
+struct svdi_context {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    svdi::Workspace ws;
+    svdi::DevPlan* pinned_plan = nullptr;   // page-locked mirror of the device plan
+    std::mutex mu;
+    size_t twiddle_n = 0;
+    int cu_count = 0;
+};
+
+namespace svdi {
+
+// preprocess.hip
+void launch_preprocess_geometry(svdi_context* ctx, const svdi_snapshot& snap, DevGeometry* geo,
+                                double* rot, float* r3_partials, int* r3_blocks);
+void launch_materialize(svdi_context* ctx, const svdi_snapshot& snap, const DevGeometry* geo,
+                        float* bin, float* vin, float* vis_re, float* vis_im);
+
+// plan.hip
+void launch_plan(svdi_context* ctx, const SampleInput& in, Semantics sem, size_t image_size,
+                 float cell, float target_alpha, DevPlan* plan);
+
+// grid.hip
+void launch_zero(svdi_context* ctx, void* p, size_t bytes);
+void launch_fill(svdi_context* ctx, float* p, size_t count, float value);
+void launch_grid_atomic(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                        float2* planes);
+size_t launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                         float2* planes, int tile);
+
+// fft.hip
+void fft_prepare(svdi_context* ctx, size_t G);
+void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size_t G,
+                         size_t row_lo, size_t rows, size_t col_lo, size_t cols,
+                         float2* row_out, float2* out);
+
+// image.hip
+void launch_conv_corr(svdi_context* ctx, size_t N, size_t G, int archive_variant, float* conv);
+struct ImageParams {
+    int N, G;
+    int P;                  // planes in this batch
+    int K;
+    int slab_start, nslabs;
+    int archive;
+    float cell;
+    float r3_min, slab_width, slab_half, r3_centre;
+    float inv_num_samples;
+    int rows, cols;         // window sizes of the transformed planes
+    int row_lo, col_lo;
+    int signed_out;
+};
+void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev,
+                  const float* conv, const ImageParams& ip, float* image);
+
+}  // namespace svdi

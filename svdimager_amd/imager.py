@@ -1,0 +1,171 @@
+"""Python front end of libsvdimager, mirroring the reference's host API.
+
+Reference entry points and their counterparts here:
+
+* ``preprocess_measurement_set_gpu`` (SVDIntegratedImager.cu:635-853) -> ``preprocess_measurement_set_gpu``
+* ``FIpipeDevice`` (SVDIntegratedImager.cu:1498-1761)               -> ``FIpipeDevice``
+* preprocess + FIpipeDevice as ``main`` chains them (:1822-1828)   -> ``image_snapshot``
+* ``FIpipe`` (archive/FIPkernels.cu:679-911, pipe_interface.cpp:8)  -> ``FIpipe`` (the exported C++ symbol)
+* ``write_fits_2d`` (SVDIntegratedImager.cu:153-176)                -> ``write_fits_2d``
+
+Device buffers are torch tensors on a ROCm device; torch is used for memory and streams only.
+Every call goes to the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import Context, PlanInfo, Snapshot, check, default_options, lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t) -> int:
+    return t.data_ptr()
+
+
+class DeviceSnapshot:
+    """A snapshot (the HostMeasurementSetData fields) resident in device memory."""
+
+    FIELDS = ("uvw", "freq", "vis0_re", "vis0_im", "vis3_re", "vis3_im", "flag0", "flag3",
+              "weight0", "weight3")
+
+    def __init__(self, snap: Dict, device: str = "cuda"):
+        torch = _torch()
+        dt = {"uvw": torch.float64, "freq": torch.float64, "flag0": torch.uint8,
+              "flag3": torch.uint8}
+        self.num_rows = int(snap["num_rows"])
+        self.num_channels = int(snap["num_channels"])
+        self.array_center_x = float(snap["array_center_x"])
+        self.tensors = {}
+        for k in self.FIELDS:
+            if snap.get(k) is None:
+                self.tensors[k] = None
+                continue
+            v = snap[k]
+            if isinstance(v, np.ndarray):
+                v = torch.from_numpy(np.ascontiguousarray(v))
+            self.tensors[k] = v.to(device=device, dtype=dt.get(k, torch.float32)).contiguous()
+
+    @property
+    def num_samples(self) -> int:
+        return self.num_rows * self.num_channels
+
+    def struct(self) -> Snapshot:
+        t = self.tensors
+        p = lambda k: (None if t[k] is None else _ptr(t[k]))
+        return Snapshot(self.num_rows, self.num_channels, p("uvw"), p("freq"), p("vis0_re"),
+                        p("vis0_im"), p("vis3_re"), p("vis3_im"), p("flag0"), p("flag3"),
+                        p("weight0"), p("weight3"), self.array_center_x)
+
+
+def _bind_stream(ctx: Context, device) -> None:
+    torch = _torch()
+    ctx.set_stream(torch.cuda.current_stream(device).cuda_stream)
+
+
+def preprocess_measurement_set_gpu(ctx: Context, dsnap: DeviceSnapshot) -> Dict:
+    """Bin (S, 3), Vin (3, 3), vis_re (S,), vis_im (S,) as device tensors."""
+    torch = _torch()
+    dev = dsnap.tensors["uvw"].device
+    S = dsnap.num_samples
+    out = {"bin": torch.empty((S, 3), dtype=torch.float32, device=dev),
+           "vin": torch.empty((3, 3), dtype=torch.float32, device=dev),
+           "vis_re": torch.empty(S, dtype=torch.float32, device=dev),
+           "vis_im": torch.empty(S, dtype=torch.float32, device=dev)}
+    _bind_stream(ctx, dev)
+    s = dsnap.struct()
+    check(lib().svdi_preprocess(ctx.handle, ctypes.byref(s), _ptr(out["bin"]), _ptr(out["vin"]),
+                                _ptr(out["vis_re"]), _ptr(out["vis_im"])))
+    return out
+
+
+def FIpipeDevice(ctx: Context, bin_, vin, vis_re, vis_im, image_size: int, cell_size: float,
+                 image=None, **options) -> Tuple[object, Dict]:
+    """Image preprocessed device buffers; returns (image tensor (N, N), plan dict)."""
+    torch = _torch()
+    dev = bin_.device
+    N = int(image_size)
+    if image is None:
+        image = torch.empty((N, N), dtype=torch.float32, device=dev)
+    opts = default_options(**options)
+    info = PlanInfo()
+    _bind_stream(ctx, dev)
+    check(lib().svdi_image_preprocessed(ctx.handle, _ptr(bin_), _ptr(vin), _ptr(vis_re),
+                                        _ptr(vis_im), bin_.shape[0], _ptr(image), N,
+                                        float(cell_size), ctypes.byref(opts),
+                                        ctypes.byref(info)))
+    return image, info.as_dict()
+
+
+def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_size: float,
+                   image=None, **options) -> Tuple[object, Dict]:
+    """Preprocess + image one device-resident snapshot; returns (image, plan dict)."""
+    torch = _torch()
+    dev = dsnap.tensors["uvw"].device
+    N = int(image_size)
+    if image is None:
+        image = torch.empty((N, N), dtype=torch.float32, device=dev)
+    opts = default_options(**options)
+    info = PlanInfo()
+    _bind_stream(ctx, dev)
+    s = dsnap.struct()
+    check(lib().svdi_image_snapshot(ctx.handle, ctypes.byref(s), _ptr(image), N,
+                                    float(cell_size), ctypes.byref(opts), ctypes.byref(info)))
+    return image, info.as_dict()
+
+
+def fipipe_host(ctx: Context, vis_re: np.ndarray, vis_im: np.ndarray, bin_: np.ndarray,
+                vin: np.ndarray, image_size: int, cell_size: float, **options):
+    """FIpipe semantics on host numpy arrays through a given context; returns (image, plan)."""
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    vr, vi, b, v = f(vis_re), f(vis_im), f(bin_).reshape(-1), f(vin).reshape(-1)
+    N = int(image_size)
+    out = np.zeros(N * N, np.float32)
+    opts = default_options(**({"semantics": _lib.SEMANTICS_ARCHIVE} | options))
+    info = PlanInfo()
+    check(lib().svdi_fipipe_host(ctx.handle, vr.ctypes.data, vi.ctypes.data, b.ctypes.data,
+                                 v.ctypes.data, out.ctypes.data, vr.size, N, float(cell_size),
+                                 ctypes.byref(opts), ctypes.byref(info)))
+    return out.reshape(N, N), info.as_dict()
+
+
+def FIpipe(Visreal: np.ndarray, Visimag: np.ndarray, Bin: np.ndarray, Vin: np.ndarray,
+           dirty_image: np.ndarray, num_baselines: int, image_size: int, cell_size: float) -> int:
+    """Call the exported C++ FIpipe symbol (archive/pipe_interface.cpp:8) on host arrays."""
+    arrs = [Visreal, Visimag, Bin, Vin, dirty_image]
+    for a in arrs:
+        if a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:
+            raise TypeError("FIpipe takes C-contiguous float32 arrays")
+    fn = getattr(lib(), "_Z6FIpipePfS_S_S_S_mmf")
+    return fn(*(a.ctypes.data for a in arrs), int(num_baselines), int(image_size),
+              float(cell_size))
+
+
+def write_fits_2d(filename: str, values: np.ndarray) -> None:
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    n1 = v.shape[-1]
+    n2 = v.size // n1
+    rc = lib().svdi_fits_write(filename.encode(), v.ctypes.data, n1, n2)
+    check(rc)
+
+
+def read_fits_image(filename: str) -> np.ndarray:
+    """Primary-HDU image as float32, shape (NAXIS2, NAXIS1) (1-D images: (NAXIS1,))."""
+    data = ctypes.c_void_p()
+    naxes = (ctypes.c_long * 2)()
+    check(lib().svdi_fits_read(filename.encode(), ctypes.byref(data), naxes))
+    n1, n2 = naxes[0], naxes[1]
+    try:
+        arr = np.ctypeslib.as_array(ctypes.cast(data, ctypes.POINTER(ctypes.c_float)),
+                                    shape=(n1 * n2,)).copy()
+    finally:
+        lib().svdi_free(data)
+    return arr if n2 == 1 else arr.reshape(n2, n1)

@@ -1,0 +1,160 @@
+"""GPU parity of the HIP imager against the oracle and the FP64 DFT (small snapshot).
+
+Tolerances (relative to the peak |DFT| of the signed, pre-reprojection image):
+* vs FP64 DFT: max <= 1e-4 over the whole image (the ES kernel's own aliasing error at the
+  image edge is ~1e-4 for support 8 at 1.5x oversampling), <= 2e-5 over the central half;
+* vs the oracle's FP32 restatement of the reference on the same Bin / Vin / vis: max <= 2e-5
+  (FP32 summation order, FP64 vs FP32 Bessel / w-phase in the image pass).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+from oracle import oracle as O
+import svdimager_amd as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _peak(d):
+    return float(np.nanmax(np.abs(d)))
+
+
+def _rel(a, b, peak):
+    return float(np.nanmax(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)))) / peak
+
+
+def _central(a, N):
+    q = N // 4
+    return a[q:N - q, q:N - q]
+
+
+@pytest.fixture(scope="module")
+def dsnap(small_snap):
+    return S.DeviceSnapshot(small_snap, "cuda")
+
+
+@pytest.fixture(scope="module")
+def oracle_pre(small_snap):
+    return O.preprocess_ref32(small_snap)
+
+
+def test_preprocess_matches_fp64_rotation(gpu_ctx, small_snap, small_dft, dsnap):
+    announce("preprocess small snapshot")
+    out = S.preprocess_measurement_set_gpu(gpu_ctx, dsnap)
+    vin = out["vin"].cpu().numpy()
+    assert np.abs(vin - small_dft["vin64"]).max() < 1e-6
+    p64 = O.preprocess_f64(small_snap)
+    b = out["bin"].cpu().numpy().astype(np.float64)
+    assert np.abs(b - p64["bin"]).max() <= 1e-6 * np.abs(p64["bin"]).max()
+    vre, vim = O.collapse_f64(small_snap)
+    assert np.abs(out["vis_re"].cpu().numpy() - vre).max() < 1e-6
+    assert np.abs(out["vis_im"].cpu().numpy() - vim).max() < 1e-6
+    R = vin.astype(np.float64)
+    assert np.abs(R @ R.T - np.eye(3)).max() < 1e-6
+    assert np.linalg.det(R) > 0.999
+
+
+@pytest.mark.parametrize("gridder", [S.GRIDDER_ATOMIC])
+def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gridder):
+    announce(f"image_snapshot signed small K=4 gridder={gridder}")
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    img, plan = S.image_snapshot(gpu_ctx, dsnap, N, cell, output=S.OUTPUT_SIGNED,
+                                 num_terms=small_snap["num_terms"], gridder=gridder)
+    img = img.cpu().numpy()
+    dft = small_dft["dft"]
+    peak = _peak(dft)
+    err = _rel(img, dft, peak)
+    err_c = _rel(_central(img, N), _central(dft, N), peak)
+    rms = float(np.sqrt(np.mean((img - dft) ** 2))) / peak
+    print(f"signed vs DFT: max {err:.3e} central {err_c:.3e} rms {rms:.3e} plan {plan}")
+    assert plan["num_terms"] == small_snap["num_terms"]
+    assert err <= 1e-4
+    assert err_c <= 2e-5
+
+
+def test_fipipedevice_matches_oracle_ref32(gpu_ctx, small_snap, oracle_pre):
+    announce("FIpipeDevice small vs oracle ref32 (bucket K)")
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
+         for k in ("bin", "vin", "vis_re", "vis_im")}
+    ref = O.fipipe_device_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                oracle_pre["vis_im"], N, cell)
+    for output, key in ((S.OUTPUT_SIGNED, "signed"), (S.OUTPUT_DIRTY, "dirty")):
+        img, plan = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N, cell,
+                                   output=output, gridder=S.GRIDDER_ATOMIC)
+        img = img.cpu().numpy()
+        peak = _peak(ref[key])
+        err = _rel(img, ref[key], peak)
+        print(f"{key}: max rel {err:.3e}")
+        assert plan["num_terms"] == ref["plan"]["num_terms"]
+        assert plan["num_slabs"] == ref["plan"]["num_slabs"]
+        assert abs(plan["r3_min"] - ref["plan"]["r3_min"]) == 0.0
+        assert abs(plan["r3_max"] - ref["plan"]["r3_max"]) == 0.0
+        assert plan["valid_pixels"] == ref["plan"]["valid_pixels"]
+        assert err <= 2e-5
+
+
+def test_fipipe_symbol_matches_archive_oracle(small_snap, oracle_pre):
+    announce("FIpipe C++ symbol small vs oracle archive")
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    S_ = oracle_pre["vis_re"].size
+    dirty = np.zeros(N * N, np.float32)
+    rc = S.FIpipe(oracle_pre["vis_re"].copy(), oracle_pre["vis_im"].copy(),
+                  np.ascontiguousarray(oracle_pre["bin"]).reshape(-1).copy(),
+                  np.ascontiguousarray(oracle_pre["vin"]).reshape(-1).copy(), dirty, S_, N, cell)
+    assert rc == 0
+    ref = O.fipipe_archive_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                 oracle_pre["vis_im"], N, cell)
+    err = _rel(dirty.reshape(N, N), ref["dirty"], _peak(ref["dirty"]))
+    print(f"archive dirty max rel {err:.3e}, K {ref['plan']['num_terms']}")
+    assert err <= 2e-5
+
+
+def test_multislab_and_batching_vs_oracle(gpu_ctx, small_snap, oracle_pre):
+    """Force several r3 slabs (small alpha target) and one slab per batch."""
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
+         for k in ("bin", "vin", "vis_re", "vis_im")}
+    for batch in (0, 1):
+        announce(f"multislab target_alpha=0.01 batch={batch}")
+        ref = O.fipipe_device_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                    oracle_pre["vis_im"], N, cell, target_alpha=0.01,
+                                    max_batch_slabs=(None if batch == 0 else 1))
+        assert ref["plan"]["num_slabs"] > 1
+        for output, key in ((S.OUTPUT_SIGNED, "signed"), (S.OUTPUT_DIRTY, "dirty")):
+            img, plan = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N,
+                                       cell, output=output, target_alpha=0.01,
+                                       max_batch_slabs=batch, gridder=S.GRIDDER_ATOMIC)
+            assert plan["num_slabs"] == ref["plan"]["num_slabs"]
+            assert plan["batch_slabs"] == ref["plan"]["batch_slabs"]
+            err = _rel(img.cpu().numpy(), ref[key], _peak(ref[key]))
+            print(f"batch={batch} {key}: max rel {err:.3e} slabs {plan['num_slabs']}")
+            assert err <= 2e-5
+
+
+@pytest.mark.parametrize("N", [6, 10, 33])
+def test_odd_grid_sizes_vs_oracle(gpu_ctx, small_snap, oracle_pre, N):
+    """Image sizes whose grid G = ceil(1.5 N) is odd or whose N is odd."""
+    announce(f"odd sizes N={N}")
+    cell = small_snap["cell"] * 128 / N
+    t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
+         for k in ("bin", "vin", "vis_re", "vis_im")}
+    ref = O.fipipe_device_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                oracle_pre["vis_im"], N, cell, num_terms=4)
+    img, _ = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N, cell,
+                            output=S.OUTPUT_SIGNED, num_terms=4, gridder=S.GRIDDER_ATOMIC)
+    err = _rel(img.cpu().numpy(), ref["signed"], _peak(ref["signed"]))
+    print(f"N={N} G={(3 * N + 1) // 2} signed max rel {err:.3e}")
+    assert err <= 2e-5
+
+
+def test_invalid_arguments(gpu_ctx, small_snap, dsnap):
+    announce("invalid arguments")
+    with pytest.raises(S.SvdiError):
+        S.image_snapshot(gpu_ctx, dsnap, 0, 1e-4)
+    with pytest.raises(S.SvdiError):
+        S.image_snapshot(gpu_ctx, dsnap, 64, -1.0)
+    with pytest.raises(S.SvdiError):
+        S.image_snapshot(gpu_ctx, dsnap, 64, 1e-4, num_terms=33)
