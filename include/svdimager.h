@@ -114,6 +114,14 @@ int svdi_set_stream(svdi_context* ctx, void* hip_stream);
 /* Release the workspace (it regrows on the next call). */
 int svdi_trim(svdi_context* ctx);
 
+/* Stage timing with HIP events on the context stream (no counterpart in the reference, which
+ * times only the whole slab loop, SVDIntegratedImager.cu:1657-1741).  When enabled, each call
+ * records events around its stages; svdi_last_stage_times returns the device milliseconds of
+ * the last call per stage: [0] preprocess, [1] plan, [2] grid, [3] fft, [4] image. */
+#define SVDI_NUM_STAGES 5
+int svdi_set_profiling(svdi_context* ctx, int enable);
+int svdi_last_stage_times(svdi_context* ctx, float* ms, int count);
+
 /* preprocess_measurement_set_gpu (SVDIntegratedImager.cu:635-853): device snapshot ->
  * Bin (S x 3, rotated, uncentred, wavelengths), Vin (3 x 3 row-major), collapsed vis (S each).
  * Output buffers are caller-allocated device memory.  The rotation is computed in FP64. */

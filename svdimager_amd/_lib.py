@@ -26,8 +26,9 @@ EXPORTED_SYMBOLS = [
     "svdi_version", "svdi_last_error", "svdi_default_options", "svdi_create", "svdi_destroy",
     "svdi_set_stream", "svdi_trim", "svdi_preprocess", "svdi_image_preprocessed",
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
-    "_Z6FIpipePfS_S_S_S_mmf",
+    "svdi_set_profiling", "svdi_last_stage_times", "_Z6FIpipePfS_S_S_S_mmf",
 ]
+STAGES = ("preprocess", "plan", "grid", "fft", "image")
 
 
 class SvdiError(RuntimeError):
@@ -109,6 +110,9 @@ def lib():
             "svdi_fits_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
                                               ctypes.POINTER(ctypes.c_long)]),
             "svdi_free": (None, [vp]),
+            "svdi_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+            "svdi_last_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
+                                                     ctypes.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -152,6 +156,15 @@ class Context:
 
     def trim(self) -> None:
         check(lib().svdi_trim(self._h))
+
+    def set_profiling(self, enable: bool) -> None:
+        check(lib().svdi_set_profiling(self._h, 1 if enable else 0))
+
+    def stage_times(self) -> dict:
+        """Device milliseconds per stage of the last call (profiling must be enabled)."""
+        arr = (ctypes.c_float * len(STAGES))()
+        check(lib().svdi_last_stage_times(self._h, arr, len(STAGES)))
+        return {name: float(arr[i]) for i, name in enumerate(STAGES)}
 
     def close(self) -> None:
         if self._h:

@@ -124,6 +124,21 @@ void check_device_pointer(const void* p, const char* what) {
     if (p == nullptr) throw ArgError(std::string(what) + " is null");
 }
 
+void begin_call(svdi_context* ctx) {
+    ctx->events_used = 0;
+    ctx->stage_marks.clear();
+    for (int i = 0; i < SVDI_NUM_STAGES; ++i) ctx->stage_ms[i] = 0.0f;
+}
+
+// After the stream has been synchronised: sum the recorded stage intervals.
+void finish_call(svdi_context* ctx) {
+    for (const auto& m : ctx->stage_marks) {
+        float ms = 0.0f;
+        SVDI_HIP(hipEventElapsedTime(&ms, ctx->event_pool[m.second], ctx->event_pool[m.second + 1]));
+        ctx->stage_ms[m.first] += ms;
+    }
+}
+
 // The imaging core shared by every entry point.  `in` describes the samples; the image
 // (N x N floats, device) is overwritten.
 void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
@@ -143,7 +158,9 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
 
     DevPlan* dplan = static_cast<DevPlan*>(ctx->ws.get(Workspace::kPlan, sizeof(DevPlan)));
     SVDI_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
+    size_t tok = stage_begin(ctx, 1);
     launch_plan(ctx, in, hp.semantics, N, cell, target_alpha, dplan);
+    stage_end(ctx, tok);
     SVDI_HIP(hipMemcpyAsync(ctx->pinned_plan, dplan, sizeof(DevPlan), hipMemcpyDeviceToHost, ctx->stream));
     SVDI_HIP(hipStreamSynchronize(ctx->stream));
     hp.dev = *ctx->pinned_plan;
@@ -252,20 +269,27 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
         gp.nslabs = static_cast<int>(nb);
+        tok = stage_begin(ctx, 2);
         if (gridder == SVDI_GRIDDER_TILED) {
             records += launch_grid_tiled(ctx, in, gp, planes, 32);
         } else {
             launch_grid_atomic(ctx, in, gp, planes);
         }
+        stage_end(ctx, tok);
         const size_t Pb = nb * static_cast<size_t>(K);
+        tok = stage_begin(ctx, 3);
         launch_fft2_inverse(ctx, planes, Pb, G, hp.row_lo, hp.rows, hp.col_lo, hp.cols, rowfft, colfft);
+        stage_end(ctx, tok);
         ip.P = static_cast<int>(Pb);
         ip.slab_start = static_cast<int>(start);
         ip.nslabs = static_cast<int>(nb);
+        tok = stage_begin(ctx, 4);
         launch_image(ctx, colfft, vin_dev, conv, ip, image);
+        stage_end(ctx, tok);
     }
     SVDI_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
     SVDI_HIP(hipEventSynchronize(ctx->ev_stop));
+    finish_call(ctx);
     float ms = 0.0f;
     SVDI_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
     if (o.verbose) std::cout << "Time elapsed: " << ms << " ms" << std::endl;
@@ -321,7 +345,9 @@ SampleInput snapshot_input(svdi_context* ctx, const svdi_snapshot& s) {
     const size_t r3_blocks = (s.num_rows + 255) / 256;
     float* r3p = static_cast<float*>(ctx->ws.get(Workspace::kR3Partials, r3_blocks * 2 * sizeof(float)));
     int nb = 0;
+    const size_t tok = stage_begin(ctx, 0);
     launch_preprocess_geometry(ctx, s, geo, rot, r3p, &nb);
+    stage_end(ctx, tok);
     SampleInput in;
     in.from_snapshot = true;
     in.num_rows = s.num_rows;
@@ -344,6 +370,25 @@ SampleInput snapshot_input(svdi_context* ctx, const svdi_snapshot& s) {
 }
 
 }  // namespace
+
+size_t stage_begin(svdi_context* ctx, int stage) {
+    if (!ctx->profiling) return SIZE_MAX;
+    while (ctx->event_pool.size() < ctx->events_used + 2) {
+        hipEvent_t e;
+        SVDI_HIP(hipEventCreate(&e));
+        ctx->event_pool.push_back(e);
+    }
+    const size_t idx = ctx->events_used;
+    ctx->events_used += 2;
+    SVDI_HIP(hipEventRecord(ctx->event_pool[idx], ctx->stream));
+    ctx->stage_marks.emplace_back(stage, idx);
+    return idx;
+}
+
+void stage_end(svdi_context* ctx, size_t token) {
+    if (token == SIZE_MAX) return;
+    SVDI_HIP(hipEventRecord(ctx->event_pool[token + 1], ctx->stream));
+}
 
 // ----------------------------------------------------------------------------- Workspace
 void* Workspace::get(Slot s, size_t bytes) {
@@ -427,6 +472,7 @@ int svdi_destroy(svdi_context* ctx) {
         if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
         if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
         delete ctx;
     });
 }
@@ -460,9 +506,11 @@ int svdi_preprocess(svdi_context* ctx, const svdi_snapshot* snap, float* bin, fl
         check_device_pointer(vis_im, "vis_im");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
+        begin_call(ctx);
         SampleInput in = snapshot_input(ctx, *snap);
         launch_materialize(ctx, *snap, in.geo, bin, vin, vis_re, vis_im);
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        finish_call(ctx);
     });
 }
 
@@ -479,6 +527,7 @@ int svdi_image_preprocessed(svdi_context* ctx, const float* bin, const float* vi
         check_device_pointer(image, "image");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
+        begin_call(ctx);
         SampleInput in;
         in.num_samples = num_samples;
         in.bin = bin;
@@ -497,6 +546,7 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snap, float* ima
         check_device_pointer(image, "image");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
+        begin_call(ctx);
         SampleInput in = snapshot_input(ctx, *snap);
         image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_INTEGRATED), info, image);
     });
@@ -513,6 +563,7 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
         if (image_size == 0) throw ArgError("Image_Size must be greater than zero");
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
+        begin_call(ctx);
         const size_t S = num_baselines;
         float* d_bin = static_cast<float*>(ctx->ws.get(Workspace::kInBin, S * 3 * sizeof(float)));
         float* d_vre = static_cast<float*>(ctx->ws.get(Workspace::kInVre, S * sizeof(float)));
@@ -533,6 +584,22 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
         SVDI_HIP(hipMemcpyAsync(dirty_image, d_img, image_size * image_size * sizeof(float),
                                 hipMemcpyDeviceToHost, ctx->stream));
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int svdi_set_profiling(svdi_context* ctx, int enable) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->profiling = enable != 0;
+    });
+}
+
+int svdi_last_stage_times(svdi_context* ctx, float* ms, int count) {
+    return guarded([&] {
+        if (!ctx || !ms) throw ArgError("null argument");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (int i = 0; i < count; ++i) ms[i] = i < SVDI_NUM_STAGES ? ctx->stage_ms[i] : 0.0f;
     });
 }
 

@@ -114,7 +114,8 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     }
     const float value = fabsf(ps) * ip.inv_num_samples;
     float oi0, oi1;
-    sin_reproject(ip.N, ip.cell, V, x, y, &oi0, &oi1);
+    if (ip.archive) sin_reproject_archive(ip.N, ip.cell, V, x, y, &oi0, &oi1);
+    else sin_reproject(ip.N, ip.cell, V, x, y, &oi0, &oi1);
     const float LL = oi0 - half;
     const float MM = oi1 - half;
     if (fabsf(LL) < half - 1 && fabsf(MM) < half - 1) {

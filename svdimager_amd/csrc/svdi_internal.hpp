@@ -139,7 +139,19 @@ struct svdi_context {
     std::mutex mu;
     size_t twiddle_n = 0;
     int cu_count = 0;
+    // stage timing: pairs of events recorded around stages of the current call
+    bool profiling = false;
+    std::vector<hipEvent_t> event_pool;
+    size_t events_used = 0;
+    std::vector<std::pair<int, size_t>> stage_marks;   // (stage, index of start event)
+    float stage_ms[SVDI_NUM_STAGES] = {};
 };
+
+namespace svdi {
+// Records a start event for `stage` now and returns a token; stage_end records its end.
+size_t stage_begin(svdi_context* ctx, int stage);
+void stage_end(svdi_context* ctx, size_t token);
+}  // namespace svdi
 
 namespace svdi {
 

@@ -258,4 +258,112 @@ SVDI_HD void sin_reproject(long di, float dc_rad, const float* V, long idx, long
     }
 }
 
+// Degree-based trig helpers of the archive imager (archive/FIPkernels.cu:50-166), FP32: exact
+// values at multiples of 90 degrees, otherwise the float functions of the angle in radians.
+SVDI_HD long floor_toward(float v) {
+    return v < 0 ? -static_cast<long>(ceilf(-v)) : static_cast<long>(floorf(v));
+}
+SVDI_HD void sincosd(float angle, float& s, float& c) {
+    if (fmodf(angle, 90.0f) == 0) {
+        const int i = static_cast<int>(fabsf(static_cast<float>(floor_toward(angle / 90.0f + 0.5f)))) % 4;
+        switch (i) {
+            case 0: s = 0.0f; c = 1.0f; break;
+            case 1: s = (angle > 0.0f) ? 1.0f : -1.0f; c = 0.0f; break;
+            case 2: s = 0.0f; c = -1.0f; break;
+            default: s = (angle > 0.0f) ? -1.0f : 1.0f; c = 0.0f; break;
+        }
+    } else {
+        s = sinf(static_cast<float>(angle * kPi / 180.0f));
+        c = cosf(static_cast<float>(angle * kPi / 180.0f));
+    }
+}
+SVDI_HD float sind(float angle) {
+    if (fmodf(angle, 90.0f) == 0) {
+        const int i = static_cast<int>(fabsf(static_cast<float>(floor_toward(angle / 90.0f - 0.5f)))) % 4;
+        return i == 0 ? 1.0f : (i == 2 ? -1.0f : 0.0f);
+    }
+    return sinf(static_cast<float>(angle * kPi / 180.0f));
+}
+SVDI_HD float cosd(float angle) {
+    if (fmodf(angle, 90.0f) == 0) {
+        const int i = static_cast<int>(fabsf(static_cast<float>(floor_toward(angle / 90.0f + 0.5f)))) % 4;
+        return i == 0 ? 1.0f : (i == 2 ? -1.0f : 0.0f);
+    }
+    return cosf(static_cast<float>(angle * kPi / 180.0f));
+}
+SVDI_HD float atan2d(float y, float x) {
+    if (y == 0.0f) return (x >= 0.0f) ? 0.0f : 180.0f;
+    if (x == 0.0f) return (y > 0.0f) ? 90.0f : -90.0f;
+    return static_cast<float>(atan2f(y, x) * 180.0f / kPi);
+}
+SVDI_HD float acosd(float v) {
+    if (v >= 1.0f && v - 1.0f < 1e-10) return 0.0f;
+    if (v == 0.0f) return 90.0f;
+    if (v <= -1.0f && v + 1.0f > -1e-10) return 180.0f;
+    return static_cast<float>(acosf(v) * 180.0f / kPi);
+}
+SVDI_HD float asind(float v) {
+    if (v <= -1.0f && v + 1.0f > -1e-10) return -90.0f;
+    if (v == 0.0f) return 0.0f;
+    if (v >= 1.0f && v - 1.0f < 1e-10) return 90.0f;
+    return static_cast<float>(asinf(v) * 180.0f / kPi);
+}
+
+// The archive's two-step form of the same re-projection: coordschange then p2p
+// (archive/FIPkernels.cu:537-640), FP32 with degree-based trig.
+SVDI_HD void sin_reproject_archive(long di, float dc_in, const float* V, long idx, long idy,
+                                   float* out0, float* out1) {
+    const float half = static_cast<float>(di / 2);
+    const float p1 = (-V[0] * (static_cast<float>(idx) - half) + V[3] * (static_cast<float>(idy) - half)) /
+                         fabsf(V[8]) + half;
+    const float p2 = (-V[1] * (static_cast<float>(idx) - half) + V[4] * (static_cast<float>(idy) - half)) /
+                         fabsf(V[8]) + half;
+    *out0 = p1;
+    *out1 = p2;
+    const float xi = V[6] / V[8];
+    const float eta = V[7] / V[8];
+    const float dc = static_cast<float>(dc_in / kPi * 180);
+    float x = -dc * (p1 - (static_cast<float>(di) / 2.0f + 1.0f));
+    float y = dc * (p2 - (static_cast<float>(di) / 2.0f + 1.0f));
+    float r0 = static_cast<float>(180.0f / kPi);
+    const float x0 = x / r0;
+    const float y0 = y / r0;
+    const float r2 = x0 * x0 + y0 * y0;
+    const float phi = (r2 != 0.0f) ? atan2d(x0, -y0) : 0.0f;
+    float theta;
+    if (r2 < 0.5f) {
+        theta = acosd(sqrtf(r2));
+    } else if (r2 <= 1.0f) {
+        theta = asind(sqrtf(1.0f - r2));
+    } else {
+        return;
+    }
+    float sinphi, cosphi;
+    sincosd(phi, sinphi, cosphi);
+    x = sinphi;
+    y = cosphi;
+    const float t = static_cast<float>((90.0f - fabsf(theta)) / 180.0f * kPi);
+    float z, costhe;
+    if (t < 1.0e-5f) {
+        z = (theta > 0.0f) ? t * t / 2.0f : 2.0f - t * t / 2.0f;
+        costhe = t;
+    } else {
+        z = 1.0f - sind(theta);
+        costhe = cosd(theta);
+    }
+    r0 = static_cast<float>(180.0f / kPi);
+    const float r = r0 * costhe;
+    const float w = xi * xi + eta * eta;
+    if (w == 0.0f) {
+        x = r * x;
+        y = -r * y;
+    } else {
+        z = z * r0;
+        x = r * x + xi * z;
+        y = -r * y + eta * z;
+    }
+    *out0 = -1.0f / dc * x + static_cast<float>(di) / 2.0f + 1.0f;
+    *out1 = 1.0f / dc * y + static_cast<float>(di) / 2.0f + 1.0f;
+}
+
 }  // namespace svdi
