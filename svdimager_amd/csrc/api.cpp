@@ -263,15 +263,16 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     ip.signed_out = o.output == SVDI_OUTPUT_SIGNED;
     const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
 
-    size_t records = 0;
-    const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_ATOMIC : o.gridder;
+    const uint32_t* tile_totals = nullptr;
+    const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_TILED : o.gridder;
+    if (gridder != SVDI_GRIDDER_ATOMIC && gridder != SVDI_GRIDDER_TILED) throw ArgError("unknown gridder");
     for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
         gp.nslabs = static_cast<int>(nb);
         tok = stage_begin(ctx, 2);
         if (gridder == SVDI_GRIDDER_TILED) {
-            records += launch_grid_tiled(ctx, in, gp, planes, 32);
+            tile_totals = launch_grid_tiled(ctx, in, gp, planes);
         } else {
             launch_grid_atomic(ctx, in, gp, planes);
         }
@@ -292,6 +293,12 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     finish_call(ctx);
     float ms = 0.0f;
     SVDI_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+    size_t records = 0;
+    if (tile_totals != nullptr) {
+        uint32_t tot[2] = {0, 0};
+        SVDI_HIP(hipMemcpy(tot, tile_totals, sizeof tot, hipMemcpyDeviceToHost));
+        records = tot[0];                                  // last slab batch
+    }
     if (o.verbose) std::cout << "Time elapsed: " << ms << " ms" << std::endl;
 
     if (info) {

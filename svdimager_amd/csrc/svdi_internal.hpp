@@ -170,8 +170,9 @@ void launch_zero(svdi_context* ctx, void* p, size_t bytes);
 void launch_fill(svdi_context* ctx, float* p, size_t count, float value);
 void launch_grid_atomic(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                         float2* planes);
-size_t launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
-                         float2* planes, int tile);
+// tiles.hip: returns a device pointer to {records, items} of the launch.
+const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                                  float2* planes);
 
 // fft.hip
 void fft_prepare(svdi_context* ctx, size_t G);
