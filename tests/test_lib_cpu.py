@@ -1,0 +1,186 @@
+"""CPU checks of libsvdimager: it loads, exports every declared symbol, its host-only entry
+points (FITS I/O) work, device entry points fail cleanly without a GPU, and the scalar device
+math (svdi_math.hpp, compiled here for the host) matches scipy / numpy / the oracle."""
+import ctypes
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+from scipy import special
+
+from conftest import ROOT
+from oracle import oracle as O
+from svdimager_amd import _lib
+
+HEADER = os.path.join(ROOT, "include", "svdimager.h")
+CSRC = os.path.join(ROOT, "svdimager_amd", "csrc")
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(_lib.LIB_PATH):
+        from svdimager_amd import build
+        build.build()
+    return _lib.lib()
+
+
+def test_exports_every_declared_symbol(L):
+    text = open(HEADER).read()
+    declared = set(re.findall(r"\b(svdi_[a-z_0-9]+)\s*\(", text))
+    assert declared == set(s for s in _lib.EXPORTED_SYMBOLS if s.startswith("svdi_"))
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines())
+    for sym in _lib.EXPORTED_SYMBOLS:
+        assert sym in exported, sym
+    assert L.svdi_version() == 1
+
+
+def test_fipipe_has_the_reference_cxx_linkage(L):
+    # int FIpipe(float*, float*, float*, float*, float*, size_t, size_t, float) with C++
+    # linkage (archive/pipe_interface.cpp:8) mangles to this name on x86-64.
+    assert hasattr(L, "_Z6FIpipePfS_S_S_S_mmf")
+
+
+def test_create_without_gpu_returns_an_error(L):
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = L.svdi_create(ctypes.byref(h), 0)
+    assert rc != 0 and not h
+    assert L.svdi_last_error()
+
+
+def test_fits_round_trip(tmp_path):
+    from svdimager_amd import read_fits_image, write_fits_2d
+    img = np.random.default_rng(1).normal(size=(12, 7)).astype(np.float32)
+    p = str(tmp_path / "a.fits")
+    write_fits_2d(p, img)
+    raw = open(p, "rb").read()
+    assert len(raw) % 2880 == 0
+    header = raw[:2880].decode("ascii")
+    cards = [header[i:i + 80] for i in range(0, 2880, 80)]
+    assert cards[0].startswith("SIMPLE  =                    T")
+    assert cards[1].startswith("BITPIX  =                  -32")
+    assert cards[3].startswith("NAXIS1  =                    7")
+    assert cards[4].startswith("NAXIS2  =                   12")
+    data = np.frombuffer(raw[2880:2880 + img.size * 4], dtype=">f4").reshape(12, 7)
+    assert np.array_equal(data, img)
+    assert np.array_equal(read_fits_image(p), img)
+    write_fits_2d(p, img[:3])                 # overwrite, as cfitsio's "!" prefix does
+    assert read_fits_image(p).shape == (3, 7)
+
+
+def _fits_bytes(cards, payload):
+    h = "".join(c.ljust(80) for c in cards + ["END"])
+    h = h.ljust((len(h) + 2879) // 2880 * 2880)
+    pad = (len(payload) + 2879) // 2880 * 2880 - len(payload)
+    return h.encode("ascii") + payload + b"\0" * pad
+
+
+def test_fits_reads_fp64_bin_and_scaled_int(tmp_path):
+    # A float64 S x 3 array as astropy writes Bin (archive/SVD_MFS.py:108), and a 1-D
+    # 16-bit image with BSCALE / BZERO.
+    b = np.random.default_rng(2).normal(size=(5, 3))
+    p = tmp_path / "bin.fits"
+    p.write_bytes(_fits_bytes(["SIMPLE  =                    T", "BITPIX  =                  -64",
+                               "NAXIS   =                    2", "NAXIS1  =                    3",
+                               "NAXIS2  =                    5"], b.astype(">f8").tobytes()))
+    from svdimager_amd import read_fits_image
+    assert np.allclose(read_fits_image(str(p)), b.astype(np.float32))
+    q = tmp_path / "v.fits"
+    vals = np.array([1, -2, 300, 7], ">i2")
+    q.write_bytes(_fits_bytes(["SIMPLE  =                    T", "BITPIX  =                   16",
+                               "NAXIS   =                    1", "NAXIS1  =                    4",
+                               "BSCALE  =                  0.5", "BZERO   =                 10.0"],
+                              vals.tobytes()))
+    assert np.allclose(read_fits_image(str(q)), 10.0 + 0.5 * vals.astype(np.float32))
+
+
+def test_cli_usage_exit_code():
+    cli = os.path.join(ROOT, "svdimager_amd", "bin", "sharedlibrary_gpu")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 2 and "Usage" in r.stderr
+    r = subprocess.run([cli, "/nonexistent.fits", "a", "b", "c", "16", "1", "1e-3", "o.fits"],
+                       capture_output=True, text=True)
+    assert r.returncode == 1
+
+
+PROBE = r"""
+#include "svdi_math.hpp"
+extern "C" {
+void p_bessel(double x, int n, double* j) { svdi::bessel_sequence_f64(x, n, j); }
+void p_eigen(const double* a, double* ev, double* vec) { svdi::eigen_sym3(a, ev, vec); }
+void p_basis(const double* vec, double acx, double* basis) { svdi::oriented_basis(vec, acx, basis); }
+float p_es(float x) { return svdi::es_kernel(x); }
+void p_reproject(long di, float cell, const float* V, long x, long y, int archive, float* o) {
+    if (archive) svdi::sin_reproject_archive(di, cell, V, x, y, o, o + 1);
+    else svdi::sin_reproject(di, cell, V, x, y, o, o + 1);
+}
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    d = tmp_path_factory.mktemp("probe")
+    src = d / "probe.cpp"
+    src.write_text(PROBE)
+    so = d / "probe.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", CSRC, str(src), "-o",
+                    str(so)], check=True)
+    P = ctypes.CDLL(str(so))
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    P.p_bessel.argtypes = [ctypes.c_double, ctypes.c_int, dp]
+    P.p_eigen.argtypes = [dp, dp, dp]
+    P.p_basis.argtypes = [dp, ctypes.c_double, dp]
+    P.p_es.argtypes = [ctypes.c_float]
+    P.p_es.restype = ctypes.c_float
+    P.p_reproject.argtypes = [ctypes.c_long, ctypes.c_float, fp, ctypes.c_long, ctypes.c_long,
+                              ctypes.c_int, fp]
+    return P
+
+
+@pytest.mark.parametrize("x", [0.0, 1e-9, 0.3, 0.999, 1.0, 4.2, 10.0, 37.5, -6.5, 999.0, 1000.0, 2.5e4])
+def test_device_bessel_matches_scipy(probe, x):
+    j = np.zeros(32)
+    probe.p_bessel(x, 32, j)
+    ref = special.jv(np.arange(32), x)
+    assert np.abs(j - ref).max() < 1e-12 + 1e-10 * np.abs(ref).max()
+
+
+def test_device_eigen_and_basis_match_oracle(probe):
+    rng = np.random.default_rng(4)
+    for _ in range(10):
+        a = rng.normal(size=(3, 3))
+        a = a @ a.T
+        ev, vec = np.zeros(3), np.zeros(9)
+        probe.p_eigen(np.ascontiguousarray(a.reshape(-1)), ev, vec)
+        ev_np = np.linalg.eigh(a)[0]
+        assert np.allclose(ev, ev_np, rtol=1e-12)
+        basis = np.zeros(9)
+        probe.p_basis(vec, -2565018.0, basis)
+        R = basis.reshape(3, 3)
+        assert np.abs(R @ R.T - np.eye(3)).max() < 1e-12 and np.linalg.det(R) > 0
+        assert R[0, 0] > 0 and R[1, 1] * -2565018.0 >= 0
+
+
+def test_device_es_and_reprojection_match_oracle(probe):
+    for x in np.linspace(-1.1, 1.1, 45, dtype=np.float32):
+        assert probe.p_es(float(x)) == O.es_f32(float(x))
+    V = O.preprocess_f64(__import__("conftest").load_small_snapshot())["vin"].astype(np.float32)
+    N, cell = 48, 3e-3
+    for archive in (0, 1):
+        m = O.reproject(N, cell, V, archive)
+        o = np.zeros(2, np.float32)
+        for y in range(0, N, 5):
+            for x in range(0, N, 3):
+                probe.p_reproject(N, cell, np.ascontiguousarray(V.reshape(-1)), x, y, archive, o)
+                assert np.array_equal(o, m[y, x]), (archive, x, y)
