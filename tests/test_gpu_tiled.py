@@ -1,7 +1,8 @@
 """GPU parity of the tiled gridder against the reference-style atomic gridder and the oracle.
 
 Both gridders grid the same sample model; they differ only in FP32 summation order, so the
-signed images agree to <= 2e-5 of peak.  Cases: the small snapshot (snapshot and preprocessed
+signed images agree to 1e-5 of peak over the central half and 1e-4 over the image (the
+grid correction amplifies FP32 rounding up to ~85x at the image edge).  Cases: the small snapshot (snapshot and preprocessed
 inputs), several slabs, exact-integer positions (nine-cell footprints), one hot position that
 forces split work items, footprints clipped at the grid edge, every supported K range.
 """
@@ -9,13 +10,13 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import announce
+from conftest import announce, assert_parity
 from oracle import oracle as O
 import svdimager_amd as S
 
 pytestmark = pytest.mark.gpu
 
-TOL = 2e-5
+
 
 
 def _rel(a, b):
@@ -47,9 +48,7 @@ def test_snapshot_tiled_vs_atomic(gpu_ctx, small_snap):
         img, plan = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, num_terms=4, gridder=g)
         out[g] = img.cpu().numpy()
     assert plan["records"] > 0
-    err = _rel(out[S.GRIDDER_TILED], out[S.GRIDDER_ATOMIC])
-    print(f"tiled vs atomic {err:.2e} records {plan['records']}")
-    assert err <= TOL
+    assert_parity(out[S.GRIDDER_TILED], out[S.GRIDDER_ATOMIC], f"tiled vs atomic records {plan['records']}")
 
 
 @pytest.mark.parametrize("K", [1, 4, 8, 12, 16, 24, 32])
@@ -60,9 +59,7 @@ def test_preprocessed_tiled_vs_oracle(gpu_ctx, small_snap, small_pre, K):
                                 small_pre["vis_im"], N, cell, num_terms=K)
     img, plan = _image(gpu_ctx, _dev(small_pre), N, cell, output=S.OUTPUT_SIGNED, num_terms=K,
                        gridder=S.GRIDDER_TILED)
-    err = _rel(img, ref["signed"])
-    print(f"K={K} tiled vs oracle {err:.2e}")
-    assert err <= TOL
+    assert_parity(img, ref["signed"], f"K={K} tiled vs oracle")
 
 
 def test_tiled_multislab_batches(gpu_ctx, small_snap, small_pre):
@@ -77,9 +74,7 @@ def test_tiled_multislab_batches(gpu_ctx, small_snap, small_pre):
             img, plan = _image(gpu_ctx, t, N, cell, output=output, target_alpha=0.005,
                                max_batch_slabs=batch, gridder=S.GRIDDER_TILED)
             assert plan["num_slabs"] == ref["plan"]["num_slabs"] > 2
-            err = _rel(img, ref[key])
-            print(f"batch={batch} {key} {err:.2e} slabs {plan['num_slabs']}")
-            assert err <= TOL
+            assert_parity(img, ref[key], f"batch={batch} {key} slabs {plan['num_slabs']}")
 
 
 def _synthetic_bin(n, rng, scale, spread):
@@ -113,9 +108,7 @@ def test_integer_positions_nine_cell_footprints(gpu_ctx):
     t = _dev(a)
     for g in (S.GRIDDER_ATOMIC, S.GRIDDER_TILED):
         img, _ = _image(gpu_ctx, t, N, cell, output=S.OUTPUT_SIGNED, num_terms=8, gridder=g)
-        err = _rel(img, ref["signed"])
-        print(f"gridder {g}: {err:.2e}")
-        assert err <= TOL
+        assert_parity(img, ref["signed"], f"gridder {g}:")
 
 
 def test_hot_position_splits_items(gpu_ctx):
@@ -135,9 +128,7 @@ def test_hot_position_splits_items(gpu_ctx):
     ref = O.fipipe_device_ref32(a["bin"], a["vin"], a["vis_re"], a["vis_im"], N, cell, num_terms=8)
     img, plan = _image(gpu_ctx, _dev(a), N, cell, output=S.OUTPUT_SIGNED, num_terms=8,
                        gridder=S.GRIDDER_TILED)
-    err = _rel(img, ref["signed"])
-    print(f"hot/edge: {err:.2e} records {plan['records']}")
-    assert err <= TOL
+    assert_parity(img, ref["signed"], f"hot/edge: records {plan['records']}")
 
 
 def test_archive_tiled_vs_oracle(gpu_ctx, small_snap, small_pre):
@@ -148,6 +139,4 @@ def test_archive_tiled_vs_oracle(gpu_ctx, small_snap, small_pre):
     img, plan = _image(gpu_ctx, _dev(small_pre), N, cell, output=S.OUTPUT_SIGNED,
                        semantics=S.SEMANTICS_ARCHIVE, gridder=S.GRIDDER_TILED)
     assert plan["num_terms"] == ref["plan"]["num_terms"]
-    err = _rel(img, ref["signed"])
-    print(f"archive tiled {err:.2e}")
-    assert err <= TOL
+    assert_parity(img, ref["signed"], f"archive tiled")
