@@ -5,14 +5,15 @@
 // with a device atomic; here:
 //   1. count:   each sample's clipped footprint is assigned to every tile it overlaps (a sample
 //               straddling a tile edge is duplicated, so tiles own their cells exclusively);
-//   2. scan:    per-bin record offsets and work items of at most kChunk records (hot tiles in
-//               the dense uv core are split into many items);
+//   2. scan:    per-bin record offsets and work items of similar size (hot tiles in the dense
+//               uv core are split into several items);
 //   3. scatter: per-(sample, tile) records written contiguously per bin;
-//   4. grid:    one workgroup per item loads its records into LDS, sorts them by exact footprint
-//               position, and lets each wavefront accumulate a run of same-position records in
-//               registers (lane = footprint cell) before one LDS add per cell and term; the tile
-//               is then stored to the planes with its checkerboard sign (split tiles add
-//               atomically into a zeroed tile instead).
+//   4. grid:    a persistent workgroup per CU takes items from a queue, loads their records
+//               into LDS in sub-chunks, sorts them by exact footprint position, and lets each
+//               wavefront accumulate a run of same-position records in registers (lane =
+//               footprint cell) before one LDS add per cell and term; the tile is then stored
+//               to the planes with its checkerboard sign (split tiles add atomically into a
+//               zeroed tile instead).
 // Bins are (local slab, tile); the planes need no clearing except split tiles.
 #include "sample.hpp"
 
@@ -22,7 +23,8 @@ namespace {
 constexpr int kBinThreads = 256;
 constexpr int kBinPerThread = 64;        // samples per thread, preprocessed input
 constexpr int kLdsBins = 16384;          // LDS histogram capacity of the binning passes
-constexpr int kChunk = 1024;             // records per work item
+constexpr int kSub = 1024;               // records staged in LDS at a time
+constexpr int kMinItem = 4 * kSub;       // smallest cap on the records of one work item
 constexpr int kTileThreads = 512;
 constexpr int kUnit = 64;                // records per accumulation unit (one wavefront)
 
@@ -90,45 +92,60 @@ k_bin_count(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ c
     }
 }
 
-// One block: exclusive scan of the bin counts into record offsets, and the work items
-// (at least one per bin so every tile is written, kChunk records at most).
-// totals[0] = records, totals[1] = items, totals[2] = queue head of the tile kernel.
+// One block: exclusive scan of the bin counts into record offsets, then the work items.  An
+// item holds at most `cap` records, cap = max(kMinItem, total / (4 x CUs)) rounded to kSub, so
+// the dense uv-core tiles split into a few items of similar size (each item adds its tile
+// back once) and every bin has at least one item (every tile is written).
+// totals[0] = records, [1] = items, [2] = queue head of the tile kernel, [3] = cap.
 __global__ void __launch_bounds__(1024)
 k_bin_scan(const uint32_t* __restrict__ counts, int nbins, uint32_t* __restrict__ cursor,
-           Item* __restrict__ items, int max_items, uint32_t* __restrict__ totals) {
-    __shared__ uint32_t s_rec[1024], s_it[1024];
+           Item* __restrict__ items, int max_items, int cu_count, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_a[1024];
+    __shared__ uint32_t s_cap;
     const int tid = threadIdx.x;
     const int per = (nbins + 1023) / 1024;
     const int lo = min(tid * per, nbins), hi = min(lo + per, nbins);
-    uint32_t rec = 0, it = 0;
-    for (int b = lo; b < hi; ++b) {
-        const uint32_t c = counts[b];
-        rec += c;
-        it += max(1u, (c + kChunk - 1) / kChunk);
+    auto block_scan = [&](uint32_t v) {            // inclusive scan of v over the block
+        s_a[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint32_t t = tid >= off ? s_a[tid - off] : 0u;
+            __syncthreads();
+            s_a[tid] += t;
+            __syncthreads();
+        }
+        const uint32_t r = s_a[tid];
+        __syncthreads();
+        return r;
+    };
+    uint32_t rec = 0;
+    for (int b = lo; b < hi; ++b) rec += counts[b];
+    const uint32_t rec_incl = block_scan(rec);
+    if (tid == 1023) {
+        const uint32_t total = rec_incl;
+        const uint32_t fair = (total + 4u * cu_count - 1) / (4u * cu_count);
+        uint32_t cap = max(static_cast<uint32_t>(kMinItem), fair);
+        s_cap = (cap + kSub - 1) / kSub * kSub;
+        totals[0] = total;
+        totals[3] = s_cap;
     }
-    s_rec[tid] = rec;
-    s_it[tid] = it;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t vr = tid >= off ? s_rec[tid - off] : 0u;
-        const uint32_t vi = tid >= off ? s_it[tid - off] : 0u;
-        __syncthreads();
-        s_rec[tid] += vr;
-        s_it[tid] += vi;
-        __syncthreads();
-    }
-    uint32_t base_rec = s_rec[tid] - rec, base_it = s_it[tid] - it;
+    const uint32_t cap = s_cap;
+    uint32_t it = 0;
+    for (int b = lo; b < hi; ++b) it += max(1u, (counts[b] + cap - 1) / cap);
+    const uint32_t it_incl = block_scan(it);
+    uint32_t base_rec = rec_incl - rec, base_it = it_incl - it;
     for (int b = lo; b < hi; ++b) {
         const uint32_t c = counts[b];
         cursor[b] = base_rec;
-        const uint32_t n = max(1u, (c + kChunk - 1) / kChunk);
+        const uint32_t n = max(1u, (c + cap - 1) / cap);
         for (uint32_t q = 0; q < n; ++q) {
             const uint32_t slot = base_it + q;
             if (slot < static_cast<uint32_t>(max_items)) {
                 Item t;
                 t.bin = b;
-                t.start = static_cast<int>(base_rec + q * kChunk);
-                t.count = static_cast<int>(min(static_cast<uint32_t>(kChunk), c - q * kChunk));
+                t.start = static_cast<int>(base_rec + q * cap);
+                t.count = static_cast<int>(min(cap, c - q * cap));
                 t.split = n > 1;
                 items[slot] = t;
             }
@@ -136,10 +153,7 @@ k_bin_scan(const uint32_t* __restrict__ counts, int nbins, uint32_t* __restrict_
         base_rec += c;
         base_it += n;
     }
-    if (tid == 1023) {
-        totals[0] = s_rec[1023];
-        totals[1] = min(s_it[1023], static_cast<uint32_t>(max_items));
-    }
+    if (tid == 1023) totals[1] = min(it_incl, static_cast<uint32_t>(max_items));
 }
 
 template <bool kSnap>
@@ -193,9 +207,10 @@ k_bin_scatter(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__
 
 // Clear the tiles that are accumulated by several items (their items add atomically).
 __global__ void __launch_bounds__(256)
-k_zero_split(const uint32_t* __restrict__ counts, TileGeom tg, int G, int K, float2* __restrict__ planes) {
+k_zero_split(const uint32_t* __restrict__ counts, const uint32_t* __restrict__ totals, TileGeom tg,
+             int G, int K, float2* __restrict__ planes) {
     const int bin = blockIdx.x;
-    if (counts[bin] <= static_cast<uint32_t>(kChunk)) return;
+    if (counts[bin] <= totals[3]) return;
     const int ls = bin / tg.ntiles, tile = bin - ls * tg.ntiles;
     const int ta = tile / tg.ntx, tb = tile - ta * tg.ntx;
     const size_t plane_size = static_cast<size_t>(G) * G;
@@ -216,225 +231,254 @@ __device__ __forceinline__ int swz(int a, int b) {
     else return b;
 }
 
+__device__ __forceinline__ int readlane_i(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// LDS layout of the tile kernel.
+template <int T, int KMAX>
+struct TileLds {
+    static constexpr int KS = T + 8;               // footprint starts in [-8, T) per axis
+    static constexpr int NKEYS = KS * KS;
+    static constexpr int MAXUNITS = NKEYS + kSub / kUnit;
+    static constexpr size_t acc = 0;                                        // float [2][KMAX][T*T]
+    static constexpr size_t srec = acc + sizeof(float) * 2 * KMAX * T * T;  // RecA [kSub]
+    static constexpr size_t svis = srec + sizeof(RecA) * kSub;             // float2 [kSub]
+    static constexpr size_t khist = svis + sizeof(float2) * kSub;          // u32 [NKEYS]
+    static constexpr size_t koff = khist + sizeof(uint32_t) * NKEYS;
+    static constexpr size_t uoff = koff + sizeof(uint32_t) * NKEYS;
+    static constexpr size_t units = uoff + sizeof(uint32_t) * NKEYS;        // u32 [MAXUNITS]
+    static constexpr size_t order = units + sizeof(uint32_t) * MAXUNITS;    // u16 [kSub]
+    static constexpr size_t bytes = order + sizeof(uint16_t) * kSub;
+};
+
+// One workgroup per CU, persistent: items come from a queue.  Per item the T x T x K tile
+// accumulator lives in LDS; the item's records are taken kSub at a time, sorted by exact
+// footprint position into units of at most 64 same-position records, and each unit is
+// accumulated by one wavefront: lane = footprint cell, the unit's records held one per lane
+// and broadcast with readlane, 2K FP32 accumulators per lane, one LDS add per cell and term
+// at the end of the unit.
 template <bool kSnap, int T, int KMAX>
 __global__ void __launch_bounds__(kTileThreads)
 k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
             const RecA* __restrict__ recs, const float2* __restrict__ vis, GridParams gp,
             TileGeom tg, float2* __restrict__ planes) {
-    constexpr int KS = T + 8;                        // footprint starts a0 in [-8, T)
-    constexpr int NKEYS = KS * KS;
-    extern __shared__ float lds[];
-    float* acc = lds;                                // [2][KMAX][T*T]
-    RecA* srec = reinterpret_cast<RecA*>(acc + 2 * KMAX * T * T);
-    float2* svis = reinterpret_cast<float2*>(srec + kChunk);
-    uint32_t* khist = reinterpret_cast<uint32_t*>(svis + kChunk);   // counts, then cursors
-    uint32_t* koff = khist + NKEYS;
-    uint32_t* uoff = koff + NKEYS;
-    uint32_t* units = uoff + NKEYS;                  // NKEYS + kChunk / kUnit
-    uint16_t* order = reinterpret_cast<uint16_t*>(units + NKEYS + kChunk / kUnit);
+    using L = TileLds<T, KMAX>;
+    constexpr int KS = L::KS, NKEYS = L::NKEYS;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float* acc_re = reinterpret_cast<float*>(lds_raw + L::acc);
+    float* acc_im = acc_re + KMAX * T * T;
+    RecA* srec = reinterpret_cast<RecA*>(lds_raw + L::srec);
+    float2* svis = reinterpret_cast<float2*>(lds_raw + L::svis);
+    uint32_t* khist = reinterpret_cast<uint32_t*>(lds_raw + L::khist);
+    uint32_t* koff = reinterpret_cast<uint32_t*>(lds_raw + L::koff);
+    uint32_t* uoff = reinterpret_cast<uint32_t*>(lds_raw + L::uoff);
+    uint32_t* units = reinterpret_cast<uint32_t*>(lds_raw + L::units);
+    uint16_t* order = reinterpret_cast<uint16_t*>(lds_raw + L::order);
     __shared__ uint32_t s_nunits;
     __shared__ int s_item;
 
-    // Persistent workgroup: take items from the queue until it is exhausted.
-    for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(&totals[2], 1u));
-    __syncthreads();
-    if (s_item >= static_cast<int>(totals[1])) break;
-    const Item it = items[s_item];
     const int K = gp.K;
-    const int ls = it.bin / tg.ntiles, tile = it.bin - ls * tg.ntiles;
-    const int ta = tile / tg.ntx, tb = tile - ta * tg.ntx;
-    const int off = gp.G / 2;
-    const int oac = ta * T - off, obc = tb * T - off;     // tile origin, centred coordinates
-
-    for (int e = threadIdx.x; e < 2 * KMAX * T * T; e += kTileThreads) acc[e] = 0.0f;
-    for (int e = threadIdx.x; e < NKEYS; e += kTileThreads) khist[e] = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < it.count; i += kTileThreads) {
-        const RecA r = recs[it.start + i];
-        srec[i] = r;
-        svis[i] = vis[it.start + i];
-        const int key = (r.ab & 63) * KS + ((r.ab >> 6) & 63);
-        atomicAdd(&khist[key], 1u);
-    }
-    __syncthreads();
-    // Exclusive scans of record counts and unit counts per key (one wavefront per pass).
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        uint32_t run_r = 0, run_u = 0;
-        for (int base = 0; base < NKEYS; base += 64) {
-            const int k = base + lane;
-            const uint32_t c = k < NKEYS ? khist[k] : 0u;
-            const uint32_t u = (c + kUnit - 1) / kUnit;
-            uint32_t sr = c, su = u;
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t tr = __shfl_up(sr, d, 64), tu = __shfl_up(su, d, 64);
-                if (lane >= d) {
-                    sr += tr;
-                    su += tu;
-                }
-            }
-            if (k < NKEYS) {
-                koff[k] = run_r + sr - c;
-                uoff[k] = run_u + su - u;
-            }
-            run_r += __shfl(sr, 63, 64);
-            run_u += __shfl(su, 63, 64);
-        }
-        if (lane == 0) s_nunits = run_u;
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < NKEYS; k += kTileThreads) {
-        const uint32_t c = khist[k];
-        khist[k] = koff[k];
-        for (uint32_t q = 0; q * kUnit < c; ++q)
-            units[uoff[k] + q] = (koff[k] + q * kUnit) | (min(static_cast<uint32_t>(kUnit), c - q * kUnit) << 11) |
-                                 (static_cast<uint32_t>(k) << 18);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < it.count; i += kTileThreads) {
-        const int ab = srec[i].ab;
-        const int key = (ab & 63) * KS + ((ab >> 6) & 63);
-        order[atomicAdd(&khist[key], 1u)] = static_cast<uint16_t>(i);
-    }
-    __syncthreads();
-
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ci = lane >> 3, cj = lane & 7;
-    float* acc_re = acc;
-    float* acc_im = acc + KMAX * T * T;
-    for (uint32_t u = wave; u < s_nunits; u += kTileThreads / 64) {
-        const uint32_t w = units[u];
-        const int start = w & 2047, cnt = (w >> 11) & 127, key = w >> 18;
-        const int a0l = key / KS - 8, b0l = key % KS - 8;
-        float re[KMAX], im[KMAX];
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) re[k] = im[k] = 0.0f;
-        for (int r = 0; r < cnt; ++r) {
-            const int ri = order[start + r];
-            const RecA rec = srec[ri];
-            const float2 v = svis[ri];
-            const int na = (rec.ab >> 12) & 15, nb = (rec.ab >> 16) & 15;
-            float xa, xb;
-            if constexpr (kSnap) {
-                xa = (static_cast<float>(ci) - rec.q1) * 0.25f;
-                xb = (static_cast<float>(cj) - rec.q2) * 0.25f;
-            } else {
-                xa = (static_cast<float>(oac + a0l + ci) - rec.q1) * 0.25f;
-                xb = (static_cast<float>(obc + b0l + cj) - rec.q2) * 0.25f;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(&totals[2], 1u));
+        __syncthreads();
+        if (s_item >= static_cast<int>(totals[1])) break;
+        const Item it = items[s_item];
+        const int ls = it.bin / tg.ntiles, tile = it.bin - ls * tg.ntiles;
+        const int ta = tile / tg.ntx, tb = tile - ta * tg.ntx;
+        const int off = gp.G / 2;
+        const int oac = ta * T - off, obc = tb * T - off;    // tile origin, centred coordinates
+        for (int e = threadIdx.x; e < 2 * KMAX * T * T; e += kTileThreads) acc_re[e] = 0.0f;
+
+        for (int sub = 0; sub < it.count; sub += kSub) {
+            const int n = min(kSub, it.count - sub);
+            for (int e = threadIdx.x; e < NKEYS; e += kTileThreads) khist[e] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += kTileThreads) {
+                const RecA r = recs[it.start + sub + i];
+                srec[i] = r;
+                svis[i] = vis[it.start + sub + i];
+                atomicAdd(&khist[(r.ab & 63) * KS + ((r.ab >> 6) & 63)], 1u);
             }
-            const float kv = (ci < na ? es_kernel(xa) : 0.0f) * (cj < nb ? es_kernel(xb) : 0.0f);
-            float tkm2 = 1.0f, tkm1 = rec.s;
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (k < K) {
-                    float tk;
-                    if (k == 0) tk = 1.0f;
-                    else if (k == 1) tk = rec.s;
-                    else {
-                        tk = 2.0f * rec.s * tkm1 - tkm2;
-                        tkm2 = tkm1;
-                        tkm1 = tk;
+            __syncthreads();
+            // Exclusive scans of records and units per key, one wavefront.
+            if (threadIdx.x < 64) {
+                uint32_t run_r = 0, run_u = 0;
+                for (int base = 0; base < NKEYS; base += 64) {
+                    const int k = base + lane;
+                    const uint32_t c = k < NKEYS ? khist[k] : 0u;
+                    const uint32_t u = (c + kUnit - 1) / kUnit;
+                    uint32_t sr = c, su = u;
+                    for (int d = 1; d < 64; d <<= 1) {
+                        const uint32_t tr = __shfl_up(sr, d, 64), tu = __shfl_up(su, d, 64);
+                        if (lane >= d) {
+                            sr += tr;
+                            su += tu;
+                        }
                     }
-                    const float m = tk * kv;
-                    re[k] += v.x * m;
-                    im[k] += v.y * m;
+                    if (k < NKEYS) {
+                        koff[k] = run_r + sr - c;
+                        uoff[k] = run_u + su - u;
+                    }
+                    run_r += __shfl(sr, 63, 64);
+                    run_u += __shfl(su, 63, 64);
                 }
+                if (lane == 0) s_nunits = run_u;
             }
-            if (na == 9 || nb == 9) {
-                // Ninth row / column of a footprint whose position is an exact integer:
-                // lanes 0..8 take row 8 (columns 0..8), lanes 9..16 take column 8 (rows 0..7).
-                int ei = -1, ej = -1;
-                if (lane < 9 && na == 9) {
-                    ei = 8;
-                    ej = lane;
-                } else if (lane >= 9 && lane < 17 && nb == 9) {
-                    ei = lane - 9;
-                    ej = 8;
+            __syncthreads();
+            for (int k = threadIdx.x; k < NKEYS; k += kTileThreads) {
+                const uint32_t c = khist[k];
+                khist[k] = koff[k];
+                for (uint32_t q = 0; q * kUnit < c; ++q)
+                    units[uoff[k] + q] = (koff[k] + q * kUnit) |
+                                         (min(static_cast<uint32_t>(kUnit), c - q * kUnit) << 11) |
+                                         (static_cast<uint32_t>(k) << 18);
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n; i += kTileThreads) {
+                const int ab = srec[i].ab;
+                order[atomicAdd(&khist[(ab & 63) * KS + ((ab >> 6) & 63)], 1u)] = static_cast<uint16_t>(i);
+            }
+            __syncthreads();
+
+            for (uint32_t u = wave; u < s_nunits; u += kTileThreads / 64) {
+                const uint32_t w = units[u];
+                const int start = w & 2047, cnt = (w >> 11) & 127, key = w >> 18;
+                const int a0l = key / KS - 8, b0l = key % KS - 8;
+                // stage: lane l holds record l of the unit
+                RecA mine = {0, 0.f, 0.f, 0.f};
+                float2 mv = make_float2(0.f, 0.f);
+                if (lane < cnt) {
+                    const int ri = order[start + lane];
+                    mine = srec[ri];
+                    mv = svis[ri];
                 }
-                if (ei >= 0 && ej < nb && ei < na) {
-                    float ya, yb;
+                float re[KMAX], im[KMAX];
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) re[k] = im[k] = 0.0f;
+                for (int r = 0; r < cnt; ++r) {
+                    const int ab = readlane_i(mine.ab, r);
+                    const float q1 = readlane_f(mine.q1, r), q2 = readlane_f(mine.q2, r);
+                    const float s = readlane_f(mine.s, r);
+                    const float vx = readlane_f(mv.x, r), vy = readlane_f(mv.y, r);
+                    const int na = (ab >> 12) & 15, nb = (ab >> 16) & 15;
+                    float xa, xb;
                     if constexpr (kSnap) {
-                        ya = (static_cast<float>(ei) - rec.q1) * 0.25f;
-                        yb = (static_cast<float>(ej) - rec.q2) * 0.25f;
+                        xa = (static_cast<float>(ci) - q1) * 0.25f;
+                        xb = (static_cast<float>(cj) - q2) * 0.25f;
                     } else {
-                        ya = (static_cast<float>(oac + a0l + ei) - rec.q1) * 0.25f;
-                        yb = (static_cast<float>(obc + b0l + ej) - rec.q2) * 0.25f;
+                        xa = (static_cast<float>(oac + a0l + ci) - q1) * 0.25f;
+                        xb = (static_cast<float>(obc + b0l + cj) - q2) * 0.25f;
                     }
-                    const int ca = a0l + ei, cb = b0l + ej;
-                    if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
-                        const float ekv = es_kernel(ya) * es_kernel(yb);
-                        float t2 = 1.0f, t1 = rec.s;
-                        for (int k = 0; k < K; ++k) {
+                    const float kv = (ci < na ? es_kernel(xa) : 0.0f) * (cj < nb ? es_kernel(xb) : 0.0f);
+                    float tkm2 = 1.0f, tkm1 = s;
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) {
+                        if (k < K) {
                             float tk;
                             if (k == 0) tk = 1.0f;
-                            else if (k == 1) tk = rec.s;
+                            else if (k == 1) tk = s;
                             else {
-                                tk = 2.0f * rec.s * t1 - t2;
-                                t2 = t1;
-                                t1 = tk;
+                                tk = 2.0f * s * tkm1 - tkm2;
+                                tkm2 = tkm1;
+                                tkm1 = tk;
                             }
-                            const float m = tk * ekv;
-                            const int idx = k * T * T + ca * T + swz<T>(ca, cb);
-                            atomicAdd(&acc_re[idx], v.x * m);
-                            atomicAdd(&acc_im[idx], v.y * m);
+                            const float m = tk * kv;
+                            re[k] += vx * m;
+                            im[k] += vy * m;
+                        }
+                    }
+                    if (na == 9 || nb == 9) {
+                        // Ninth row / column of a footprint at an exact integer position:
+                        // lanes 0..8 take row 8 (columns 0..8), lanes 9..16 column 8 (rows 0..7).
+                        int ei = -1, ej = -1;
+                        if (lane < 9 && na == 9) {
+                            ei = 8;
+                            ej = lane;
+                        } else if (lane >= 9 && lane < 17 && nb == 9) {
+                            ei = lane - 9;
+                            ej = 8;
+                        }
+                        if (ei >= 0 && ej < nb && ei < na) {
+                            float ya, yb;
+                            if constexpr (kSnap) {
+                                ya = (static_cast<float>(ei) - q1) * 0.25f;
+                                yb = (static_cast<float>(ej) - q2) * 0.25f;
+                            } else {
+                                ya = (static_cast<float>(oac + a0l + ei) - q1) * 0.25f;
+                                yb = (static_cast<float>(obc + b0l + ej) - q2) * 0.25f;
+                            }
+                            const int ca = a0l + ei, cb = b0l + ej;
+                            if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
+                                const float ekv = es_kernel(ya) * es_kernel(yb);
+                                float t2 = 1.0f, t1 = s;
+                                for (int k = 0; k < K; ++k) {
+                                    float tk;
+                                    if (k == 0) tk = 1.0f;
+                                    else if (k == 1) tk = s;
+                                    else {
+                                        tk = 2.0f * s * t1 - t2;
+                                        t2 = t1;
+                                        t1 = tk;
+                                    }
+                                    const float m = tk * ekv;
+                                    const int idx = k * T * T + ca * T + swz<T>(ca, cb);
+                                    atomicAdd(&acc_re[idx], vx * m);
+                                    atomicAdd(&acc_im[idx], vy * m);
+                                }
+                            }
+                        }
+                    }
+                }
+                const int ca = a0l + ci, cb = b0l + cj;
+                if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
+                    const int cell = ca * T + swz<T>(ca, cb);
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k) {
+                        if (k < K) {
+                            atomicAdd(&acc_re[k * T * T + cell], re[k]);
+                            atomicAdd(&acc_im[k * T * T + cell], im[k]);
                         }
                     }
                 }
             }
+            __syncthreads();
         }
-        const int ca = a0l + ci, cb = b0l + cj;
-        if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
-            const int cell = ca * T + swz<T>(ca, cb);
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (k < K) {
-                    atomicAdd(&acc_re[k * T * T + cell], re[k]);
-                    atomicAdd(&acc_im[k * T * T + cell], im[k]);
-                }
+
+        const int G = gp.G;
+        const size_t plane_size = static_cast<size_t>(G) * G;
+        for (int e = threadIdx.x; e < K * T * T; e += kTileThreads) {
+            const int k = e / (T * T), c = e - k * T * T;
+            const int a = c / T, b = c - a * T;
+            const int ag = ta * T + a, bg = tb * T + b;
+            if (ag >= G || bg >= G) continue;
+            const int lc = k * T * T + a * T + swz<T>(a, b);
+            float2 val = make_float2(acc_re[lc], acc_im[lc]);
+            if ((ag + bg) & 1) {
+                val.x = -val.x;
+                val.y = -val.y;
+            }
+            float2* dst = planes + (static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(ag) * G + bg;
+            if (it.split) {
+                unsafeAtomicAdd(&dst->x, val.x);
+                unsafeAtomicAdd(&dst->y, val.y);
+            } else {
+                *dst = val;
             }
         }
     }
-    __syncthreads();
-
-    const int G = gp.G;
-    const size_t plane_size = static_cast<size_t>(G) * G;
-    for (int e = threadIdx.x; e < K * T * T; e += kTileThreads) {
-        const int k = e / (T * T), c = e - k * T * T;
-        const int a = c / T, b = c - a * T;
-        const int ag = ta * T + a, bg = tb * T + b;
-        if (ag >= G || bg >= G) continue;
-        const int lc = k * T * T + a * T + swz<T>(a, b);
-        float2 val = make_float2(acc_re[lc], acc_im[lc]);
-        if ((ag + bg) & 1) {
-            val.x = -val.x;
-            val.y = -val.y;
-        }
-        float2* dst = planes + (static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(ag) * G + bg;
-        if (it.split) {
-            unsafeAtomicAdd(&dst->x, val.x);
-            unsafeAtomicAdd(&dst->y, val.y);
-        } else {
-            *dst = val;
-        }
-    }
-    }   // item loop
-}
-
-template <int T, int KMAX>
-constexpr size_t tile_lds_bytes() {
-    constexpr int KS = T + 8, NKEYS = KS * KS;
-    return sizeof(float) * 2 * KMAX * T * T + (sizeof(RecA) + sizeof(float2)) * kChunk +
-           sizeof(uint32_t) * (3 * NKEYS + NKEYS + kChunk / kUnit) + sizeof(uint16_t) * kChunk;
 }
 
 template <bool kSnap, int T, int KMAX>
 void launch_tile_kernel(svdi_context* ctx, unsigned grid, const Item* items, uint32_t* totals,
                         const RecA* recs, const float2* vis, const GridParams& gp, const TileGeom& tg,
                         float2* planes) {
-    constexpr size_t bytes = tile_lds_bytes<T, KMAX>();
-    static_assert(bytes <= 160 * 1024, "tile LDS exceeds 160 KB");
+    constexpr size_t bytes = TileLds<T, KMAX>::bytes;
+    static_assert(bytes <= 150 * 1024, "tile LDS exceeds the CU's LDS");
     SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     k_tile_grid<kSnap, T, KMAX><<<grid, kTileThreads, bytes, ctx->stream>>>(items, totals, recs, vis, gp, tg, planes);
@@ -453,19 +497,17 @@ void launch_tile_dispatch(svdi_context* ctx, unsigned grid, const Item* items, u
 
 }  // namespace
 
-int tile_side_for(int K) { return K <= 8 ? 32 : 16; }
-
 const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                                   float2* planes) {
     TileGeom tg;
-    tg.T = tile_side_for(gp.K);
+    tg.T = gp.K <= 8 ? 32 : 16;
     tg.ntx = (gp.G + tg.T - 1) / tg.T;
     tg.ntiles = tg.ntx * tg.ntx;
     tg.nbins = tg.ntiles * gp.nslabs;
     tg.use_lds = tg.nbins <= kLdsBins;
     const size_t S = in.num_samples;
-    const size_t max_records = 4 * S;
-    const size_t max_items = static_cast<size_t>(tg.nbins) + (max_records + kChunk - 1) / kChunk;
+    const size_t max_records = 4 * S;                      // a footprint covers at most 2 x 2 tiles
+    const size_t max_items = static_cast<size_t>(tg.nbins) + (max_records + kMinItem - 1) / kMinItem;
     if (max_records >= (1ull << 31)) throw UnsupportedError("too many samples for the tiled gridder");
 
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (tg.nbins + 4) * sizeof(uint32_t)));
@@ -483,9 +525,10 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
     if (in.from_snapshot) k_bin_count<true><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, counts);
     else k_bin_count<false><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, counts);
     SVDI_LAUNCH_CHECK();
-    k_bin_scan<<<1, 1024, 0, ctx->stream>>>(counts, tg.nbins, cursor, items, static_cast<int>(max_items), totals);
+    const int cus = std::max(1, ctx->cu_count);
+    k_bin_scan<<<1, 1024, 0, ctx->stream>>>(counts, tg.nbins, cursor, items, static_cast<int>(max_items), cus, totals);
     SVDI_LAUNCH_CHECK();
-    k_zero_split<<<tg.nbins, 256, 0, ctx->stream>>>(counts, tg, gp.G, gp.K, planes);
+    k_zero_split<<<tg.nbins, 256, 0, ctx->stream>>>(counts, totals, tg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     if (in.from_snapshot)
         k_bin_scatter<true><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, recs, vis);
@@ -493,7 +536,7 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
         k_bin_scatter<false><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, recs, vis);
     SVDI_LAUNCH_CHECK();
     // One persistent workgroup per CU (the tile kernel's LDS admits one per CU).
-    const unsigned grid = static_cast<unsigned>(std::min<size_t>(max_items, std::max(1, ctx->cu_count)));
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>(max_items, static_cast<size_t>(cus)));
     if (in.from_snapshot) launch_tile_dispatch<true>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
     else launch_tile_dispatch<false>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
     return totals;

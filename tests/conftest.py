@@ -61,6 +61,11 @@ def parity_errors(img, ref):
     return float(d.max()) / peak, float(d[q0:n0 - q0, q1:n1 - q1].max()) / peak
 
 
+# Dirty images (|.| then bilinear splat): the splat targets come from FP32 hypot / division
+# on the device vs glibc on the host, a few ulp apart, which moves each splat weight by ~1e-5.
+DIRTY_CENTRAL = 5e-5
+
+
 def assert_parity(img, ref, what="", whole=1e-4, central=1e-5):
     e, c = parity_errors(img, ref)
     print(f"{what}: max {e:.2e} central {c:.2e}", flush=True)

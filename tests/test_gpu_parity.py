@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import announce, assert_parity
+from conftest import DIRTY_CENTRAL, announce, assert_parity
 from oracle import oracle as O
 import svdimager_amd as S
 
@@ -92,7 +92,7 @@ def test_fipipedevice_matches_oracle_ref32(gpu_ctx, small_snap, oracle_pre):
         assert abs(plan["r3_min"] - ref["plan"]["r3_min"]) == 0.0
         assert abs(plan["r3_max"] - ref["plan"]["r3_max"]) == 0.0
         assert plan["valid_pixels"] == ref["plan"]["valid_pixels"]
-        assert_parity(img, ref[key], key)
+        assert_parity(img, ref[key], key, central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
 
 
 def test_fipipe_symbol_matches_archive_oracle(small_snap, oracle_pre):
@@ -136,7 +136,8 @@ def test_multislab_and_batching_vs_oracle(gpu_ctx, small_snap, oracle_pre):
                                        max_batch_slabs=batch, gridder=S.GRIDDER_ATOMIC)
             assert plan["num_slabs"] == ref["plan"]["num_slabs"]
             assert plan["batch_slabs"] == ref["plan"]["batch_slabs"]
-            assert_parity(img.cpu().numpy(), ref[key], f"batch={batch} {key}")
+            assert_parity(img.cpu().numpy(), ref[key], f"batch={batch} {key}",
+                          central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
 
 
 @pytest.mark.parametrize("N", [6, 10, 33])

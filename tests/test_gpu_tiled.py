@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import announce, assert_parity
+from conftest import DIRTY_CENTRAL, announce, assert_parity
 from oracle import oracle as O
 import svdimager_amd as S
 
@@ -74,7 +74,8 @@ def test_tiled_multislab_batches(gpu_ctx, small_snap, small_pre):
             img, plan = _image(gpu_ctx, t, N, cell, output=output, target_alpha=0.005,
                                max_batch_slabs=batch, gridder=S.GRIDDER_TILED)
             assert plan["num_slabs"] == ref["plan"]["num_slabs"] > 2
-            assert_parity(img, ref[key], f"batch={batch} {key} slabs {plan['num_slabs']}")
+            assert_parity(img, ref[key], f"batch={batch} {key} slabs {plan['num_slabs']}",
+                          central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
 
 
 def _synthetic_bin(n, rng, scale, spread):
