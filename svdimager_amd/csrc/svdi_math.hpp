@@ -40,6 +40,20 @@ SVDI_HD float es_kernel(float x) {
     return (xx > 1.0f) ? 0.0f : expf(kBeta * (sqrtf(1.0f - xx) - 1.0f));
 }
 
+// The same kernel with the hardware square root and base-2 exponential on the device (about
+// 1 ulp each, no denormal fix-ups: ES >= e^-beta > 1e-7 wherever it is non-zero).
+SVDI_HD float es_kernel_fast(float x) {
+    const float xx = x * x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_sqrtf(1.0f - xx);
+    const float e = __builtin_amdgcn_exp2f((kBeta * 1.4426950408889634f) * (r - 1.0f));
+#else
+    const float r = sqrtf(1.0f - xx);
+    const float e = exp2f((kBeta * 1.4426950408889634f) * (r - 1.0f));
+#endif
+    return (xx > 1.0f) ? 0.0f : e;
+}
+
 SVDI_HD double es_kernel_f64(double x) {
     const double xx = x * x;
     return (xx > 1.0) ? 0.0 : exp(15.3704324328 * (sqrt(1.0 - xx) - 1.0));

@@ -8,12 +8,11 @@
 //   2. scan:    per-bin record offsets and work items of similar size (hot tiles in the dense
 //               uv core are split into several items);
 //   3. scatter: per-(sample, tile) records written contiguously per bin;
-//   4. grid:    a persistent workgroup per CU takes items from a queue, loads their records
-//               into LDS in sub-chunks, sorts them by exact footprint position, and lets each
-//               wavefront accumulate a run of same-position records in registers (lane =
-//               footprint cell) before one LDS add per cell and term; the tile is then stored
-//               to the planes with its checkerboard sign (split tiles add atomically into a
-//               zeroed tile instead).
+//   4. grid:    a persistent workgroup per CU takes items from a queue, counting-sorts each
+//               item's records by exact footprint position, and lets each wavefront accumulate
+//               a run of same-position records in registers (lane = footprint cell) before one
+//               LDS add per cell and term; the tile is then stored to the planes with its
+//               checkerboard sign (split tiles add atomically into a zeroed tile instead).
 // Bins are (local slab, tile); the planes need no clearing except split tiles.
 #include "sample.hpp"
 
@@ -25,6 +24,7 @@ constexpr int kBinPerThread = 64;        // samples per thread, preprocessed inp
 constexpr int kLdsBins = 16384;          // LDS histogram capacity of the binning passes
 constexpr int kSub = 1024;               // records staged in LDS at a time
 constexpr int kMinItem = 4 * kSub;       // smallest cap on the records of one work item
+constexpr int kMaxItem = 16384;          // largest cap (the per-item order array is 16-bit)
 constexpr int kTileThreads = 512;
 constexpr int kUnit = 64;                // records per accumulation unit (one wavefront)
 
@@ -36,11 +36,12 @@ struct Item {
     int bin, start, count, split;
 };
 
-// Packed footprint of a record relative to its tile: bits [0,6) a0 + 8, [6,12) b0 + 8,
-// [12,16) rows na, [16,20) columns nb.
-struct RecA {
-    int ab;
-    float q1, q2, s;
+// Records, structure of arrays.  ab: footprint relative to the record's tile, bits [0,6)
+// a0 + 8, [6,12) b0 + 8, [12,16) rows na, [16,20) columns nb; q = (q1, q2, s, Re v); vy = Im v.
+struct Records {
+    uint32_t* ab;
+    float4* q;
+    float* vy;
 };
 
 __device__ __forceinline__ int pack_ab(int a0l, int b0l, int na, int nb) {
@@ -124,7 +125,7 @@ k_bin_scan(const uint32_t* __restrict__ counts, int nbins, uint32_t* __restrict_
     if (tid == 1023) {
         const uint32_t total = rec_incl;
         const uint32_t fair = (total + 4u * cu_count - 1) / (4u * cu_count);
-        uint32_t cap = max(static_cast<uint32_t>(kMinItem), fair);
+        const uint32_t cap = min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
         s_cap = (cap + kSub - 1) / kSub * kSub;
         totals[0] = total;
         totals[3] = s_cap;
@@ -159,16 +160,12 @@ k_bin_scan(const uint32_t* __restrict__ counts, int nbins, uint32_t* __restrict_
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_scatter(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ cursor,
-              RecA* __restrict__ recs, float2* __restrict__ vis) {
+              Records rec) {
     extern __shared__ uint32_t hist[];
     auto emit = [&](const Footprint& f, int ta, int tb, uint32_t slot) {
-        RecA r;
-        r.ab = pack_ab(f.a0 - ta * tg.T, f.b0 - tb * tg.T, f.na, f.nb);
-        r.q1 = f.q1;
-        r.q2 = f.q2;
-        r.s = f.s;
-        recs[slot] = r;
-        vis[slot] = f.vw;
+        rec.ab[slot] = static_cast<uint32_t>(pack_ab(f.a0 - ta * tg.T, f.b0 - tb * tg.T, f.na, f.nb));
+        rec.q[slot] = make_float4(f.q1, f.q2, f.s, f.vw.x);
+        rec.vy[slot] = f.vw.y;
     };
     if (tg.use_lds) {
         for (int i = threadIdx.x; i < tg.nbins; i += kBinThreads) hist[i] = 0;
@@ -236,41 +233,40 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
+__device__ __forceinline__ int rec_key(uint32_t ab, int KS) {
+    return static_cast<int>(ab & 63) * KS + static_cast<int>((ab >> 6) & 63);
+}
+
 // LDS layout of the tile kernel.
 template <int T, int KMAX>
 struct TileLds {
     static constexpr int KS = T + 8;               // footprint starts in [-8, T) per axis
     static constexpr int NKEYS = KS * KS;
-    static constexpr int MAXUNITS = NKEYS + kSub / kUnit;
+    static constexpr int MAXUNITS = NKEYS + kMaxItem / kUnit;
     static constexpr size_t acc = 0;                                        // float [2][KMAX][T*T]
-    static constexpr size_t srec = acc + sizeof(float) * 2 * KMAX * T * T;  // RecA [kSub]
-    static constexpr size_t svis = srec + sizeof(RecA) * kSub;             // float2 [kSub]
-    static constexpr size_t khist = svis + sizeof(float2) * kSub;          // u32 [NKEYS]
+    static constexpr size_t khist = acc + sizeof(float) * 2 * KMAX * T * T;  // u32 [NKEYS]
     static constexpr size_t koff = khist + sizeof(uint32_t) * NKEYS;
     static constexpr size_t uoff = koff + sizeof(uint32_t) * NKEYS;
     static constexpr size_t units = uoff + sizeof(uint32_t) * NKEYS;        // u32 [MAXUNITS]
-    static constexpr size_t order = units + sizeof(uint32_t) * MAXUNITS;    // u16 [kSub]
-    static constexpr size_t bytes = order + sizeof(uint16_t) * kSub;
+    static constexpr size_t order = units + sizeof(uint32_t) * MAXUNITS;    // u16 [kMaxItem]
+    static constexpr size_t bytes = order + sizeof(uint16_t) * kMaxItem;
 };
 
-// One workgroup per CU, persistent: items come from a queue.  Per item the T x T x K tile
-// accumulator lives in LDS; the item's records are taken kSub at a time, sorted by exact
-// footprint position into units of at most 64 same-position records, and each unit is
-// accumulated by one wavefront: lane = footprint cell, the unit's records held one per lane
-// and broadcast with readlane, 2K FP32 accumulators per lane, one LDS add per cell and term
-// at the end of the unit.
+// One workgroup per CU, persistent: items come from a queue.  Per item, the T x T x K tile
+// accumulator lives in LDS; the item's records are counting-sorted by exact footprint position
+// (u16 order in LDS) and cut into units of at most 64 same-position records.  A wavefront
+// accumulates one unit: lane = footprint cell, the unit's records gathered one per lane and
+// broadcast with readlane, 2K FP32 accumulators per lane, one LDS add per cell and term at the
+// end of the unit.
 template <bool kSnap, int T, int KMAX>
 __global__ void __launch_bounds__(kTileThreads)
-k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
-            const RecA* __restrict__ recs, const float2* __restrict__ vis, GridParams gp,
-            TileGeom tg, float2* __restrict__ planes) {
+k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals, Records rec,
+            GridParams gp, TileGeom tg, float2* __restrict__ planes) {
     using L = TileLds<T, KMAX>;
     constexpr int KS = L::KS, NKEYS = L::NKEYS;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* acc_re = reinterpret_cast<float*>(lds_raw + L::acc);
     float* acc_im = acc_re + KMAX * T * T;
-    RecA* srec = reinterpret_cast<RecA*>(lds_raw + L::srec);
-    float2* svis = reinterpret_cast<float2*>(lds_raw + L::svis);
     uint32_t* khist = reinterpret_cast<uint32_t*>(lds_raw + L::khist);
     uint32_t* koff = reinterpret_cast<uint32_t*>(lds_raw + L::koff);
     uint32_t* uoff = reinterpret_cast<uint32_t*>(lds_raw + L::uoff);
@@ -292,162 +288,156 @@ k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
         const int ta = tile / tg.ntx, tb = tile - ta * tg.ntx;
         const int off = gp.G / 2;
         const int oac = ta * T - off, obc = tb * T - off;    // tile origin, centred coordinates
+        const uint32_t* ab_g = rec.ab + it.start;
+        const float4* q_g = rec.q + it.start;
+        const float* vy_g = rec.vy + it.start;
         for (int e = threadIdx.x; e < 2 * KMAX * T * T; e += kTileThreads) acc_re[e] = 0.0f;
-
-        for (int sub = 0; sub < it.count; sub += kSub) {
-            const int n = min(kSub, it.count - sub);
-            for (int e = threadIdx.x; e < NKEYS; e += kTileThreads) khist[e] = 0;
-            __syncthreads();
-            for (int i = threadIdx.x; i < n; i += kTileThreads) {
-                const RecA r = recs[it.start + sub + i];
-                srec[i] = r;
-                svis[i] = vis[it.start + sub + i];
-                atomicAdd(&khist[(r.ab & 63) * KS + ((r.ab >> 6) & 63)], 1u);
-            }
-            __syncthreads();
-            // Exclusive scans of records and units per key, one wavefront.
-            if (threadIdx.x < 64) {
-                uint32_t run_r = 0, run_u = 0;
-                for (int base = 0; base < NKEYS; base += 64) {
-                    const int k = base + lane;
-                    const uint32_t c = k < NKEYS ? khist[k] : 0u;
-                    const uint32_t u = (c + kUnit - 1) / kUnit;
-                    uint32_t sr = c, su = u;
-                    for (int d = 1; d < 64; d <<= 1) {
-                        const uint32_t tr = __shfl_up(sr, d, 64), tu = __shfl_up(su, d, 64);
-                        if (lane >= d) {
-                            sr += tr;
-                            su += tu;
-                        }
-                    }
-                    if (k < NKEYS) {
-                        koff[k] = run_r + sr - c;
-                        uoff[k] = run_u + su - u;
-                    }
-                    run_r += __shfl(sr, 63, 64);
-                    run_u += __shfl(su, 63, 64);
-                }
-                if (lane == 0) s_nunits = run_u;
-            }
-            __syncthreads();
-            for (int k = threadIdx.x; k < NKEYS; k += kTileThreads) {
-                const uint32_t c = khist[k];
-                khist[k] = koff[k];
-                for (uint32_t q = 0; q * kUnit < c; ++q)
-                    units[uoff[k] + q] = (koff[k] + q * kUnit) |
-                                         (min(static_cast<uint32_t>(kUnit), c - q * kUnit) << 11) |
-                                         (static_cast<uint32_t>(k) << 18);
-            }
-            __syncthreads();
-            for (int i = threadIdx.x; i < n; i += kTileThreads) {
-                const int ab = srec[i].ab;
-                order[atomicAdd(&khist[(ab & 63) * KS + ((ab >> 6) & 63)], 1u)] = static_cast<uint16_t>(i);
-            }
-            __syncthreads();
-
-            for (uint32_t u = wave; u < s_nunits; u += kTileThreads / 64) {
-                const uint32_t w = units[u];
-                const int start = w & 2047, cnt = (w >> 11) & 127, key = w >> 18;
-                const int a0l = key / KS - 8, b0l = key % KS - 8;
-                // stage: lane l holds record l of the unit
-                RecA mine = {0, 0.f, 0.f, 0.f};
-                float2 mv = make_float2(0.f, 0.f);
-                if (lane < cnt) {
-                    const int ri = order[start + lane];
-                    mine = srec[ri];
-                    mv = svis[ri];
-                }
-                float re[KMAX], im[KMAX];
-#pragma unroll
-                for (int k = 0; k < KMAX; ++k) re[k] = im[k] = 0.0f;
-                for (int r = 0; r < cnt; ++r) {
-                    const int ab = readlane_i(mine.ab, r);
-                    const float q1 = readlane_f(mine.q1, r), q2 = readlane_f(mine.q2, r);
-                    const float s = readlane_f(mine.s, r);
-                    const float vx = readlane_f(mv.x, r), vy = readlane_f(mv.y, r);
-                    const int na = (ab >> 12) & 15, nb = (ab >> 16) & 15;
-                    float xa, xb;
-                    if constexpr (kSnap) {
-                        xa = (static_cast<float>(ci) - q1) * 0.25f;
-                        xb = (static_cast<float>(cj) - q2) * 0.25f;
-                    } else {
-                        xa = (static_cast<float>(oac + a0l + ci) - q1) * 0.25f;
-                        xb = (static_cast<float>(obc + b0l + cj) - q2) * 0.25f;
-                    }
-                    const float kv = (ci < na ? es_kernel(xa) : 0.0f) * (cj < nb ? es_kernel(xb) : 0.0f);
-                    float tkm2 = 1.0f, tkm1 = s;
-#pragma unroll
-                    for (int k = 0; k < KMAX; ++k) {
-                        if (k < K) {
-                            float tk;
-                            if (k == 0) tk = 1.0f;
-                            else if (k == 1) tk = s;
-                            else {
-                                tk = 2.0f * s * tkm1 - tkm2;
-                                tkm2 = tkm1;
-                                tkm1 = tk;
-                            }
-                            const float m = tk * kv;
-                            re[k] += vx * m;
-                            im[k] += vy * m;
-                        }
-                    }
-                    if (na == 9 || nb == 9) {
-                        // Ninth row / column of a footprint at an exact integer position:
-                        // lanes 0..8 take row 8 (columns 0..8), lanes 9..16 column 8 (rows 0..7).
-                        int ei = -1, ej = -1;
-                        if (lane < 9 && na == 9) {
-                            ei = 8;
-                            ej = lane;
-                        } else if (lane >= 9 && lane < 17 && nb == 9) {
-                            ei = lane - 9;
-                            ej = 8;
-                        }
-                        if (ei >= 0 && ej < nb && ei < na) {
-                            float ya, yb;
-                            if constexpr (kSnap) {
-                                ya = (static_cast<float>(ei) - q1) * 0.25f;
-                                yb = (static_cast<float>(ej) - q2) * 0.25f;
-                            } else {
-                                ya = (static_cast<float>(oac + a0l + ei) - q1) * 0.25f;
-                                yb = (static_cast<float>(obc + b0l + ej) - q2) * 0.25f;
-                            }
-                            const int ca = a0l + ei, cb = b0l + ej;
-                            if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
-                                const float ekv = es_kernel(ya) * es_kernel(yb);
-                                float t2 = 1.0f, t1 = s;
-                                for (int k = 0; k < K; ++k) {
-                                    float tk;
-                                    if (k == 0) tk = 1.0f;
-                                    else if (k == 1) tk = s;
-                                    else {
-                                        tk = 2.0f * s * t1 - t2;
-                                        t2 = t1;
-                                        t1 = tk;
-                                    }
-                                    const float m = tk * ekv;
-                                    const int idx = k * T * T + ca * T + swz<T>(ca, cb);
-                                    atomicAdd(&acc_re[idx], vx * m);
-                                    atomicAdd(&acc_im[idx], vy * m);
-                                }
-                            }
-                        }
+        for (int e = threadIdx.x; e < NKEYS; e += kTileThreads) khist[e] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < it.count; i += kTileThreads) atomicAdd(&khist[rec_key(ab_g[i], KS)], 1u);
+        __syncthreads();
+        // Exclusive scans of records and units per key, one wavefront.
+        if (threadIdx.x < 64) {
+            uint32_t run_r = 0, run_u = 0;
+            for (int base = 0; base < NKEYS; base += 64) {
+                const int k = base + lane;
+                const uint32_t c = k < NKEYS ? khist[k] : 0u;
+                const uint32_t u = (c + kUnit - 1) / kUnit;
+                uint32_t sr = c, su = u;
+                for (int d = 1; d < 64; d <<= 1) {
+                    const uint32_t tr = __shfl_up(sr, d, 64), tu = __shfl_up(su, d, 64);
+                    if (lane >= d) {
+                        sr += tr;
+                        su += tu;
                     }
                 }
-                const int ca = a0l + ci, cb = b0l + cj;
-                if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
-                    const int cell = ca * T + swz<T>(ca, cb);
-#pragma unroll
-                    for (int k = 0; k < KMAX; ++k) {
-                        if (k < K) {
-                            atomicAdd(&acc_re[k * T * T + cell], re[k]);
-                            atomicAdd(&acc_im[k * T * T + cell], im[k]);
-                        }
-                    }
+                if (k < NKEYS) {
+                    koff[k] = run_r + sr - c;
+                    uoff[k] = run_u + su - u;
                 }
+                run_r += __shfl(sr, 63, 64);
+                run_u += __shfl(su, 63, 64);
             }
-            __syncthreads();
+            if (lane == 0) s_nunits = run_u;
         }
+        __syncthreads();
+        for (int k = threadIdx.x; k < NKEYS; k += kTileThreads) {
+            const uint32_t c = khist[k];
+            khist[k] = koff[k];
+            for (uint32_t q = 0; q * kUnit < c; ++q)
+                units[uoff[k] + q] = (koff[k] + q * kUnit) |
+                                     ((min(static_cast<uint32_t>(kUnit), c - q * kUnit) - 1) << 14) |
+                                     (static_cast<uint32_t>(k) << 21);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < it.count; i += kTileThreads)
+            order[atomicAdd(&khist[rec_key(ab_g[i], KS)], 1u)] = static_cast<uint16_t>(i);
+        __syncthreads();
+
+        for (uint32_t u = wave; u < s_nunits; u += kTileThreads / 64) {
+            const uint32_t w = units[u];
+            const int start = w & 16383, cnt = ((w >> 14) & 127) + 1, key = w >> 21;
+            const int a0l = key / KS - 8, b0l = key % KS - 8;
+            int m_ab = 0;
+            float4 m_q = make_float4(0.f, 0.f, 0.f, 0.f);
+            float m_vy = 0.f;
+            if (lane < cnt) {                             // lane l gathers record l of the unit
+                const int ri = order[start + lane];
+                m_ab = static_cast<int>(ab_g[ri]);
+                m_q = q_g[ri];
+                m_vy = vy_g[ri];
+            }
+            float re[KMAX], im[KMAX];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) re[k] = im[k] = 0.0f;
+            for (int r = 0; r < cnt; ++r) {
+                const int ab = readlane_i(m_ab, r);
+                const float q1 = readlane_f(m_q.x, r), q2 = readlane_f(m_q.y, r);
+                const float s = readlane_f(m_q.z, r);
+                const float vx = readlane_f(m_q.w, r), vy = readlane_f(m_vy, r);
+                const int na = (ab >> 12) & 15, nb = (ab >> 16) & 15;
+                float xa, xb;
+                if constexpr (kSnap) {
+                    xa = (static_cast<float>(ci) - q1) * 0.25f;
+                    xb = (static_cast<float>(cj) - q2) * 0.25f;
+                } else {
+                    xa = (static_cast<float>(oac + a0l + ci) - q1) * 0.25f;
+                    xb = (static_cast<float>(obc + b0l + cj) - q2) * 0.25f;
+                }
+                const float ka = es_kernel_fast(xa), kb = es_kernel_fast(xb);
+                const float kv = (ci < na ? ka : 0.0f) * (cj < nb ? kb : 0.0f);
+                const float wr = vx * kv, wi = vy * kv;
+                float tkm2 = 1.0f, tkm1 = s;
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    if (k < K) {
+                        float tk;
+                        if (k == 0) tk = 1.0f;
+                        else if (k == 1) tk = s;
+                        else {
+                            tk = 2.0f * s * tkm1 - tkm2;
+                            tkm2 = tkm1;
+                            tkm1 = tk;
+                        }
+                        re[k] += tk * wr;
+                        im[k] += tk * wi;
+                    }
+                }
+                if (na == 9 || nb == 9) {
+                    // Ninth row / column of a footprint at an exact integer position:
+                    // lanes 0..8 take row 8 (columns 0..8), lanes 9..16 column 8 (rows 0..7).
+                    int ei = -1, ej = -1;
+                    if (lane < 9 && na == 9) {
+                        ei = 8;
+                        ej = lane;
+                    } else if (lane >= 9 && lane < 17 && nb == 9) {
+                        ei = lane - 9;
+                        ej = 8;
+                    }
+                    if (ei >= 0 && ej < nb && ei < na) {
+                        float ya, yb;
+                        if constexpr (kSnap) {
+                            ya = (static_cast<float>(ei) - q1) * 0.25f;
+                            yb = (static_cast<float>(ej) - q2) * 0.25f;
+                        } else {
+                            ya = (static_cast<float>(oac + a0l + ei) - q1) * 0.25f;
+                            yb = (static_cast<float>(obc + b0l + ej) - q2) * 0.25f;
+                        }
+                        const int ca = a0l + ei, cb = b0l + ej;
+                        if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
+                            const float ekv = es_kernel_fast(ya) * es_kernel_fast(yb);
+                            const float er = vx * ekv, eim = vy * ekv;
+                            float t2 = 1.0f, t1 = s;
+                            for (int k = 0; k < K; ++k) {
+                                float tk;
+                                if (k == 0) tk = 1.0f;
+                                else if (k == 1) tk = s;
+                                else {
+                                    tk = 2.0f * s * t1 - t2;
+                                    t2 = t1;
+                                    t1 = tk;
+                                }
+                                const int idx = k * T * T + ca * T + swz<T>(ca, cb);
+                                atomicAdd(&acc_re[idx], tk * er);
+                                atomicAdd(&acc_im[idx], tk * eim);
+                            }
+                        }
+                    }
+                }
+            }
+            const int ca = a0l + ci, cb = b0l + cj;
+            if (ca >= 0 && ca < T && cb >= 0 && cb < T) {
+                const int cell = ca * T + swz<T>(ca, cb);
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k) {
+                    if (k < K) {
+                        atomicAdd(&acc_re[k * T * T + cell], re[k]);
+                        atomicAdd(&acc_im[k * T * T + cell], im[k]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
 
         const int G = gp.G;
         const size_t plane_size = static_cast<size_t>(G) * G;
@@ -475,24 +465,22 @@ k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
 
 template <bool kSnap, int T, int KMAX>
 void launch_tile_kernel(svdi_context* ctx, unsigned grid, const Item* items, uint32_t* totals,
-                        const RecA* recs, const float2* vis, const GridParams& gp, const TileGeom& tg,
-                        float2* planes) {
+                        const Records& rec, const GridParams& gp, const TileGeom& tg, float2* planes) {
     constexpr size_t bytes = TileLds<T, KMAX>::bytes;
     static_assert(bytes <= 150 * 1024, "tile LDS exceeds the CU's LDS");
     SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-    k_tile_grid<kSnap, T, KMAX><<<grid, kTileThreads, bytes, ctx->stream>>>(items, totals, recs, vis, gp, tg, planes);
+    k_tile_grid<kSnap, T, KMAX><<<grid, kTileThreads, bytes, ctx->stream>>>(items, totals, rec, gp, tg, planes);
     SVDI_LAUNCH_CHECK();
 }
 
 template <bool kSnap>
 void launch_tile_dispatch(svdi_context* ctx, unsigned grid, const Item* items, uint32_t* totals,
-                          const RecA* recs, const float2* vis, const GridParams& gp, const TileGeom& tg,
-                          float2* planes) {
-    if (gp.K <= 4) launch_tile_kernel<kSnap, 32, 4>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
-    else if (gp.K <= 8) launch_tile_kernel<kSnap, 32, 8>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
-    else if (gp.K <= 16) launch_tile_kernel<kSnap, 16, 16>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
-    else launch_tile_kernel<kSnap, 16, 32>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
+                          const Records& rec, const GridParams& gp, const TileGeom& tg, float2* planes) {
+    if (gp.K <= 4) launch_tile_kernel<kSnap, 32, 4>(ctx, grid, items, totals, rec, gp, tg, planes);
+    else if (gp.K <= 8) launch_tile_kernel<kSnap, 32, 8>(ctx, grid, items, totals, rec, gp, tg, planes);
+    else if (gp.K <= 16) launch_tile_kernel<kSnap, 16, 16>(ctx, grid, items, totals, rec, gp, tg, planes);
+    else launch_tile_kernel<kSnap, 16, 32>(ctx, grid, items, totals, rec, gp, tg, planes);
 }
 
 }  // namespace
@@ -514,8 +502,11 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
     uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, tg.nbins * sizeof(uint32_t)));
     uint32_t* totals = counts + tg.nbins;
     Item* items = static_cast<Item*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(Item)));
-    RecA* recs = static_cast<RecA*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(RecA)));
-    float2* vis = static_cast<float2*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float2)));
+    Records rec;
+    rec.q = static_cast<float4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(float4)));
+    char* aux = static_cast<char*>(ctx->ws.get(Workspace::kRecVim, max_records * (sizeof(uint32_t) + sizeof(float))));
+    rec.ab = reinterpret_cast<uint32_t*>(aux);
+    rec.vy = reinterpret_cast<float*>(aux + max_records * sizeof(uint32_t));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), tg.nbins + 4, 0.0f);
     const size_t hist_bytes = tg.use_lds ? tg.nbins * sizeof(uint32_t) : 0;
@@ -531,14 +522,14 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
     k_zero_split<<<tg.nbins, 256, 0, ctx->stream>>>(counts, totals, tg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     if (in.from_snapshot)
-        k_bin_scatter<true><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, recs, vis);
+        k_bin_scatter<true><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, rec);
     else
-        k_bin_scatter<false><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, recs, vis);
+        k_bin_scatter<false><<<bin_blocks, kBinThreads, hist_bytes, ctx->stream>>>(in, gp, tg, cursor, rec);
     SVDI_LAUNCH_CHECK();
     // One persistent workgroup per CU (the tile kernel's LDS admits one per CU).
     const unsigned grid = static_cast<unsigned>(std::min<size_t>(max_items, static_cast<size_t>(cus)));
-    if (in.from_snapshot) launch_tile_dispatch<true>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
-    else launch_tile_dispatch<false>(ctx, grid, items, totals, recs, vis, gp, tg, planes);
+    if (in.from_snapshot) launch_tile_dispatch<true>(ctx, grid, items, totals, rec, gp, tg, planes);
+    else launch_tile_dispatch<false>(ctx, grid, items, totals, rec, gp, tg, planes);
     return totals;
 }
 
