@@ -7,8 +7,13 @@ from raw device-resident visibilities (uvw, frequencies, XX/YY, flags, WEIGHT_SP
 dirty image in HBM: FP64 rotation, planning, ES gridding of the Chebyshev moments, 2D inverse
 FFT, image-plane w-correction, SIN splat.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--gridder auto|atomic|tiled]
-                  [--no-cpu-baseline] [--cpu-samples N]
+The headline gridder is the MFMA one (runs of same-position records contracted on f16 hi/lo
+MFMA, FP32 accumulation); the plain FP32 tiled imager is timed in the same run on the same
+snapshot.  The accuracy of both against an FP64 direct DFT is measured (untimed) at 64 seeded
+pixels plus the peak pixel of the signed, pre-reprojection image.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--gridder mfma|tiled|atomic|auto]
+                  [--no-compare] [--no-accuracy] [--no-cpu-baseline] [--cpu-samples N]
 """
 from __future__ import annotations
 
@@ -25,8 +30,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("snapshots/s and Mvis/s @1024², batched MFMA vs plain FP32, "
           "≤1e-4 err vs FP64 DFT")
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3          # MI355X FP32 matrix peak
+FP16_PEAK_TFLOPS = 2500.0         # MI355X dense FP16 MFMA peak
 ES_SUPPORT = 8
+C_LIGHT = 299792458.0
 
 
 def log(*a):
@@ -68,12 +75,107 @@ def cpu_baseline(snap, cfg, n_snapshots):
             "mvis_per_s": cfg.num_samples / t / 1e6}
 
 
+def time_gridder(S, ctx, dsnap, cfg, image, gridder, steps, warmup, label):
+    """Warm up, then time `steps` snapshots; returns (seconds, mean stage ms, last plan)."""
+    import torch
+    for i in range(warmup):
+        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
+        log(f"bench[{label}]: warmup {i + 1}/{warmup} done ({plan['elapsed_ms']:.3f} ms device)")
+    ctx.set_profiling(True)
+    stage_sum = {k: 0.0 for k in S.STAGES}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
+        for k, v in ctx.stage_times().items():
+            stage_sum[k] += v
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    log(f"bench[{label}]: {steps} steps in {elapsed:.3f} s")
+    return elapsed, {k: v / steps for k, v in stage_sum.items()}, plan
+
+
+def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64):
+    """Max / RMS error of the signed image against an FP64 direct DFT (torch, on the GPU).
+
+    Samples whose ES footprint is not wholly inside the grid are flagged for this comparison
+    (the gridder clips them, the DFT cannot), on both sides; S in the 1/S scaling still counts
+    every sample.  Pixels: 64 seeded ones plus the peak of the first gridder's image."""
+    import numpy as np
+    import torch
+    dev = "cuda"
+    N, cell, G = cfg.image_size, cfg.cell, cfg.grid_size
+    R, C = cfg.num_rows, cfg.n_channels
+    dsnap = S.DeviceSnapshot(snap, dev)
+    rot, _ = S.snapshot_geometry(ctx, dsnap)
+    rot_t = torch.from_numpy(rot).to(dev)
+    uvw = dsnap.tensors["uvw"]
+    freq = dsnap.tensors["freq"]
+    ruvw = uvw @ rot_t.T                                   # (R, 3) rotated, metres
+    scale = freq / C_LIGHT                                 # (C,)
+    b = (ruvw[None, :, :] * scale[:, None, None]).reshape(R * C, 3)   # idx = row + chan R
+    p = b[:, :2] * (cell * G)
+    lo = torch.ceil(p - 4.0)
+    hi = torch.floor(p + 4.0)
+    inside = ((lo >= -(G // 2)) & (hi <= (G - 1) // 2)).all(dim=1)
+    t = dsnap.tensors
+    f0 = t["flag0"] | (~inside).to(torch.uint8)
+    f3 = t["flag3"] | (~inside).to(torch.uint8)
+    acc_snap = dict(snap)
+    acc_snap["flag0"] = f0
+    acc_snap["flag3"] = f3
+    asnap = S.DeviceSnapshot(acc_snap, dev)
+    w0 = torch.where(f0 != 0, 0.0, t["weight0"].double())
+    w3 = torch.where(f3 != 0, 0.0, t["weight3"].double())
+    ws = w0 + w3
+    keep = ws > 0
+    vre = torch.where(keep, (t["vis0_re"].double() * w0 + t["vis3_re"].double() * w3) / ws, 0.0)
+    vim = torch.where(keep, (t["vis0_im"].double() * w0 + t["vis3_im"].double() * w3) / ws, 0.0)
+    det = abs(rot[0, 0] * rot[1, 1] - rot[0, 1] * rot[1, 0])
+
+    images = {}
+    for name, g in gridders.items():
+        img, _ = S.image_snapshot(ctx, asnap, N, cell, output=S.OUTPUT_SIGNED, gridder=g)
+        images[name] = img.double()
+    first = next(iter(images.values()))
+    peak_idx = int(torch.argmax(first.abs()))
+    rng = np.random.default_rng(20261203)
+    pix = np.stack([rng.integers(0, N, n_pixels), rng.integers(0, N, n_pixels)], axis=1)
+    pix = np.concatenate([pix, [[peak_idx % N, peak_idx // N]]], axis=0)   # (x, y)
+    bk, vrk, vik = b[keep], vre[keep], vim[keep]
+    dft = np.zeros(len(pix))
+    for i0 in range(0, len(pix), 8):
+        pp = torch.from_numpy(pix[i0:i0 + 8]).to(dev).double()
+        t1 = cell * (pp[:, 1] - N // 2)
+        t2 = cell * (pp[:, 0] - N // 2)
+        ra = 1.0 - (t1 + rot[0, 2]) ** 2 - (t2 + rot[1, 2]) ** 2
+        t3 = -rot[2, 2] + math.copysign(1.0, rot[2, 2]) * torch.sqrt(ra)
+        ph = 2.0 * math.pi * (bk[None, :, 0] * t1[:, None] + bk[None, :, 1] * t2[:, None]
+                              + bk[None, :, 2] * t3[:, None])
+        val = (vrk[None, :] * torch.cos(ph) - vik[None, :] * torch.sin(ph)).sum(dim=1)
+        dft[i0:i0 + 8] = (val / (det * R * C)).cpu().numpy()
+    dft_peak = abs(dft[-1])
+    out = {"pixels": int(len(pix)), "peak_pixel_xy": [int(pix[-1, 0]), int(pix[-1, 1])],
+           "dft_peak": float(dft[-1]),
+           "samples_flagged_outside_grid": int((~inside).sum()),
+           "samples_kept": int(keep.sum())}
+    for name, img in images.items():
+        vals = img.cpu().numpy()[pix[:, 1], pix[:, 0]]
+        d = np.abs(vals - dft) / dft_peak
+        out[name] = {"max_rel_err": float(d.max()), "rms_rel_err": float(np.sqrt(np.mean(d * d)))}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--gridder", default="auto", choices=["auto", "atomic", "tiled"])
+    ap.add_argument("--gridder", default="mfma", choices=["mfma", "tiled", "atomic", "auto"])
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip timing the plain FP32 tiled imager")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=1)
     args = ap.parse_args()
@@ -96,46 +198,41 @@ def main():
     dsnap = S.DeviceSnapshot(snap, "cuda")
     w_ok = (snap["flag0"] == 0) | (snap["flag3"] == 0)
     S_eff = int(np.count_nonzero(w_ok))
-    gridder = {"auto": S.GRIDDER_AUTO, "atomic": S.GRIDDER_ATOMIC, "tiled": S.GRIDDER_TILED}[args.gridder]
+    gmap = {"auto": S.GRIDDER_AUTO, "atomic": S.GRIDDER_ATOMIC, "tiled": S.GRIDDER_TILED,
+            "mfma": S.GRIDDER_MFMA}
+    gridder = gmap[args.gridder]
 
     ctx = S.Context(0)
-    stream = torch.cuda.current_stream()
     image = torch.empty((cfg.image_size, cfg.image_size), dtype=torch.float32, device="cuda")
-    for i in range(args.warmup):
-        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
-        log(f"bench: warmup {i + 1}/{args.warmup} done ({plan['elapsed_ms']:.3f} ms device)")
-    ctx.set_profiling(True)
-    stage_sum = {k: 0.0 for k in ("preprocess", "plan", "grid", "fft", "image")}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
-        for k, v in ctx.stage_times().items():
-            stage_sum[k] += v
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    log(f"bench: {args.steps} steps in {elapsed:.3f} s; plan {plan}")
+    elapsed, stages, plan = time_gridder(S, ctx, dsnap, cfg, image, gridder, args.steps,
+                                         args.warmup, args.gridder)
+    log(f"bench: plan {plan}")
     ms_per_step = elapsed * 1e3 / args.steps
-    stages = {k: v / args.steps for k, v in stage_sum.items()}
     snaps_per_s = args.steps / elapsed
     K, P = plan["num_terms"], plan["num_planes"]
     alg = algorithmic(cfg, S_eff, K, P)
 
-    # Dominant stage: the gridder.  One gridding launch per slab batch (one batch here).
+    # Dominant kernel: the gridder's accumulation kernel, timed alone with HIP events on the
+    # library's stream.  One launch per slab batch (one batch here).
     launches = max(1, math.ceil(plan["num_slabs"] / max(1, plan["batch_slabs"])))
-    grid_ms = stages["grid"] / launches
-    achieved_tflops = alg["F_grid"] / launches / (grid_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved_tflops, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved_tflops / FP32_PEAK_TFLOPS, "traffic": None,
-                "kernel": "grid", "kernel_ms": grid_ms,
-                "algorithmic": "F_grid = 4 * 8^2 * K * S_unflagged per launch"}
+    kernel_ms = stages["grid_accumulate"] / launches
+    achieved_tflops = alg["F_grid"] / launches / (kernel_ms * 1e-3) / 1e12
+    mfma = args.gridder == "mfma"
+    peak = FP16_PEAK_TFLOPS if mfma else FP32_PEAK_TFLOPS
+    kernel = {"mfma": "k_tile_grid (MFMA)", "tiled": "k_tile_grid (FP32)",
+              "auto": "k_tile_grid (FP32)", "atomic": "k_grid_atomic"}[args.gridder]
+    roofline = {"bound": "mfma", "achieved": achieved_tflops, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved_tflops / peak, "traffic": None,
+                "kernel": kernel, "kernel_ms": kernel_ms,
+                "algorithmic": "F_grid = 4 * 8^2 * K * S_unflagged per launch",
+                "peak_kind": "dense FP16 MFMA" if mfma else "FP32 matrix"}
     hbm_bytes = alg["B_vis"] + alg["B_grid"] + alg["B_fft"] + alg["B_img"]
 
     out = {
         "metric": METRIC, "value": snaps_per_s, "unit": "snapshots/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 hi/lo MFMA, f32 accumulate, f64 geometry" if mfma else "f32, f64 geometry",
         "data": "synthetic",
         "config": {"workload": cfg.name, "stations": cfg.n_stations, "rows": cfg.num_rows,
                    "channels": cfg.n_channels, "samples": cfg.num_samples,
@@ -144,10 +241,25 @@ def main():
                    "gridder": args.gridder, "parallelism": "single-gpu"},
         "mvis_per_s": cfg.num_samples * snaps_per_s / 1e6,
         "stages_ms": stages,
+        "records": plan["records"],
         "path_hbm_bytes_algorithmic": hbm_bytes,
         "path_roofline_cap_snapshots_per_s": HBM_PEAK_GBS * 1e9 / hbm_bytes,
         "roofline": roofline,
     }
+    if not args.no_compare and args.gridder != "tiled":
+        el2, st2, _ = time_gridder(S, ctx, dsnap, cfg, image, S.GRIDDER_TILED, args.steps,
+                                   args.warmup, "plain-fp32")
+        out["plain_fp32"] = {"snapshots_per_s": args.steps / el2,
+                             "ms_per_step": el2 * 1e3 / args.steps,
+                             "mvis_per_s": cfg.num_samples * args.steps / el2 / 1e6,
+                             "stages_ms": st2}
+        out["speedup_vs_plain_fp32"] = snaps_per_s / (args.steps / el2)
+    if not args.no_accuracy:
+        log("bench: accuracy against the FP64 DFT ...")
+        gl = {args.gridder: gridder}
+        if not args.no_compare and args.gridder != "tiled":
+            gl["plain-fp32"] = S.GRIDDER_TILED
+        out["accuracy_vs_fp64_dft"] = accuracy(S, ctx, snap, cfg, gl)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline ...")
         host = {k: snap[k] for k in snap}

@@ -60,7 +60,9 @@ typedef struct svdi_snapshot {
 typedef enum svdi_gridder {
     SVDI_GRIDDER_AUTO = 0,      /* fastest available */
     SVDI_GRIDDER_ATOMIC = 1,    /* one thread per sample, device atomics (the reference's scheme) */
-    SVDI_GRIDDER_TILED = 2      /* samples binned into LDS-resident grid tiles, FP32 */
+    SVDI_GRIDDER_TILED = 2,     /* samples binned into LDS-resident grid tiles, FP32 */
+    SVDI_GRIDDER_MFMA = 3       /* tiles as SVDI_GRIDDER_TILED, same-position record runs contracted
+                                   on f16 hi/lo MFMA with FP32 accumulation */
 } svdi_gridder;
 
 typedef enum svdi_output {
@@ -81,6 +83,8 @@ typedef struct svdi_options {
     float target_alpha;     /* <= 0: SVD_CHEB_TARGET_ALPHA from the environment, else 10 */
     size_t max_batch_slabs; /* 0: SVDIMAGER_MAX_BATCH_SLABS from the environment, else 25% of free HBM */
     int verbose;            /* 1: print the reference's Chebyshev report and "Time elapsed" to stdout */
+    int mfma_min_records;   /* MFMA gridder: runs of fewer same-position records take the FP32
+                               loop; 0 = default (6), 1 = every run on MFMA */
 } svdi_options;
 
 /* Planning results of one imaging call. */
@@ -117,8 +121,9 @@ int svdi_trim(svdi_context* ctx);
 /* Stage timing with HIP events on the context stream (no counterpart in the reference, which
  * times only the whole slab loop, SVDIntegratedImager.cu:1657-1741).  When enabled, each call
  * records events around its stages; svdi_last_stage_times returns the device milliseconds of
- * the last call per stage: [0] preprocess, [1] plan, [2] grid, [3] fft, [4] image. */
-#define SVDI_NUM_STAGES 5
+ * the last call per stage: [0] preprocess, [1] plan, [2] grid, [3] fft, [4] image, and
+ * [5] the accumulation kernel alone, a part of [2] (the binning and sort are the rest). */
+#define SVDI_NUM_STAGES 6
 int svdi_set_profiling(svdi_context* ctx, int enable);
 int svdi_last_stage_times(svdi_context* ctx, float* ms, int count);
 
@@ -127,6 +132,13 @@ int svdi_last_stage_times(svdi_context* ctx, float* ms, int count);
  * Output buffers are caller-allocated device memory.  The rotation is computed in FP64. */
 int svdi_preprocess(svdi_context* ctx, const svdi_snapshot* snapshot, float* bin, float* vin,
                     float* vis_re, float* vis_im);
+
+/* The FP64 rotation the snapshot imager uses (rows = principal axes after the orientation rules
+ * of SVDIntegratedImager.cu:538-586; the reference keeps it only in FP32 as Vin) and the
+ * ascending eigenvalues of the baseline scatter.  rot: host double[9], row-major; evals: host
+ * double[3] or NULL. */
+int svdi_snapshot_geometry(svdi_context* ctx, const svdi_snapshot* snapshot, double* rot,
+                           double* evals);
 
 /* FIpipeDevice (SVDIntegratedImager.cu:1498-1761): preprocessed device buffers -> image_size^2
  * float image in device memory (row-major, x fastest).  opts / info may be NULL. */

@@ -5,14 +5,16 @@ include/svdimager.h); this package builds it, binds it with ctypes, and generate
 snapshots.  Importing the package does not load the library; the first imaging call does, and
 raises if the library has not been built.
 """
-from ._lib import (GRIDDER_ATOMIC, GRIDDER_AUTO, GRIDDER_TILED, OUTPUT_DIRTY, OUTPUT_SIGNED,
-                   SEMANTICS_ARCHIVE, SEMANTICS_INTEGRATED, Context, SvdiError)
+from ._lib import (GRIDDER_ATOMIC, GRIDDER_AUTO, GRIDDER_MFMA, GRIDDER_TILED, OUTPUT_DIRTY, OUTPUT_SIGNED,
+                   SEMANTICS_ARCHIVE, SEMANTICS_INTEGRATED, STAGES, Context, SvdiError)
 from .imager import (DeviceSnapshot, FIpipe, FIpipeDevice, fipipe_host, image_snapshot,
-                     preprocess_measurement_set_gpu, read_fits_image, write_fits_2d)
+                     preprocess_measurement_set_gpu, read_fits_image, snapshot_geometry,
+                     write_fits_2d)
 
 __all__ = [
     "Context", "SvdiError", "DeviceSnapshot", "FIpipe", "FIpipeDevice", "fipipe_host",
-    "image_snapshot", "preprocess_measurement_set_gpu", "read_fits_image", "write_fits_2d",
-    "GRIDDER_ATOMIC", "GRIDDER_AUTO", "GRIDDER_TILED", "OUTPUT_DIRTY", "OUTPUT_SIGNED",
-    "SEMANTICS_ARCHIVE", "SEMANTICS_INTEGRATED",
+    "image_snapshot", "preprocess_measurement_set_gpu", "read_fits_image", "snapshot_geometry",
+    "write_fits_2d",
+    "GRIDDER_ATOMIC", "GRIDDER_AUTO", "GRIDDER_MFMA", "GRIDDER_TILED", "OUTPUT_DIRTY", "OUTPUT_SIGNED",
+    "SEMANTICS_ARCHIVE", "SEMANTICS_INTEGRATED", "STAGES",
 ]

@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libsvdimager.so")
 
 SVDI_OK = 0
-GRIDDER_AUTO, GRIDDER_ATOMIC, GRIDDER_TILED = 0, 1, 2
+GRIDDER_AUTO, GRIDDER_ATOMIC, GRIDDER_TILED, GRIDDER_MFMA = 0, 1, 2, 3
 OUTPUT_DIRTY, OUTPUT_SIGNED = 0, 1
 SEMANTICS_INTEGRATED, SEMANTICS_ARCHIVE = 0, 1
 
@@ -26,9 +26,10 @@ EXPORTED_SYMBOLS = [
     "svdi_version", "svdi_last_error", "svdi_default_options", "svdi_create", "svdi_destroy",
     "svdi_set_stream", "svdi_trim", "svdi_preprocess", "svdi_image_preprocessed",
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
-    "svdi_set_profiling", "svdi_last_stage_times", "_Z6FIpipePfS_S_S_S_mmf",
+    "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
+    "_Z6FIpipePfS_S_S_S_mmf",
 ]
-STAGES = ("preprocess", "plan", "grid", "fft", "image")
+STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
 
 
 class SvdiError(RuntimeError):
@@ -54,6 +55,7 @@ class Options(ctypes.Structure):
         ("gridder", ctypes.c_int), ("output", ctypes.c_int), ("semantics", ctypes.c_int),
         ("num_terms", ctypes.c_int), ("target_alpha", ctypes.c_float),
         ("max_batch_slabs", ctypes.c_size_t), ("verbose", ctypes.c_int),
+        ("mfma_min_records", ctypes.c_int),
     ]
 
 
@@ -98,6 +100,7 @@ def lib():
             "svdi_set_stream": (ctypes.c_int, [vp, vp]),
             "svdi_trim": (ctypes.c_int, [vp]),
             "svdi_preprocess": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, vp, vp, vp]),
+            "svdi_snapshot_geometry": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, vp]),
             "svdi_image_preprocessed": (ctypes.c_int, [vp, vp, vp, vp, vp, sz, vp, sz, f,
                                                        ctypes.POINTER(Options),
                                                        ctypes.POINTER(PlanInfo)]),

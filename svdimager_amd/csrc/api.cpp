@@ -265,14 +265,17 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
 
     const uint32_t* tile_totals = nullptr;
     const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_TILED : o.gridder;
-    if (gridder != SVDI_GRIDDER_ATOMIC && gridder != SVDI_GRIDDER_TILED) throw ArgError("unknown gridder");
+    if (gridder != SVDI_GRIDDER_ATOMIC && gridder != SVDI_GRIDDER_TILED && gridder != SVDI_GRIDDER_MFMA)
+        throw ArgError("unknown gridder");
+    if (o.mfma_min_records < 0) throw ArgError("mfma_min_records must be >= 0");
+    gp.mfma_min = o.mfma_min_records > 0 ? o.mfma_min_records : kDefaultMfmaMin;
     for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
         gp.nslabs = static_cast<int>(nb);
         tok = stage_begin(ctx, 2);
-        if (gridder == SVDI_GRIDDER_TILED) {
-            tile_totals = launch_grid_tiled(ctx, in, gp, planes);
+        if (gridder == SVDI_GRIDDER_TILED || gridder == SVDI_GRIDDER_MFMA) {
+            tile_totals = launch_grid_tiled(ctx, in, gp, planes, gridder == SVDI_GRIDDER_MFMA);
         } else {
             launch_grid_atomic(ctx, in, gp, planes);
         }
@@ -518,6 +521,22 @@ int svdi_preprocess(svdi_context* ctx, const svdi_snapshot* snap, float* bin, fl
         launch_materialize(ctx, *snap, in.geo, bin, vin, vis_re, vis_im);
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
         finish_call(ctx);
+    });
+}
+
+int svdi_snapshot_geometry(svdi_context* ctx, const svdi_snapshot* snap, double* rot, double* evals) {
+    return guarded([&] {
+        if (!ctx || !snap || !rot) throw ArgError("ctx / snapshot / rot is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        begin_call(ctx);
+        SampleInput in = snapshot_input(ctx, *snap);
+        DevGeometry host;
+        SVDI_HIP(hipMemcpyAsync(&host, in.geo, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        finish_call(ctx);
+        std::memcpy(rot, host.R, sizeof host.R);
+        if (evals) std::memcpy(evals, host.evals, sizeof host.evals);
     });
 }
 

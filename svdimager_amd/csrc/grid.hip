@@ -88,6 +88,7 @@ void launch_grid_atomic(svdi_context* ctx, const SampleInput& in, const GridPara
                         float2* planes) {
     const size_t plane_bytes = static_cast<size_t>(gp.G) * gp.G * sizeof(float2);
     launch_zero(ctx, planes, plane_bytes * static_cast<size_t>(gp.nslabs) * gp.K);
+    const size_t tok = stage_begin(ctx, 5);
     if (in.from_snapshot) {
         dim3 grid(static_cast<unsigned>((in.num_rows + kThreads - 1) / kThreads),
                   static_cast<unsigned>(in.num_channels));
@@ -97,6 +98,7 @@ void launch_grid_atomic(svdi_context* ctx, const SampleInput& in, const GridPara
         k_grid_atomic<false><<<blocks, kThreads, 0, ctx->stream>>>(in, gp, planes);
     }
     SVDI_LAUNCH_CHECK();
+    stage_end(ctx, tok);
 }
 
 }  // namespace svdi

@@ -87,6 +87,17 @@ def preprocess_measurement_set_gpu(ctx: Context, dsnap: DeviceSnapshot) -> Dict:
     return out
 
 
+def snapshot_geometry(ctx: Context, dsnap: DeviceSnapshot) -> Tuple[np.ndarray, np.ndarray]:
+    """The FP64 rotation (3, 3; rows = oriented principal axes) and ascending eigenvalues that
+    the snapshot imager uses."""
+    rot = np.zeros(9, np.float64)
+    evals = np.zeros(3, np.float64)
+    _bind_stream(ctx, dsnap.tensors["uvw"].device)
+    s = dsnap.struct()
+    check(lib().svdi_snapshot_geometry(ctx.handle, ctypes.byref(s), rot.ctypes.data, evals.ctypes.data))
+    return rot.reshape(3, 3), evals
+
+
 def FIpipeDevice(ctx: Context, bin_, vin, vis_re, vis_im, image_size: int, cell_size: float,
                  image=None, **options) -> Tuple[object, Dict]:
     """Image preprocessed device buffers; returns (image tensor (N, N), plan dict)."""

@@ -58,12 +58,14 @@ def test_preprocess_matches_fp64_rotation(gpu_ctx, small_snap, small_dft, dsnap)
     assert np.linalg.det(R) > 0.999
 
 
-@pytest.mark.parametrize("gridder", [S.GRIDDER_ATOMIC])
-def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gridder):
-    announce(f"image_snapshot signed small K=4 gridder={gridder}")
+@pytest.mark.parametrize("gridder,mfma_min", [(S.GRIDDER_ATOMIC, 0), (S.GRIDDER_TILED, 0),
+                                               (S.GRIDDER_MFMA, 0), (S.GRIDDER_MFMA, 1)])
+def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gridder, mfma_min):
+    announce(f"image_snapshot signed small K=4 gridder={gridder} mfma_min={mfma_min}")
     N, cell = small_snap["image_size"], small_snap["cell"]
     img, plan = S.image_snapshot(gpu_ctx, dsnap, N, cell, output=S.OUTPUT_SIGNED,
-                                 num_terms=small_snap["num_terms"], gridder=gridder)
+                                 num_terms=small_snap["num_terms"], gridder=gridder,
+                                 mfma_min_records=mfma_min)
     img = img.cpu().numpy()
     dft = small_dft["dft"]
     peak = _peak(dft)

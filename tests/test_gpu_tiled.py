@@ -1,10 +1,15 @@
-"""GPU parity of the tiled gridder against the reference-style atomic gridder and the oracle.
+"""GPU parity of the tiled gridders (FP32 and MFMA) against the reference-style atomic gridder
+and the oracle.
 
-Both gridders grid the same sample model; they differ only in FP32 summation order, so the
-signed images agree to 1e-5 of peak over the central half and 1e-4 over the image (the
-grid correction amplifies FP32 rounding up to ~85x at the image edge).  Cases: the small snapshot (snapshot and preprocessed
-inputs), several slabs, exact-integer positions (nine-cell footprints), one hot position that
-forces split work items, footprints clipped at the grid edge, every supported K range.
+The gridders grid the same sample model.  The FP32 ones differ only in summation order; the
+MFMA one contracts runs of same-position records on f16 hi/lo operands with FP32 accumulation
+(~22 significant bits per product).  The signed images agree to 1e-5 of peak over the central
+half and 1e-4 over the image (the grid correction amplifies rounding up to ~85x at the image
+edge).  Each case runs with the FP32 tiled gridder, the MFMA gridder at its default run
+threshold (mixed MFMA / FP32 runs), and the MFMA gridder with every run on MFMA.  Cases: the
+small snapshot (snapshot and preprocessed inputs), several slabs, exact-integer positions
+(nine-cell footprints), one hot position that forces split work items, footprints clipped at the
+grid edge, every supported K range.
 """
 import numpy as np
 import pytest
@@ -16,7 +21,12 @@ import svdimager_amd as S
 
 pytestmark = pytest.mark.gpu
 
-
+GRIDDERS = {
+    "tiled": dict(gridder=S.GRIDDER_TILED),
+    "mfma": dict(gridder=S.GRIDDER_MFMA),
+    "mfma_all": dict(gridder=S.GRIDDER_MFMA, mfma_min_records=1),
+}
+gridders = pytest.mark.parametrize("gname", list(GRIDDERS))
 
 
 def _rel(a, b):
@@ -39,40 +49,43 @@ def small_pre(small_snap):
     return O.preprocess_ref32(small_snap)
 
 
-def test_snapshot_tiled_vs_atomic(gpu_ctx, small_snap):
-    announce("snapshot tiled vs atomic")
+@gridders
+def test_snapshot_tiled_vs_atomic(gpu_ctx, small_snap, gname):
+    announce(f"snapshot {gname} vs atomic")
     d = S.DeviceSnapshot(small_snap, "cuda")
     N, cell = small_snap["image_size"], small_snap["cell"]
-    out = {}
-    for g in (S.GRIDDER_ATOMIC, S.GRIDDER_TILED):
-        img, plan = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, num_terms=4, gridder=g)
-        out[g] = img.cpu().numpy()
+    ref, _ = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, num_terms=4,
+                              gridder=S.GRIDDER_ATOMIC)
+    img, plan = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, num_terms=4,
+                                 **GRIDDERS[gname])
     assert plan["records"] > 0
-    assert_parity(out[S.GRIDDER_TILED], out[S.GRIDDER_ATOMIC], f"tiled vs atomic records {plan['records']}")
+    assert_parity(img.cpu().numpy(), ref.cpu().numpy(), f"{gname} vs atomic records {plan['records']}")
 
 
+@gridders
 @pytest.mark.parametrize("K", [1, 4, 8, 12, 16, 24, 32])
-def test_preprocessed_tiled_vs_oracle(gpu_ctx, small_snap, small_pre, K):
-    announce(f"preprocessed tiled K={K}")
+def test_preprocessed_tiled_vs_oracle(gpu_ctx, small_snap, small_pre, K, gname):
+    announce(f"preprocessed {gname} K={K}")
     N, cell = small_snap["image_size"], small_snap["cell"]
     ref = O.fipipe_device_ref32(small_pre["bin"], small_pre["vin"], small_pre["vis_re"],
                                 small_pre["vis_im"], N, cell, num_terms=K)
     img, plan = _image(gpu_ctx, _dev(small_pre), N, cell, output=S.OUTPUT_SIGNED, num_terms=K,
-                       gridder=S.GRIDDER_TILED)
-    assert_parity(img, ref["signed"], f"K={K} tiled vs oracle")
+                       **GRIDDERS[gname])
+    assert_parity(img, ref["signed"], f"K={K} {gname} vs oracle")
 
 
-def test_tiled_multislab_batches(gpu_ctx, small_snap, small_pre):
+@gridders
+def test_tiled_multislab_batches(gpu_ctx, small_snap, small_pre, gname):
     N, cell = small_snap["image_size"], small_snap["cell"]
     t = _dev(small_pre)
     for batch in (0, 2):
-        announce(f"tiled multislab batch={batch}")
+        announce(f"{gname} multislab batch={batch}")
         ref = O.fipipe_device_ref32(small_pre["bin"], small_pre["vin"], small_pre["vis_re"],
                                     small_pre["vis_im"], N, cell, target_alpha=0.005,
                                     max_batch_slabs=(None if batch == 0 else batch))
         for output, key in ((S.OUTPUT_SIGNED, "signed"), (S.OUTPUT_DIRTY, "dirty")):
             img, plan = _image(gpu_ctx, t, N, cell, output=output, target_alpha=0.005,
-                               max_batch_slabs=batch, gridder=S.GRIDDER_TILED)
+                               max_batch_slabs=batch, **GRIDDERS[gname])
             assert plan["num_slabs"] == ref["plan"]["num_slabs"] > 2
             assert_parity(img, ref[key], f"batch={batch} {key} slabs {plan['num_slabs']}",
                           central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
@@ -107,15 +120,17 @@ def test_integer_positions_nine_cell_footprints(gpu_ctx):
     a["bin"][:cand.shape[0], :2] = cand
     ref = O.fipipe_device_ref32(a["bin"], a["vin"], a["vis_re"], a["vis_im"], N, cell, num_terms=8)
     t = _dev(a)
-    for g in (S.GRIDDER_ATOMIC, S.GRIDDER_TILED):
-        img, _ = _image(gpu_ctx, t, N, cell, output=S.OUTPUT_SIGNED, num_terms=8, gridder=g)
-        assert_parity(img, ref["signed"], f"gridder {g}:")
+    cases = dict(GRIDDERS, atomic=dict(gridder=S.GRIDDER_ATOMIC))
+    for gname, kw in cases.items():
+        img, _ = _image(gpu_ctx, t, N, cell, output=S.OUTPUT_SIGNED, num_terms=8, **kw)
+        assert_parity(img, ref["signed"], f"gridder {gname}:")
 
 
-def test_hot_position_splits_items(gpu_ctx):
+@gridders
+def test_hot_position_splits_items(gpu_ctx, gname):
     """Thousands of samples at nearly one position exceed a work item and exercise the split
     (zeroed tile + atomic write-back) path; edge samples exercise grid clipping."""
-    announce("hot position + grid edge")
+    announce(f"hot position + grid edge, {gname}")
     rng = np.random.default_rng(12)
     N, cell = 128, 1e-4
     G = (3 * N + 1) // 2
@@ -128,16 +143,17 @@ def test_hot_position_splits_items(gpu_ctx):
         a["bin"][6010 + i, 1] = e / scale
     ref = O.fipipe_device_ref32(a["bin"], a["vin"], a["vis_re"], a["vis_im"], N, cell, num_terms=8)
     img, plan = _image(gpu_ctx, _dev(a), N, cell, output=S.OUTPUT_SIGNED, num_terms=8,
-                       gridder=S.GRIDDER_TILED)
-    assert_parity(img, ref["signed"], f"hot/edge: records {plan['records']}")
+                       **GRIDDERS[gname])
+    assert_parity(img, ref["signed"], f"hot/edge {gname}: records {plan['records']}")
 
 
-def test_archive_tiled_vs_oracle(gpu_ctx, small_snap, small_pre):
-    announce("archive semantics tiled")
+@gridders
+def test_archive_tiled_vs_oracle(gpu_ctx, small_snap, small_pre, gname):
+    announce(f"archive semantics {gname}")
     N, cell = small_snap["image_size"], small_snap["cell"]
     ref = O.fipipe_archive_ref32(small_pre["bin"], small_pre["vin"], small_pre["vis_re"],
                                  small_pre["vis_im"], N, cell)
     img, plan = _image(gpu_ctx, _dev(small_pre), N, cell, output=S.OUTPUT_SIGNED,
-                       semantics=S.SEMANTICS_ARCHIVE, gridder=S.GRIDDER_TILED)
+                       semantics=S.SEMANTICS_ARCHIVE, **GRIDDERS[gname])
     assert plan["num_terms"] == ref["plan"]["num_terms"]
-    assert_parity(img, ref["signed"], f"archive tiled")
+    assert_parity(img, ref["signed"], f"archive {gname}")
