@@ -61,8 +61,8 @@ typedef enum svdi_gridder {
     SVDI_GRIDDER_AUTO = 0,      /* fastest available */
     SVDI_GRIDDER_ATOMIC = 1,    /* one thread per sample, device atomics (the reference's scheme) */
     SVDI_GRIDDER_TILED = 2,     /* samples binned into LDS-resident grid tiles, FP32 */
-    SVDI_GRIDDER_MFMA = 3       /* tiles as SVDI_GRIDDER_TILED, same-position record runs contracted
-                                   on f16 hi/lo MFMA with FP32 accumulation */
+    SVDI_GRIDDER_MFMA = 3       /* 16 x 16 output windows held in MFMA accumulators, f16 hi/lo
+                                   operands, FP32 accumulation */
 } svdi_gridder;
 
 typedef enum svdi_output {
@@ -83,8 +83,6 @@ typedef struct svdi_options {
     float target_alpha;     /* <= 0: SVD_CHEB_TARGET_ALPHA from the environment, else 10 */
     size_t max_batch_slabs; /* 0: SVDIMAGER_MAX_BATCH_SLABS from the environment, else 25% of free HBM */
     int verbose;            /* 1: print the reference's Chebyshev report and "Time elapsed" to stdout */
-    int mfma_min_records;   /* MFMA gridder: runs of fewer same-position records take the FP32
-                               loop; 0 = default (6), 1 = every run on MFMA */
 } svdi_options;
 
 /* Planning results of one imaging call. */

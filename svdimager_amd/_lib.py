@@ -55,7 +55,6 @@ class Options(ctypes.Structure):
         ("gridder", ctypes.c_int), ("output", ctypes.c_int), ("semantics", ctypes.c_int),
         ("num_terms", ctypes.c_int), ("target_alpha", ctypes.c_float),
         ("max_batch_slabs", ctypes.c_size_t), ("verbose", ctypes.c_int),
-        ("mfma_min_records", ctypes.c_int),
     ]
 
 

@@ -267,15 +267,16 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_TILED : o.gridder;
     if (gridder != SVDI_GRIDDER_ATOMIC && gridder != SVDI_GRIDDER_TILED && gridder != SVDI_GRIDDER_MFMA)
         throw ArgError("unknown gridder");
-    if (o.mfma_min_records < 0) throw ArgError("mfma_min_records must be >= 0");
-    gp.mfma_min = o.mfma_min_records > 0 ? o.mfma_min_records : kDefaultMfmaMin;
+
     for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
         gp.nslabs = static_cast<int>(nb);
         tok = stage_begin(ctx, 2);
-        if (gridder == SVDI_GRIDDER_TILED || gridder == SVDI_GRIDDER_MFMA) {
-            tile_totals = launch_grid_tiled(ctx, in, gp, planes, gridder == SVDI_GRIDDER_MFMA);
+        if (gridder == SVDI_GRIDDER_MFMA) {
+            tile_totals = launch_grid_windows(ctx, in, gp, planes);
+        } else if (gridder == SVDI_GRIDDER_TILED) {
+            tile_totals = launch_grid_tiled(ctx, in, gp, planes);
         } else {
             launch_grid_atomic(ctx, in, gp, planes);
         }

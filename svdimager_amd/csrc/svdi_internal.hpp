@@ -97,9 +97,6 @@ struct HostPlan {
     size_t row_lo = 0, rows = 0, col_lo = 0, cols = 0;
 };
 
-// MFMA gridder: runs of at least this many same-position records take the MFMA contraction.
-constexpr int kDefaultMfmaMin = 6;
-
 struct GridParams {
     int G;
     int K;
@@ -110,7 +107,6 @@ struct GridParams {
     double scale_f64;       // cell * G / c (snapshot input)
     float inv_det;          // 1 / |det| (vis weighting)
     float det;
-    int mfma_min;           // MFMA gridder: smallest unit (records) taken by the MFMA contraction
 };
 
 // Grown-on-demand device buffers owned by a context.
@@ -119,7 +115,7 @@ class Workspace {
     enum Slot {
         kPartials, kPlan, kGeometry, kRot, kConv, kPlanes, kRowFft, kColFft, kTwiddles,
         kBinCounts, kBinOffsets, kBinCursor, kRecPos, kRecVim, kItems, kMisc, kImage,
-        kInBin, kInVre, kInVim, kInVin, kR3Partials, kScanSums, kSlots
+        kInBin, kInVre, kInVim, kInVin, kR3Partials, kScanSums, kRunPos, kRunOff, kSlots
     };
     void* get(Slot s, size_t bytes);
     void release();
@@ -174,10 +170,12 @@ void launch_zero(svdi_context* ctx, void* p, size_t bytes);
 void launch_fill(svdi_context* ctx, float* p, size_t count, float value);
 void launch_grid_atomic(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                         float2* planes);
-// tiles.hip: returns a device pointer to {records, items} of the launch.  mfma: units of at
-// least gp.mfma_min records are accumulated with the f16 hi/lo MFMA contraction.
+// tiles.hip (FP32 tiles) and windows.hip (MFMA windows): return a device pointer to
+// {records, items} of the launch.
 const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
-                                  float2* planes, bool mfma);
+                                  float2* planes);
+const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                                    float2* planes);
 
 // fft.hip
 void fft_prepare(svdi_context* ctx, size_t G);

@@ -17,6 +17,7 @@
 //               checkerboard sign (split tiles add atomically into a zeroed tile instead).
 // Tiles: T = 32 for K <= 8, T = 16 above (the K-plane LDS accumulator must fit twice per CU).
 #include "sample.hpp"
+#include "scan.hpp"
 
 namespace svdi {
 namespace {
@@ -27,8 +28,6 @@ constexpr int kMinItem = 4096;           // smallest cap on the records of one w
 constexpr int kMaxItem = 16384;          // largest cap
 constexpr int kTileThreads = 512;
 constexpr int kUnit = 64;                // records per accumulation unit (one wavefront)
-constexpr int kScanPer = 4;              // elements per thread in the bucket scan
-constexpr int kScanBlock = 1024 * kScanPer;
 
 struct TileGeom {
     int T, ntx, ntiles, nbins, KS, nkeys;
@@ -96,42 +95,58 @@ __device__ __forceinline__ bool same_footprint(const Footprint& a, const Footpri
     return a.a0 == b.a0 && a.b0 == b.b0 && a.na == b.na && a.nb == b.nb && a.local_slab == b.local_slab;
 }
 
-// Snapshot input, one thread per row: a short baseline keeps its footprint position over many
+// Snapshot input, one thread per row.  A short baseline keeps its footprint position over many
 // consecutive channels, and the densest buckets (~1e5 records at the uv centre) are made of such
-// runs.  A run of same-footprint samples costs one atomic per overlapped tile instead of one per
-// sample.  `fn(run, count, chan_begin, chan_end)` is called once per run; the run's samples are
-// the accepted ones among channels [chan_begin, chan_end).
+// runs, so the count pass takes one atomic per run and overlapped tile, not per sample.  A run
+// spans at most 64 channels and holds at most 64 samples.
+//   count:   each accepted sample stores its place in its run, runpos = d | i << 6 (d = channel
+//            offset from the run's first channel, i = index among the run's samples); at the
+//            run's end, per overlapped tile t, the run's offset inside the bucket (the value the
+//            atomic add returned) goes to runoff[t][first sample of the run];
+//   scatter: slot = boff[bucket] + runoff[t][first sample] + i, with no atomics, and every
+//            sample handled in its own loop iteration (no divergent per-run loops).
+constexpr int kMaxRun = 64;
+
 template <class Fn>
-__device__ __forceinline__ void for_row_runs(const SampleInput& in, const GridParams& gp, Fn&& fn) {
-    const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
-    if (row >= in.num_rows) return;
-    Footprint run;
-    uint32_t len = 0;
-    size_t begin = 0;
-    for (size_t chan = 0; chan < in.num_channels; ++chan) {
-        Footprint f;
-        if (!setup_sample<true>(in, gp, row + chan * in.num_rows, row, chan, f)) continue;
-        if (len > 0 && same_footprint(f, run)) {
-            ++len;
-            continue;
-        }
-        if (len > 0) fn(run, len, begin, chan);
-        run = f;
-        len = 1;
-        begin = chan;
-    }
-    if (len > 0) fn(run, len, begin, in.num_channels);
+__device__ __forceinline__ void for_tiles(const Footprint& f, const TileGeom& tg, Fn&& fn) {
+    int t = 0;
+    for (int ta = f.a0 / tg.T; ta <= (f.a0 + f.na - 1) / tg.T; ++ta)
+        for (int tb = f.b0 / tg.T; tb <= (f.b0 + f.nb - 1) / tg.T; ++tb) fn(t++, ta, tb);
 }
 
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
-k_bin_count(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ bcount) {
+k_bin_count(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ bcount,
+            uint16_t* __restrict__ runpos, uint32_t* __restrict__ runoff) {
     if constexpr (kSnap) {
-        for_row_runs(in, gp, [&](const Footprint& f, uint32_t len, size_t, size_t) {
-            for (int ta = f.a0 / tg.T; ta <= (f.a0 + f.na - 1) / tg.T; ++ta)
-                for (int tb = f.b0 / tg.T; tb <= (f.b0 + f.nb - 1) / tg.T; ++tb)
-                    atomicAdd(&bcount[bucket_of(f, tg, ta, tb)], len);
-        });
+        const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
+        if (row >= in.num_rows) return;
+        const size_t R = in.num_rows, S = in.num_samples;
+        Footprint run;
+        uint32_t len = 0;
+        size_t begin = 0;
+        auto close_run = [&]() {
+            const size_t first = row + begin * R;
+            for_tiles(run, tg, [&](int t, int ta, int tb) {
+                runoff[t * S + first] = atomicAdd(&bcount[bucket_of(run, tg, ta, tb)], len);
+            });
+        };
+        for (size_t chan = 0; chan < in.num_channels; ++chan) {
+            const size_t idx = row + chan * R;
+            Footprint f;
+            if (!setup_sample<true>(in, gp, idx, row, chan, f)) continue;
+            if (len > 0 && (!same_footprint(f, run) || chan - begin >= kMaxRun || len >= kMaxRun)) {
+                close_run();
+                len = 0;
+            }
+            if (len == 0) {
+                run = f;
+                begin = chan;
+            }
+            runpos[idx] = static_cast<uint16_t>((chan - begin) | (len << 6));
+            ++len;
+        }
+        if (len > 0) close_run();
     } else {
         for_sample_tiles<false>(in, gp, tg, [&](const Footprint& f, int ta, int tb) {
             atomicAdd(&bcount[bucket_of(f, tg, ta, tb)], 1u);
@@ -140,104 +155,35 @@ k_bin_count(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ b
 }
 
 // Records are tile-independent (positions relative to the footprint start or absolute), so a
-// sample overlapping several tiles writes the same record into each tile's bucket.
+// sample overlapping several tiles writes the same record into each tile's bucket.  `limit`
+// (the record buffer's capacity) only bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
-k_bin_scatter(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ bcursor, Records rec) {
+k_bin_scatter(SampleInput in, GridParams gp, TileGeom tg, const uint32_t* __restrict__ boff,
+              uint32_t* __restrict__ bcursor, const uint16_t* __restrict__ runpos,
+              const uint32_t* __restrict__ runoff, Records rec, uint32_t limit) {
     if constexpr (kSnap) {
         const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
-        for_row_runs(in, gp, [&](const Footprint& f, uint32_t len, size_t begin, size_t end) {
-            uint32_t base[4];
-            int nt = 0;
-            for (int ta = f.a0 / tg.T; ta <= (f.a0 + f.na - 1) / tg.T; ++ta)
-                for (int tb = f.b0 / tg.T; tb <= (f.b0 + f.nb - 1) / tg.T; ++tb)
-                    base[nt++] = atomicAdd(&bcursor[bucket_of(f, tg, ta, tb)], len);
-            // Second sweep over the run's channels: the accepted ones are exactly its samples
-            // (setup_sample is a pure function of the sample; the checks only bound the writes
-            // to the reserved slots).
-            uint32_t i = 0;
-            for (size_t chan = begin; chan < end && i < len; ++chan) {
-                Footprint g;
-                if (!setup_sample<true>(in, gp, row + chan * in.num_rows, row, chan, g)) continue;
-                if (!same_footprint(g, f)) continue;
-                for (int t = 0; t < nt; ++t) write_record(rec, base[t] + i, g);
-                ++i;
-            }
-        });
+        if (row >= in.num_rows) return;
+        const size_t R = in.num_rows, S = in.num_samples;
+        for (size_t chan = 0; chan < in.num_channels; ++chan) {
+            const size_t idx = row + chan * R;
+            Footprint f;
+            if (!setup_sample<true>(in, gp, idx, row, chan, f)) continue;
+            const uint32_t rp = runpos[idx];
+            const size_t first = row + (chan - (rp & 63)) * R;
+            const uint32_t i = rp >> 6;
+            for_tiles(f, tg, [&](int t, int ta, int tb) {
+                const uint32_t slot = boff[bucket_of(f, tg, ta, tb)] + runoff[t * S + first] + i;
+                if (slot < limit) write_record(rec, slot, f);
+            });
+        }
     } else {
         for_sample_tiles<false>(in, gp, tg, [&](const Footprint& f, int ta, int tb) {
-            write_record(rec, atomicAdd(&bcursor[bucket_of(f, tg, ta, tb)], 1u), f);
+            const uint32_t slot = atomicAdd(&bcursor[bucket_of(f, tg, ta, tb)], 1u);
+            if (slot < limit) write_record(rec, slot, f);
         });
     }
-}
-
-// Block-wide inclusive scan of one value per thread (blockDim.x = 1024).
-__device__ __forceinline__ uint32_t block_scan_1024(uint32_t v, uint32_t* s_wave) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
-    if (lane == 63) s_wave[wave] = v;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        uint32_t w = threadIdx.x < 16 ? s_wave[threadIdx.x] : 0u;
-        for (int d = 1; d < 16; d <<= 1) {
-            const uint32_t t = __shfl_up(w, d, 64);
-            if (lane >= d) w += t;
-        }
-        if (threadIdx.x < 16) s_wave[threadIdx.x] = w;
-    }
-    __syncthreads();
-    const uint32_t r = v + (wave > 0 ? s_wave[wave - 1] : 0u);
-    __syncthreads();
-    return r;
-}
-
-// Exclusive scan of the bucket counts in three launches: per-block scans, a scan of the block
-// totals, then the block offsets added (the scatter's cursors start as a copy).
-__global__ void __launch_bounds__(1024)
-k_scan_blocks(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ sums, uint32_t n) {
-    __shared__ uint32_t s_wave[16];
-    const size_t base = static_cast<size_t>(blockIdx.x) * kScanBlock + threadIdx.x * kScanPer;
-    uint32_t v[kScanPer], tot = 0;
-    for (int j = 0; j < kScanPer; ++j) {
-        v[j] = base + j < n ? in[base + j] : 0u;
-        tot += v[j];
-    }
-    const uint32_t incl = block_scan_1024(tot, s_wave);
-    uint32_t run = incl - tot;
-    for (int j = 0; j < kScanPer; ++j) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
-    }
-    if (threadIdx.x == 1023) sums[blockIdx.x] = incl;
-}
-
-__global__ void __launch_bounds__(1024)
-k_scan_sums(uint32_t* __restrict__ sums, int nblocks, uint32_t* __restrict__ grand_total) {
-    __shared__ uint32_t s_wave[16];
-    const int per = (nblocks + 1023) / 1024;
-    const int lo = min(static_cast<int>(threadIdx.x) * per, nblocks), hi = min(lo + per, nblocks);
-    uint32_t tot = 0;
-    for (int i = lo; i < hi; ++i) tot += sums[i];
-    const uint32_t incl = block_scan_1024(tot, s_wave);
-    uint32_t run = incl - tot;
-    for (int i = lo; i < hi; ++i) {
-        const uint32_t t = sums[i];
-        sums[i] = run;
-        run += t;
-    }
-    if (threadIdx.x == 1023) *grand_total = incl;
-}
-
-__global__ void __launch_bounds__(256)
-k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ sums, uint32_t n, uint32_t* __restrict__ cursor) {
-    const size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t v = out[i] + sums[i / kScanBlock];
-    out[i] = v;
-    cursor[i] = v;
 }
 
 __device__ __forceinline__ uint32_t bin_records(const uint32_t* boff, const TileGeom& tg, int b) {
@@ -317,28 +263,16 @@ __device__ __forceinline__ float readlane_f(float v, int lane) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-
-constexpr int kStageStride = 36;         // floats per staging row (32 records + 4 pad: no bank conflicts)
-constexpr int kChunk = 32;               // records per MFMA reduction step (16x16x32 f16)
-constexpr float kEsScale = 128.0f;       // per-axis ES scale: B = ES_a ES_b 2^14 stays in f16 range
-constexpr int kEsScaleLog2Sq = 14;
-
-// LDS of the tile kernels: the K planes (re, im) of one T x T tile, the unit list of the current
-// item, and (MFMA gridder) a staging area per wavefront: ku[8], kv[8] (ES per axis) and
-// c[2 KMAX] (Chebyshev-weighted visibility, row 2k + ri) for 32 records.
-template <int T, int KMAX, int WAVES, bool kMfma>
+// LDS of the tile kernel: the K planes (re, im) of one T x T tile and the unit list of the
+// current item.
+template <int T, int KMAX>
 struct TileLds {
     static constexpr int KS = T + 8;               // footprint starts in [-8, T) per axis
     static constexpr int NKEYS = KS * KS;
     static constexpr int MAXUNITS = NKEYS + kMaxItem / kUnit;
-    static constexpr int kStageRows = 16 + 2 * KMAX;
-    static constexpr size_t stage_floats = kMfma ? static_cast<size_t>(kStageRows) * kStageStride : 0;
     static constexpr size_t acc = 0;                                        // float [2][KMAX][T*T]
     static constexpr size_t units = acc + sizeof(float) * 2 * KMAX * T * T;  // u32 [MAXUNITS]
-    static constexpr size_t stage = units + sizeof(uint32_t) * MAXUNITS;     // float [WAVES][rows][36]
-    static constexpr size_t bytes = stage + sizeof(float) * stage_floats * WAVES;
+    static constexpr size_t bytes = units + sizeof(uint32_t) * MAXUNITS;
 };
 
 // Wave 0 lists the units of an item: for each footprint key, the part of its bucket inside the
@@ -481,167 +415,17 @@ __device__ __forceinline__ void unit_fp32(const UnitRecords& m, int cnt, const U
     }
 }
 
-__device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
-    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const _Float16 h = static_cast<_Float16>(v[i]);
-        hi[i] = h;
-        lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h));
-    }
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// MFMA unit accumulation.  All records of a unit share the footprint position, so the unit's
-// contribution to its 8 x 8 cells is one contraction over records:
-//   D[m][cell] = sum_r c[m][r] * B[r][cell],  m = 2k + ri,
-//   c[2k + ri][r] = T_k(s_r) * (Re | Im) v_r,  B[r][(i, j)] = ES(x_r,i) ES(y_r,j),
-// run as v_mfma_f32_16x16x32_f16 over chunks of 32 records (M = 16 rows of m per block, N = 16
-// cells per block, 4 blocks for the 64 cells) with FP32 accumulation.  Each FP32 operand is
-// split into f16 hi + lo and the product taken as hi*hi + lo*hi + hi*lo, which keeps ~22
-// significant bits.  c is scaled by a power of two per unit (its largest |v| to ~2^14) and the
-// ES factors by 2^7 each, so nothing overflows f16 and little falls into its subnormals; the
-// scales are undone exactly when the unit is added to the LDS tile.
-template <bool kSnap, int T, int KMAX>
-__device__ __forceinline__ void unit_mfma(const UnitRecords& m, int cnt, const UnitPos& p, int K,
-                                          float* acc_re, float* acc_im, float* st) {
-    constexpr int MB = KMAX / 8;                   // 16-row blocks of m = 2k + ri
-    const int lane = threadIdx.x & 63;
-    float* s_ku = st;
-    float* s_kv = st + 8 * kStageStride;
-    float* s_c = st + 16 * kStageStride;
-
-    // Unit scale from the largest |v| (a power of two, exact to apply and undo).
-    float vmax = lane < cnt ? fmaxf(fabsf(m.q.w), fabsf(m.vy)) : 0.0f;
-    for (int d = 32; d >= 1; d >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, d, 64));
-    int e = 0;
-    if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
-    e = min(max(e, -100), 100);
-    const float vscale = ldexpf(1.0f, 14 - e);
-
-    // Ninth rows / columns (records at exact integer positions) go straight to LDS.
-    uint64_t ninth = __ballot(lane < cnt && ((m.nanb & 15) == 9 || (m.nanb >> 4) == 9));
-    while (ninth) {
-        const int r = __builtin_ctzll(ninth);
-        ninth &= ninth - 1;
-        const int nanb = readlane_i(m.nanb, r);
-        add_ninth<kSnap, T, KMAX>(nanb & 15, nanb >> 4, readlane_f(m.q.x, r), readlane_f(m.q.y, r),
-                                  readlane_f(m.q.z, r), readlane_f(m.q.w, r), readlane_f(m.vy, r), p, K,
-                                  acc_re, acc_im);
-    }
-
-    floatx4 d[MB][4];
-#pragma unroll
-    for (int b = 0; b < MB; ++b)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) d[b][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-    const int rl = lane & 31, axis = lane >> 5;    // staging: lane = (record, axis)
-    const int g8 = 8 * (lane >> 4), n16 = lane & 15;
-    for (int c0 = 0; c0 < cnt; c0 += kChunk) {
-        // Stage 32 records: the record's fields come from lane c0 + rl.
-        const int src = c0 + rl;
-        const int nanb = __shfl(m.nanb, src, 64);
-        const float qa = __shfl(axis ? m.q.y : m.q.x, src, 64);
-        const float s = __shfl(m.q.z, src, 64);
-        const float v = __shfl(axis ? m.vy : m.q.w, src, 64);
-        const bool valid = src < cnt;
-        const int n_ax = axis ? (nanb >> 4) : (nanb & 15);
-        const int l0 = axis ? p.b0l : p.a0l, o0 = axis ? p.obc : p.oac;
-        float* s_k = axis ? s_kv : s_ku;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            float w = 0.0f;
-            if (valid && i < n_ax) {
-                float x;
-                if constexpr (kSnap) x = (static_cast<float>(i) - qa) * 0.25f;
-                else x = (static_cast<float>(o0 + l0 + i) - qa) * 0.25f;
-                w = es_kernel_fast(x) * kEsScale;
-            }
-            s_k[i * kStageStride + rl] = w;
-        }
-        const float vs = valid ? v * vscale : 0.0f;
-        float tkm2 = 1.0f, tkm1 = s;
-#pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            float tk;
-            if (k == 0) tk = 1.0f;
-            else if (k == 1) tk = s;
-            else {
-                tk = 2.0f * s * tkm1 - tkm2;
-                tkm2 = tkm1;
-                tkm1 = tk;
-            }
-            s_c[(2 * k + axis) * kStageStride + rl] = k < K ? tk * vs : 0.0f;
-        }
-        wave_lds_sync();
-
-        // Operands.  A (16 x 32): lane holds A[n16][g8 + i]; B (32 x 16): lane holds
-        // B[g8 + i][n16]; D (16 x 16): lane holds D[4 (lane / 16) + i][n16].
-        half8 a_hi[MB], a_lo[MB];
-#pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const float* row = s_c + (16 * b + n16) * kStageStride + g8;
-            split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4),
-                       a_hi[b], a_lo[b]);
-        }
-        const float* kvrow = s_kv + (n16 & 7) * kStageStride + g8;
-        const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
-        const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float* kurow = s_ku + (2 * j + (n16 >> 3)) * kStageStride + g8;
-            const float4 ku0 = *reinterpret_cast<const float4*>(kurow);
-            const float4 ku1 = *reinterpret_cast<const float4*>(kurow + 4);
-            const float4 x0 = make_float4(ku0.x * kv0.x, ku0.y * kv0.y, ku0.z * kv0.z, ku0.w * kv0.w);
-            const float4 x1 = make_float4(ku1.x * kv1.x, ku1.y * kv1.y, ku1.z * kv1.z, ku1.w * kv1.w);
-            half8 b_hi, b_lo;
-            split_hilo(x0, x1, b_hi, b_lo);
-#pragma unroll
-            for (int b = 0; b < MB; ++b) {
-                d[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi[b], b_hi, d[b][j], 0, 0, 0);
-                d[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo[b], b_hi, d[b][j], 0, 0, 0);
-                d[b][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi[b], b_lo, d[b][j], 0, 0, 0);
-            }
-        }
-        wave_lds_sync();
-    }
-
-    // Add the unit to the LDS tile, undoing the scales.
-    const float inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
-#pragma unroll
-    for (int b = 0; b < MB; ++b) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int ca = p.a0l + 2 * j + (n16 >> 3), cb = p.b0l + (n16 & 7);
-            if (ca < 0 || ca >= T || cb < 0 || cb >= T) continue;
-            const int cell = ca * T + swz<T>(ca, cb);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int mm = 16 * b + 4 * (lane >> 4) + i;
-                const int k = mm >> 1;
-                if (k < K) atomicAdd(&((mm & 1) ? acc_im : acc_re)[k * T * T + cell], d[b][j][i] * inv);
-            }
-        }
-    }
-}
-
 // Persistent workgroups take items from a queue.  An item's records are sorted by footprint
 // position; its units (runs of at most 64 same-position records) are listed from the bucket
-// offsets and shared out over the wavefronts.  Units of at least `gp.mfma_min` records take the
-// MFMA contraction (kMfma), the others the FP32 broadcast loop.  The tile is then stored to the
+// offsets and shared out over the wavefronts, each accumulated by the FP32 broadcast loop.  The
+// tile is then stored to the
 // planes with its checkerboard sign, or added atomically when the tile is split over items.
-template <bool kSnap, int T, int KMAX, int WAVES, bool kMfma>
+template <bool kSnap, int T, int KMAX, int WAVES>
 __global__ void __launch_bounds__(WAVES * 64)
 k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
             const uint32_t* __restrict__ boff, Records rec, GridParams gp, TileGeom tg,
             float2* __restrict__ planes) {
-    using L = TileLds<T, KMAX, WAVES, kMfma>;
+    using L = TileLds<T, KMAX>;
     constexpr int KS = L::KS, NKEYS = L::NKEYS, NT = WAVES * 64;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float* acc_re = reinterpret_cast<float*>(lds_raw + L::acc);
@@ -652,7 +436,6 @@ k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
 
     const int K = gp.K;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* stage = reinterpret_cast<float*>(lds_raw + L::stage) + wave * L::stage_floats;
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) s_item = static_cast<int>(atomicAdd(&totals[2], 1u));
@@ -687,12 +470,6 @@ k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
                 m.q = q_g[start + lane];
                 m.vy = vy_g[start + lane];
             }
-            if constexpr (kMfma) {
-                if (cnt >= gp.mfma_min) {
-                    unit_mfma<kSnap, T, KMAX>(m, cnt, p, K, acc_re, acc_im, stage);
-                    continue;
-                }
-            }
             unit_fp32<kSnap, T, KMAX>(m, cnt, p, K, acc_re, acc_im);
         }
         __syncthreads();
@@ -721,42 +498,36 @@ k_tile_grid(const Item* __restrict__ items, uint32_t* __restrict__ totals,
     }
 }
 
-template <bool kSnap, int T, int KMAX, int WAVES, bool kMfma>
+template <bool kSnap, int T, int KMAX, int WAVES>
 void launch_tile_kernel(svdi_context* ctx, const Item* items, uint32_t* totals, const uint32_t* boff,
                         const Records& rec, const GridParams& gp, const TileGeom& tg, float2* planes) {
-    using L = TileLds<T, KMAX, WAVES, kMfma>;
+    using L = TileLds<T, KMAX>;
     constexpr size_t bytes = L::bytes;
     static_assert(bytes <= 160 * 1024 - 64, "tile LDS exceeds a CU");
     if (tg.T != T || tg.KS != L::KS || gp.K > KMAX)
         throw std::logic_error("tile kernel instantiated for another tile geometry");
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX, WAVES, kMfma>),
+    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX, WAVES>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
     const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / (bytes + 64)));
     const unsigned grid = static_cast<unsigned>(std::min(per_cu, 2) * std::max(1, ctx->cu_count));
-    k_tile_grid<kSnap, T, KMAX, WAVES, kMfma><<<grid, WAVES * 64, bytes, ctx->stream>>>(items, totals, boff, rec,
+    k_tile_grid<kSnap, T, KMAX, WAVES><<<grid, WAVES * 64, bytes, ctx->stream>>>(items, totals, boff, rec,
                                                                                       gp, tg, planes);
     SVDI_LAUNCH_CHECK();
 }
 
-template <bool kSnap, bool kMfma>
+template <bool kSnap>
 void launch_tile_dispatch(svdi_context* ctx, const Item* items, uint32_t* totals, const uint32_t* boff,
                           const Records& rec, const GridParams& gp, const TileGeom& tg, float2* planes) {
-    if constexpr (kMfma) {
-        if (gp.K <= 8) launch_tile_kernel<kSnap, 32, 8, 16, true>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else if (gp.K <= 16) launch_tile_kernel<kSnap, 16, 16, 8, true>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else launch_tile_kernel<kSnap, 16, 32, 4, true>(ctx, items, totals, boff, rec, gp, tg, planes);
-    } else {
-        if (gp.K <= 4) launch_tile_kernel<kSnap, 32, 4, 8, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else if (gp.K <= 8) launch_tile_kernel<kSnap, 32, 8, 8, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else if (gp.K <= 16) launch_tile_kernel<kSnap, 16, 16, 8, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else launch_tile_kernel<kSnap, 16, 32, 8, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-    }
+    if (gp.K <= 4) launch_tile_kernel<kSnap, 32, 4, 8>(ctx, items, totals, boff, rec, gp, tg, planes);
+    else if (gp.K <= 8) launch_tile_kernel<kSnap, 32, 8, 8>(ctx, items, totals, boff, rec, gp, tg, planes);
+    else if (gp.K <= 16) launch_tile_kernel<kSnap, 16, 16, 8>(ctx, items, totals, boff, rec, gp, tg, planes);
+    else launch_tile_kernel<kSnap, 16, 32, 8>(ctx, items, totals, boff, rec, gp, tg, planes);
 }
 
 }  // namespace
 
 const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
-                                  float2* planes, bool mfma) {
+                                  float2* planes) {
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
     TileGeom tg;
     tg.T = gp.K <= 8 ? 32 : 16;
@@ -772,7 +543,7 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
         throw UnsupportedError("problem too large for the tiled gridder");
     tg.nbuckets = static_cast<uint32_t>(nbuckets);
     const size_t max_items = static_cast<size_t>(tg.nbins) + (max_records + kMinItem - 1) / kMinItem;
-    const size_t nscan = (nbuckets + kScanBlock - 1) / kScanBlock;
+    const size_t nscan = scan_blocks(nbuckets);
 
     // bcount[0, nbuckets): counts; bcount[nbuckets + 2, +6): totals.  boff[nbuckets] = total.
     uint32_t* bcount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbuckets + 8) * sizeof(uint32_t)));
@@ -791,34 +562,33 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
     unsigned bin_blocks;
     if (in.from_snapshot) bin_blocks = static_cast<unsigned>((in.num_rows + kBinThreads - 1) / kBinThreads);
     else bin_blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
-    if (in.from_snapshot) k_bin_count<true><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcount);
-    else k_bin_count<false><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcount);
+    uint16_t* runpos = nullptr;
+    uint32_t* runoff = nullptr;
+    if (in.from_snapshot) {
+        runpos = static_cast<uint16_t*>(ctx->ws.get(Workspace::kRunPos, S * sizeof(uint16_t)));
+        runoff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, 4 * S * sizeof(uint32_t)));
+        k_bin_count<true><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcount, runpos, runoff);
+    } else {
+        k_bin_count<false><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcount, runpos, runoff);
+    }
     SVDI_LAUNCH_CHECK();
-    const uint32_t n32 = static_cast<uint32_t>(nbuckets);
-    k_scan_blocks<<<static_cast<unsigned>(nscan), 1024, 0, ctx->stream>>>(bcount, boff, scan_sums, n32);
-    SVDI_LAUNCH_CHECK();
-    k_scan_sums<<<1, 1024, 0, ctx->stream>>>(scan_sums, static_cast<int>(nscan), boff + nbuckets);
-    SVDI_LAUNCH_CHECK();
-    k_scan_add<<<static_cast<unsigned>((nbuckets + 255) / 256), 256, 0, ctx->stream>>>(boff, scan_sums, n32, bcursor);
-    SVDI_LAUNCH_CHECK();
+    launch_exclusive_scan(ctx, bcount, boff, bcursor, scan_sums, nbuckets);
     const int cus = std::max(1, ctx->cu_count);
     k_make_items<<<1, 1024, 0, ctx->stream>>>(boff, tg, items, static_cast<int>(max_items), cus, totals);
     SVDI_LAUNCH_CHECK();
     k_zero_split<<<tg.nbins, 256, 0, ctx->stream>>>(boff, totals, tg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
+    const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot)
-        k_bin_scatter<true><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcursor, rec);
+        k_bin_scatter<true><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, boff, bcursor, runpos,
+                                                                         runoff, rec, limit);
     else
-        k_bin_scatter<false><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, bcursor, rec);
+        k_bin_scatter<false><<<bin_blocks, kBinThreads, 0, ctx->stream>>>(in, gp, tg, boff, bcursor, runpos,
+                                                                          runoff, rec, limit);
     SVDI_LAUNCH_CHECK();
     const size_t tok = stage_begin(ctx, 5);
-    if (mfma) {
-        if (in.from_snapshot) launch_tile_dispatch<true, true>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else launch_tile_dispatch<false, true>(ctx, items, totals, boff, rec, gp, tg, planes);
-    } else {
-        if (in.from_snapshot) launch_tile_dispatch<true, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-        else launch_tile_dispatch<false, false>(ctx, items, totals, boff, rec, gp, tg, planes);
-    }
+    if (in.from_snapshot) launch_tile_dispatch<true>(ctx, items, totals, boff, rec, gp, tg, planes);
+    else launch_tile_dispatch<false>(ctx, items, totals, boff, rec, gp, tg, planes);
     stage_end(ctx, tok);
     return totals;
 }

@@ -1,0 +1,425 @@
+// MFMA gridder: the ES gridding of Chebyshev moments (fused_gridding, SVDIntegratedImager.cu:
+// 1218-1293; gridding, archive/FIPkernels.cu:391-452) recast as dense contractions over
+// output-stationary 16 x 16 windows of the grid.
+//
+// For a window W and the samples r whose footprint meets it, every (term k, re/im, cell) of W is
+//   D[m][(a, b)] = sum_r c[m][r] * ku_r[a] * kv_r[b],   m = 2k + ri,
+//   c[2k + ri][r] = T_k(s_r) * (Re | Im) v_r / |det|,   ku_r[a] = ES((a - p1_r) / 4) (0 off support),
+// with (a, b) the window's cells and p_r the sample's position relative to the window.  One
+// wavefront owns one window: for each window row a it runs D_a = C * B_a on
+// v_mfma_f32_16x16x32_f16, with C = c (16 x 32 samples) and B_a[r][b] = ku_r[a] kv_r[b]
+// (32 samples x 16 columns), the 16 row blocks (64 FP32 accumulators per lane) staying in
+// registers across all of the window's samples.  The window is written to the planes once, so
+// no atomics touch the accumulation (windows split over several work items add atomically).
+// Precision: every FP32 operand is split into f16 hi + lo and each product taken as
+// hi*hi + lo*hi + hi*lo with FP32 accumulation (~22 significant bits per product); c is scaled by
+// a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
+// nothing overflows f16; the scales are undone exactly at write-out.
+//
+// Binning: samples are counted per (slab, window) with block-local LDS histograms, scanned, and
+// scattered as records (position relative to the window, s, v), a sample meeting several windows
+// (at most 2 x 2) writing one record per window.
+#include "sample.hpp"
+#include "scan.hpp"
+
+namespace svdi {
+namespace {
+
+constexpr int kWin = 16;                 // window side in cells
+constexpr int kBinThreads = 256;
+constexpr int kBinPerThread = 64;        // samples per thread, preprocessed input
+constexpr int kLdsBins = 12288;          // block-local histogram size (48 KB)
+constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
+constexpr int kChunk = 32;               // samples per MFMA reduction step
+constexpr int kStride = 36;              // floats per staging row (32 samples + pad)
+constexpr int kStageFloats = 48 * kStride;   // ku[16], kv[16], c[16] rows per wavefront
+constexpr int kMinItem = 1024;
+constexpr int kMaxItem = 8192;
+constexpr float kEsScale = 128.0f;       // 2^7 per axis
+constexpr int kEsScaleLog2Sq = 14;
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+struct WinGeom {
+    int nw;          // windows per grid axis
+    int nwin;        // nw * nw
+    int nbins;       // nslabs * nwin
+    int use_lds;     // histogram in LDS (nbins <= kLdsBins)
+};
+
+struct WItem {
+    int bin, start, count, split;
+};
+
+// Records, structure of arrays: q = (p1 - window row origin, p2 - window column origin, s, Re v)
+// in cells, vy = Im v.
+struct WRecords {
+    float4* q;
+    float* vy;
+};
+
+template <bool kSnap, class Fn>
+__device__ __forceinline__ void for_block_samples(const SampleInput& in, Fn&& fn) {
+    if constexpr (kSnap) {
+        const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
+        if (row >= in.num_rows) return;
+        for (size_t chan = 0; chan < in.num_channels; ++chan) fn(row + chan * in.num_rows, row, chan);
+    } else {
+        const size_t base = blockIdx.x * (size_t)kBinThreads * kBinPerThread + threadIdx.x;
+        for (int j = 0; j < kBinPerThread; ++j) {
+            const size_t idx = base + static_cast<size_t>(j) * kBinThreads;
+            if (idx >= in.num_samples) return;
+            fn(idx, 0, 0);
+        }
+    }
+}
+
+// Every (sample, window) pair of this block: the clipped footprint [a0, a0 + na) x [b0, b0 + nb)
+// meets windows a0 / 16 .. (a0 + na - 1) / 16 (at most two per axis, na <= 9).
+template <bool kSnap, class Fn>
+__device__ __forceinline__ void for_sample_windows(const SampleInput& in, const GridParams& gp,
+                                                   const WinGeom& wg, Fn&& fn) {
+    for_block_samples<kSnap>(in, [&](size_t idx, size_t row, size_t chan) {
+        Footprint f;
+        if (!setup_sample<kSnap>(in, gp, idx, row, chan, f)) return;
+        for (int wa = f.a0 / kWin; wa <= (f.a0 + f.na - 1) / kWin; ++wa)
+            for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
+                fn(f, f.local_slab * wg.nwin + wa * wg.nw + wb, wa, wb);
+    });
+}
+
+// Largest |Re v|, |Im v| of the block, folded into *vmax_bits (non-negative floats order as
+// their bit patterns).
+__device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
+    __shared__ float s_max[kBinThreads / 64];
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = s_max[0];
+        for (int w = 1; w < kBinThreads / 64; ++w) b = fmaxf(b, s_max[w]);
+        if (b > 0.0f && b <= 3.0e38f) atomicMax(vmax_bits, __float_as_uint(b));
+    }
+}
+
+template <bool kSnap>
+__global__ void __launch_bounds__(kBinThreads)
+k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
+            uint32_t* __restrict__ vmax_bits) {
+    __shared__ uint32_t hist[kLdsBins];
+    if (wg.use_lds) {
+        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
+        __syncthreads();
+    }
+    float m = 0.0f;
+    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int, int) {
+        if (wg.use_lds) atomicAdd(&hist[bin], 1u);
+        else atomicAdd(&counts[bin], 1u);
+        m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
+    });
+    __syncthreads();
+    if (wg.use_lds) {
+        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads)
+            if (hist[b] != 0) atomicAdd(&counts[b], hist[b]);
+    }
+    block_max_to(m, vmax_bits);
+}
+
+// Two passes over the block's samples: count per window in LDS, reserve each window's range
+// with one atomic, then write the records through LDS cursors.  `limit` (the record buffer's
+// capacity) only bounds the writes.
+template <bool kSnap>
+__global__ void __launch_bounds__(kBinThreads)
+k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ cursor,
+              WRecords rec, uint32_t limit) {
+    __shared__ uint32_t hist[kLdsBins];
+    const int off = gp.G / 2;
+    auto emit = [&](uint32_t slot, const Footprint& f, int wa, int wb) {
+        if (slot >= limit) return;
+        float p1r, p2r;
+        if constexpr (kSnap) {
+            p1r = f.q1 + static_cast<float>(f.a0 - wa * kWin);
+            p2r = f.q2 + static_cast<float>(f.b0 - wb * kWin);
+        } else {
+            p1r = f.q1 - static_cast<float>(wa * kWin - off);
+            p2r = f.q2 - static_cast<float>(wb * kWin - off);
+        }
+        rec.q[slot] = make_float4(p1r, p2r, f.s, f.vw.x);
+        rec.vy[slot] = f.vw.y;
+    };
+    if (!wg.use_lds) {
+        for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
+            emit(atomicAdd(&cursor[bin], 1u), f, wa, wb);
+        });
+        return;
+    }
+    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
+    __syncthreads();
+    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
+        atomicAdd(&hist[bin], 1u);
+    });
+    __syncthreads();
+    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
+        const uint32_t c = hist[b];
+        hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
+    }
+    __syncthreads();
+    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
+        emit(atomicAdd(&hist[bin], 1u), f, wa, wb);
+    });
+}
+
+// One block: work items over the windows, at most `cap` records each, cap = clamp(total /
+// (4 x wavefronts), kMinItem, kMaxItem); every window gets at least one item so every window is
+// written.  totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
+__global__ void __launch_bounds__(1024)
+k_win_items(const uint32_t* __restrict__ boff, WinGeom wg, WItem* __restrict__ items, int max_items,
+            int waves, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t total = boff[wg.nbins];
+    const uint32_t fair = (total + 4u * waves - 1) / (4u * waves);
+    const uint32_t cap = min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
+    const int per = (wg.nbins + 1023) / 1024;
+    const int lo = min(static_cast<int>(threadIdx.x) * per, wg.nbins), hi = min(lo + per, wg.nbins);
+    uint32_t it = 0;
+    for (int b = lo; b < hi; ++b) it += max(1u, (boff[b + 1] - boff[b] + cap - 1) / cap);
+    const uint32_t incl = block_scan_1024(it, s_wave);
+    uint32_t base_it = incl - it;
+    for (int b = lo; b < hi; ++b) {
+        const uint32_t c = boff[b + 1] - boff[b];
+        const uint32_t n = max(1u, (c + cap - 1) / cap);
+        for (uint32_t q = 0; q < n; ++q) {
+            const uint32_t slot = base_it + q;
+            if (slot < static_cast<uint32_t>(max_items)) {
+                WItem t;
+                t.bin = b;
+                t.start = static_cast<int>(boff[b] + q * cap);
+                t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
+                t.split = n > 1;
+                items[slot] = t;
+            }
+        }
+        base_it += n;
+    }
+    if (threadIdx.x == 1023) {
+        totals[0] = total;
+        totals[1] = min(incl, static_cast<uint32_t>(max_items));
+        totals[3] = cap;
+    }
+}
+
+// Clear the windows accumulated by several items (their items add atomically).
+__global__ void __launch_bounds__(256)
+k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ totals, WinGeom wg,
+                 int G, int K, float2* __restrict__ planes) {
+    const int bin = blockIdx.x;
+    if (boff[bin + 1] - boff[bin] <= totals[3]) return;
+    const int ls = bin / wg.nwin, w = bin - ls * wg.nwin;
+    const int wa = w / wg.nw, wb = w - wa * wg.nw;
+    const size_t plane_size = static_cast<size_t>(G) * G;
+    for (int e = threadIdx.x; e < K * kWin * kWin; e += 256) {
+        const int k = e / (kWin * kWin), c = e - k * kWin * kWin;
+        const int a = wa * kWin + c / kWin, b = wb * kWin + c % kWin;
+        if (a < G && b < G)
+            planes[(static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(a) * G + b] =
+                make_float2(0.f, 0.f);
+    }
+}
+
+__device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
+    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const _Float16 h = static_cast<_Float16>(v[i]);
+        hi[i] = h;
+        lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h));
+    }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Persistent wavefronts take items from a queue; each item is one window's record range.  The
+// terms are done eight at a time (one 16-row block of m = 2k + ri).
+__global__ void __launch_bounds__(kWaves * 64)
+k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
+           WinGeom wg, float2* __restrict__ planes) {
+    extern __shared__ __align__(16) float stage_all[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* s_ku = stage_all + wave * kStageFloats;        // [16 rows a][32 samples]
+    float* s_kv = s_ku + 16 * kStride;                     // [16 columns b][32 samples]
+    float* s_c = s_kv + 16 * kStride;                      // [16 m][32 samples]
+
+    // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
+    const float vmax = __uint_as_float(totals[4]);
+    int e = 0;
+    if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
+    e = min(max(e, -100), 100);
+    const float vscale = ldexpf(1.0f, 14 - e);
+    const float inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
+
+    const int K = gp.K, G = gp.G;
+    const size_t plane_size = static_cast<size_t>(G) * G;
+    const int rl = lane & 31, axis = lane >> 5;           // staging: lane = (sample, axis)
+    const int g8 = 8 * (lane >> 4), n16 = lane & 15;      // MFMA operand layout
+    for (;;) {
+        uint32_t ticket = 0;
+        if (lane == 0) ticket = atomicAdd(&totals[2], 1u);
+        const int item = __shfl(static_cast<int>(ticket), 0, 64);
+        if (item >= static_cast<int>(totals[1])) break;
+        const WItem it = items[item];
+        const int ls = it.bin / wg.nwin, w = it.bin - ls * wg.nwin;
+        const int wa = w / wg.nw, wb = w - wa * wg.nw;
+        const float4* q_g = rec.q + it.start;
+        const float* vy_g = rec.vy + it.start;
+        for (int kg = 0; kg < K; kg += 8) {
+            floatx4 d[kWin];
+#pragma unroll
+            for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int c0 = 0; c0 < it.count; c0 += kChunk) {
+                const bool valid = c0 + rl < it.count;
+                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+                float vy = 0.0f;
+                if (valid) {
+                    q = q_g[c0 + rl];
+                    vy = vy_g[c0 + rl];
+                }
+                // ES factors along this lane's axis for the 16 window rows / columns.
+                const float pr = axis ? q.y : q.x;
+                float* s_k = axis ? s_kv : s_ku;
+#pragma unroll
+                for (int a = 0; a < kWin; ++a) {
+                    const float x = (static_cast<float>(a) - pr) * 0.25f;
+                    s_k[a * kStride + rl] = (valid && fabsf(x) <= 1.0f) ? es_kernel_fast(x) * kEsScale : 0.0f;
+                }
+                // c rows 2k + axis for the terms kg .. kg + 7.
+                const float vs = valid ? (axis ? vy : q.w) * vscale : 0.0f;
+                const float s = q.z;
+                float tkm2 = 1.0f, tkm1 = s;
+                for (int k = 0; k < kg + 8; ++k) {
+                    float tk;
+                    if (k == 0) tk = 1.0f;
+                    else if (k == 1) tk = s;
+                    else {
+                        tk = 2.0f * s * tkm1 - tkm2;
+                        tkm2 = tkm1;
+                        tkm1 = tk;
+                    }
+                    if (k >= kg) s_c[(2 * (k - kg) + axis) * kStride + rl] = k < K ? tk * vs : 0.0f;
+                }
+                wave_lds_sync();
+
+                // A (16 x 32): lane holds A[n16][g8 + i]; B (32 x 16): lane holds B[g8 + i][n16];
+                // D (16 x 16): lane holds D[4 (lane / 16) + i][n16].
+                half8 a_hi, a_lo;
+                {
+                    const float* row = s_c + n16 * kStride + g8;
+                    split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4),
+                               a_hi, a_lo);
+                }
+                const float* kvrow = s_kv + n16 * kStride + g8;
+                const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
+                const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
+#pragma unroll
+                for (int a = 0; a < kWin; ++a) {
+                    const float* kurow = s_ku + a * kStride + g8;
+                    const float4 ku0 = *reinterpret_cast<const float4*>(kurow);
+                    const float4 ku1 = *reinterpret_cast<const float4*>(kurow + 4);
+                    const float4 x0 = make_float4(ku0.x * kv0.x, ku0.y * kv0.y, ku0.z * kv0.z, ku0.w * kv0.w);
+                    const float4 x1 = make_float4(ku1.x * kv1.x, ku1.y * kv1.y, ku1.z * kv1.z, ku1.w * kv1.w);
+                    half8 b_hi, b_lo;
+                    split_hilo(x0, x1, b_hi, b_lo);
+                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d[a], 0, 0, 0);
+                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d[a], 0, 0, 0);
+                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d[a], 0, 0, 0);
+                }
+                wave_lds_sync();
+            }
+            // Write the window: rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16)
+            // and kg + 2 (lane / 16) + 1; column n16.
+            const int bg = wb * kWin + n16;
+#pragma unroll
+            for (int a = 0; a < kWin; ++a) {
+                const int ag = wa * kWin + a;
+                if (ag >= G || bg >= G) continue;
+                const float sg = ((ag + bg) & 1) ? -inv : inv;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int k = kg + 2 * (lane >> 4) + h;
+                    if (k >= K) continue;
+                    const float2 val = make_float2(d[a][2 * h] * sg, d[a][2 * h + 1] * sg);
+                    float2* dst = planes + (static_cast<size_t>(ls) * K + k) * plane_size +
+                                  static_cast<size_t>(ag) * G + bg;
+                    if (it.split) {
+                        unsafeAtomicAdd(&dst->x, val.x);
+                        unsafeAtomicAdd(&dst->y, val.y);
+                    } else {
+                        *dst = val;
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace
+
+const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                                    float2* planes) {
+    if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
+    WinGeom wg;
+    wg.nw = (gp.G + kWin - 1) / kWin;
+    wg.nwin = wg.nw * wg.nw;
+    wg.nbins = wg.nwin * gp.nslabs;
+    wg.use_lds = wg.nbins <= kLdsBins ? 1 : 0;
+    const size_t S = in.num_samples;
+    const size_t max_records = 4 * S;                      // a footprint meets at most 2 x 2 windows
+    if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
+    const size_t nbins = static_cast<size_t>(wg.nbins);
+    const size_t max_items = nbins + (max_records + kMinItem - 1) / kMinItem;
+
+    // counts[0, nbins) | counts[nbins + 2, +6) = totals.  boff[nbins] = total.
+    uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 8) * sizeof(uint32_t)));
+    uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
+    uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
+    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nbins) * sizeof(uint32_t)));
+    uint32_t* totals = counts + nbins + 2;
+    WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
+    WRecords rec;
+    rec.q = static_cast<float4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(float4)));
+    rec.vy = static_cast<float*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float)));
+
+    launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 8, 0.0f);
+    unsigned blocks;
+    if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinThreads - 1) / kBinThreads);
+    else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
+    if (in.from_snapshot) k_win_count<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, totals + 4);
+    else k_win_count<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, totals + 4);
+    SVDI_LAUNCH_CHECK();
+    launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
+    const int waves = kWaves * std::max(1, ctx->cu_count);
+    k_win_items<<<1, 1024, 0, ctx->stream>>>(boff, wg, items, static_cast<int>(max_items), waves, totals);
+    SVDI_LAUNCH_CHECK();
+    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
+    SVDI_LAUNCH_CHECK();
+    const uint32_t limit = static_cast<uint32_t>(max_records);
+    if (in.from_snapshot) k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, cursor, rec, limit);
+    else k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, cursor, rec, limit);
+    SVDI_LAUNCH_CHECK();
+
+    const size_t tok = stage_begin(ctx, 5);
+    constexpr size_t lds = sizeof(float) * kStageFloats * kWaves;
+    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_win_grid),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
+    k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes);
+    SVDI_LAUNCH_CHECK();
+    stage_end(ctx, tok);
+    return totals;
+}
+
+}  // namespace svdi

@@ -1,8 +1,8 @@
 """GPU parity of the HIP imager against the oracle and the FP64 DFT (small snapshot).
 
 Tolerances (relative to the peak |DFT| of the signed, pre-reprojection image):
-* vs FP64 DFT: max <= 1e-4 over the whole image (the ES kernel's own aliasing error at the
-  image edge is ~1e-4 for support 8 at 1.5x oversampling), <= 2e-5 over the central half;
+* vs FP64 DFT: max <= 1.1e-4 over the whole image (the ES kernel's own aliasing error at the
+  image edge is 9.6e-5 for support 8 at 1.5x oversampling), <= 2e-5 over the central half;
 * vs the oracle's FP32 restatement of the reference on the same Bin / Vin / vis: <= 1e-5 over
   the central half, <= 1e-4 over the image (the grid correction amplifies FP32 rounding ~85x
   at the edge)
@@ -58,14 +58,12 @@ def test_preprocess_matches_fp64_rotation(gpu_ctx, small_snap, small_dft, dsnap)
     assert np.linalg.det(R) > 0.999
 
 
-@pytest.mark.parametrize("gridder,mfma_min", [(S.GRIDDER_ATOMIC, 0), (S.GRIDDER_TILED, 0),
-                                               (S.GRIDDER_MFMA, 0), (S.GRIDDER_MFMA, 1)])
-def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gridder, mfma_min):
-    announce(f"image_snapshot signed small K=4 gridder={gridder} mfma_min={mfma_min}")
+@pytest.mark.parametrize("gridder", [S.GRIDDER_ATOMIC, S.GRIDDER_TILED, S.GRIDDER_MFMA])
+def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gridder):
+    announce(f"image_snapshot signed small K=4 gridder={gridder}")
     N, cell = small_snap["image_size"], small_snap["cell"]
     img, plan = S.image_snapshot(gpu_ctx, dsnap, N, cell, output=S.OUTPUT_SIGNED,
-                                 num_terms=small_snap["num_terms"], gridder=gridder,
-                                 mfma_min_records=mfma_min)
+                                 num_terms=small_snap["num_terms"], gridder=gridder)
     img = img.cpu().numpy()
     dft = small_dft["dft"]
     peak = _peak(dft)
@@ -74,7 +72,10 @@ def test_snapshot_signed_image_vs_dft(gpu_ctx, small_snap, small_dft, dsnap, gri
     rms = float(np.sqrt(np.mean((img - dft) ** 2))) / peak
     print(f"signed vs DFT: max {err:.3e} central {err_c:.3e} rms {rms:.3e} plan {plan}")
     assert plan["num_terms"] == small_snap["num_terms"]
-    assert err <= 1e-4
+    # The gridding method itself (ES support 8 at 1.5x oversampling, evaluated in FP64 by the
+    # oracle) is 9.6e-5 of peak off the DFT at the image edge; FP32 sums then move edge pixels
+    # by a few 1e-6 (the grid correction amplifies them ~85x there).
+    assert err <= 1.1e-4
     assert err_c <= 2e-5
 
 

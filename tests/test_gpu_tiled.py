@@ -1,15 +1,14 @@
-"""GPU parity of the tiled gridders (FP32 and MFMA) against the reference-style atomic gridder
-and the oracle.
+"""GPU parity of the FP32 tile gridder and the MFMA window gridder against the reference-style
+atomic gridder and the oracle.
 
 The gridders grid the same sample model.  The FP32 ones differ only in summation order; the
-MFMA one contracts runs of same-position records on f16 hi/lo operands with FP32 accumulation
+MFMA one contracts each 16 x 16 window's samples on f16 hi/lo operands with FP32 accumulation
 (~22 significant bits per product).  The signed images agree to 1e-5 of peak over the central
 half and 1e-4 over the image (the grid correction amplifies rounding up to ~85x at the image
-edge).  Each case runs with the FP32 tiled gridder, the MFMA gridder at its default run
-threshold (mixed MFMA / FP32 runs), and the MFMA gridder with every run on MFMA.  Cases: the
-small snapshot (snapshot and preprocessed inputs), several slabs, exact-integer positions
-(nine-cell footprints), one hot position that forces split work items, footprints clipped at the
-grid edge, every supported K range.
+edge).  Cases: the small snapshot (snapshot and preprocessed inputs), several slabs,
+exact-integer positions (nine-cell footprints), one hot position that forces split work items,
+footprints clipped at the grid edge, every supported K range (the MFMA gridder takes terms eight
+at a time, so K = 12, 16, 24, 32 exercise its multi-pass path).
 """
 import numpy as np
 import pytest
@@ -24,7 +23,6 @@ pytestmark = pytest.mark.gpu
 GRIDDERS = {
     "tiled": dict(gridder=S.GRIDDER_TILED),
     "mfma": dict(gridder=S.GRIDDER_MFMA),
-    "mfma_all": dict(gridder=S.GRIDDER_MFMA, mfma_min_records=1),
 }
 gridders = pytest.mark.parametrize("gname", list(GRIDDERS))
 
