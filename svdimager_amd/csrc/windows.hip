@@ -16,9 +16,10 @@
 // a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
 // nothing overflows f16; the scales are undone exactly at write-out.
 //
-// Binning: samples are counted per (slab, window) with block-local LDS histograms, scanned, and
-// scattered as records (position relative to the window, s, v), a sample meeting several windows
-// (at most 2 x 2) writing one record per window.
+// Binning: samples are counted per (slab, window) with block-local LDS histograms; one scan over
+// the (window, block) histogram entries gives each block its slots, and one more pass over the
+// samples writes the records (position relative to the window, s, v), a sample meeting several
+// windows (at most 2 x 2) writing one record per window.
 #include "sample.hpp"
 #include "scan.hpp"
 
@@ -103,10 +104,14 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
+// Window counts.  With LDS histograms each block writes its whole histogram to
+// blockhist[bin * nblocks + block], so that one exclusive scan of that array gives every
+// (window, block) pair its first record slot and the scatter needs one pass over the samples;
+// otherwise the counts go straight to counts[bin].
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
-            uint32_t* __restrict__ vmax_bits) {
+            uint32_t* __restrict__ blockhist, uint32_t* __restrict__ vmax_bits) {
     __shared__ uint32_t hist[kLdsBins];
     if (wg.use_lds) {
         for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
@@ -120,19 +125,21 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     });
     __syncthreads();
     if (wg.use_lds) {
+        const size_t nblocks = gridDim.x;
         for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads)
-            if (hist[b] != 0) atomicAdd(&counts[b], hist[b]);
+            blockhist[static_cast<size_t>(b) * nblocks + blockIdx.x] = hist[b];
     }
     block_max_to(m, vmax_bits);
 }
 
-// Two passes over the block's samples: count per window in LDS, reserve each window's range
-// with one atomic, then write the records through LDS cursors.  `limit` (the record buffer's
-// capacity) only bounds the writes.
+// Records of this block's samples.  With LDS histograms the block's first slot per window comes
+// from the scanned block histograms (`starts`, stride nblocks) and the slots are handed out by LDS
+// cursors; otherwise by a global cursor per window.  `limit` (the record buffer's capacity) only
+// bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
-k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ cursor,
-              WRecords rec, uint32_t limit) {
+k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ starts,
+              uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit) {
     __shared__ uint32_t hist[kLdsBins];
     const int off = gp.G / 2;
     auto emit = [&](uint32_t slot, const Footprint& f, int wa, int wb) {
@@ -154,67 +161,67 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ 
         });
         return;
     }
-    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
-    __syncthreads();
-    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
-        atomicAdd(&hist[bin], 1u);
-    });
-    __syncthreads();
-    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
-        const uint32_t c = hist[b];
-        hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
-    }
+    const size_t nblocks = gridDim.x;
+    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads)
+        hist[b] = starts[static_cast<size_t>(b) * nblocks + blockIdx.x];
     __syncthreads();
     for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
         emit(atomicAdd(&hist[bin], 1u), f, wa, wb);
     });
 }
 
-// One block: work items over the windows, at most `cap` records each, cap = clamp(total /
-// (4 x wavefronts), kMinItem, kMaxItem); every window gets at least one item so every window is
-// written.  totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
-__global__ void __launch_bounds__(1024)
-k_win_items(const uint32_t* __restrict__ boff, WinGeom wg, WItem* __restrict__ items, int max_items,
-            int waves, uint32_t* __restrict__ totals) {
-    __shared__ uint32_t s_wave[16];
-    const uint32_t total = boff[wg.nbins];
+// Work items over the windows, at most `cap` records each, cap = clamp(total / (4 x wavefronts),
+// kMinItem, kMaxItem); every window gets at least one item so every window is written.  Window
+// b's records are [boff[b * stride], boff[(b + 1) * stride]).
+// totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
+__device__ __forceinline__ uint32_t item_cap(uint32_t total, int waves) {
     const uint32_t fair = (total + 4u * waves - 1) / (4u * waves);
-    const uint32_t cap = min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
-    const int per = (wg.nbins + 1023) / 1024;
-    const int lo = min(static_cast<int>(threadIdx.x) * per, wg.nbins), hi = min(lo + per, wg.nbins);
-    uint32_t it = 0;
-    for (int b = lo; b < hi; ++b) it += max(1u, (boff[b + 1] - boff[b] + cap - 1) / cap);
-    const uint32_t incl = block_scan_1024(it, s_wave);
-    uint32_t base_it = incl - it;
-    for (int b = lo; b < hi; ++b) {
-        const uint32_t c = boff[b + 1] - boff[b];
-        const uint32_t n = max(1u, (c + cap - 1) / cap);
-        for (uint32_t q = 0; q < n; ++q) {
-            const uint32_t slot = base_it + q;
-            if (slot < static_cast<uint32_t>(max_items)) {
-                WItem t;
-                t.bin = b;
-                t.start = static_cast<int>(boff[b] + q * cap);
-                t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
-                t.split = n > 1;
-                items[slot] = t;
-            }
-        }
-        base_it += n;
-    }
-    if (threadIdx.x == 1023) {
+    return min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
+}
+
+__global__ void __launch_bounds__(256)
+k_win_item_counts(const uint32_t* __restrict__ boff, size_t stride, WinGeom wg, int waves,
+                  uint32_t* __restrict__ icount, uint32_t* __restrict__ totals) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t total = boff[static_cast<size_t>(wg.nbins) * stride];
+    const uint32_t cap = item_cap(total, waves);
+    if (b == 0) {
         totals[0] = total;
-        totals[1] = min(incl, static_cast<uint32_t>(max_items));
         totals[3] = cap;
+    }
+    if (b >= wg.nbins) return;
+    const uint32_t c = boff[static_cast<size_t>(b + 1) * stride] - boff[static_cast<size_t>(b) * stride];
+    icount[b] = max(1u, (c + cap - 1) / cap);
+}
+
+__global__ void __launch_bounds__(256)
+k_win_item_write(const uint32_t* __restrict__ boff, size_t stride, const uint32_t* __restrict__ ioff,
+                 WinGeom wg, WItem* __restrict__ items, uint32_t max_items, uint32_t* __restrict__ totals) {
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b == 0) totals[1] = min(ioff[wg.nbins], max_items);
+    if (b >= wg.nbins) return;
+    const uint32_t cap = totals[3];
+    const uint32_t start = boff[static_cast<size_t>(b) * stride];
+    const uint32_t c = boff[static_cast<size_t>(b + 1) * stride] - start;
+    const uint32_t n = ioff[b + 1] - ioff[b];
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t slot = ioff[b] + q;
+        if (slot >= max_items) break;
+        WItem t;
+        t.bin = b;
+        t.start = static_cast<int>(start + q * cap);
+        t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
+        t.split = n > 1;
+        items[slot] = t;
     }
 }
 
 // Clear the windows accumulated by several items (their items add atomically).
 __global__ void __launch_bounds__(256)
-k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ totals, WinGeom wg,
-                 int G, int K, float2* __restrict__ planes) {
+k_win_zero_split(const uint32_t* __restrict__ boff, size_t stride, const uint32_t* __restrict__ totals,
+                 WinGeom wg, int G, int K, float2* __restrict__ planes) {
     const int bin = blockIdx.x;
-    if (boff[bin + 1] - boff[bin] <= totals[3]) return;
+    if (boff[static_cast<size_t>(bin + 1) * stride] - boff[static_cast<size_t>(bin) * stride] <= totals[3]) return;
     const int ls = bin / wg.nwin, w = bin - ls * wg.nwin;
     const int wa = w / wg.nw, wb = w - wa * wg.nw;
     const size_t plane_size = static_cast<size_t>(G) * G;
@@ -227,13 +234,19 @@ k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
     }
 }
 
+// x = hi + lo: hi rounds toward zero (one packed conversion per two values), lo = x - hi
+// rounded to nearest (the compiler fuses the product that forms x into a mixed-precision fma),
+// so |x - hi - lo| <= 2^-11 |lo| < 2^-21 |x|.
+typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
     const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const _Float16 h = static_cast<_Float16>(v[i]);
-        hi[i] = h;
-        lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h));
+    for (int i = 0; i < 8; i += 2) {
+        const fp16x2 h = __builtin_amdgcn_cvt_pkrtz(v[i], v[i + 1]);
+        hi[i] = static_cast<_Float16>(h[0]);
+        hi[i + 1] = static_cast<_Float16>(h[1]);
+        lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h[0]));
+        lo[i + 1] = static_cast<_Float16>(v[i + 1] - static_cast<float>(h[1]));
     }
 }
 
@@ -280,21 +293,36 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             floatx4 d[kWin];
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+            // Records of the next chunk are loaded while the current one is contracted.
+            float4 q_next = make_float4(0.f, 0.f, 0.f, 0.f);
+            float vy_next = 0.0f;
+            if (rl < it.count) {
+                q_next = q_g[rl];
+                vy_next = vy_g[rl];
+            }
             for (int c0 = 0; c0 < it.count; c0 += kChunk) {
                 const bool valid = c0 + rl < it.count;
-                float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
-                float vy = 0.0f;
-                if (valid) {
-                    q = q_g[c0 + rl];
-                    vy = vy_g[c0 + rl];
+                const float4 q = q_next;
+                const float vy = vy_next;
+                if (c0 + kChunk + rl < it.count) {
+                    q_next = q_g[c0 + kChunk + rl];
+                    vy_next = vy_g[c0 + kChunk + rl];
                 }
-                // ES factors along this lane's axis for the 16 window rows / columns.
+                // ES factors along this lane's axis: zero the 16 window rows / columns, then
+                // evaluate the at most 9 cells of the support |a - p| <= 4 that lie in the window.
                 const float pr = axis ? q.y : q.x;
                 float* s_k = axis ? s_kv : s_ku;
 #pragma unroll
-                for (int a = 0; a < kWin; ++a) {
-                    const float x = (static_cast<float>(a) - pr) * 0.25f;
-                    s_k[a * kStride + rl] = (valid && fabsf(x) <= 1.0f) ? es_kernel_fast(x) * kEsScale : 0.0f;
+                for (int a = 0; a < kWin; ++a) s_k[a * kStride + rl] = 0.0f;
+                if (valid) {
+                    const int first = static_cast<int>(ceilf(pr - 4.0f));
+#pragma unroll
+                    for (int j = 0; j < 9; ++j) {
+                        const int a = first + j;
+                        const float x = (static_cast<float>(a) - pr) * 0.25f;
+                        if (a >= 0 && a < kWin && fabsf(x) <= 1.0f)
+                            s_k[a * kStride + rl] = es_kernel_fast(x) * kEsScale;
+                    }
                 }
                 // c rows 2k + axis for the terms kg .. kg + 7.
                 const float vs = valid ? (axis ? vy : q.w) * vscale : 0.0f;
@@ -381,12 +409,22 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
     const size_t nbins = static_cast<size_t>(wg.nbins);
     const size_t max_items = nbins + (max_records + kMinItem - 1) / kMinItem;
+    unsigned blocks;
+    if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinThreads - 1) / kBinThreads);
+    else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
+    // Offsets: with LDS histograms, over (window, block) pairs (stride = blocks); else per window.
+    const size_t stride = wg.use_lds ? blocks : 1;
+    const size_t nscan = nbins * stride;
+    if (nscan >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
 
-    // counts[0, nbins) | counts[nbins + 2, +6) = totals.  boff[nbins] = total.
+    // counts[0, nbins) | counts[nbins + 2, +6) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 8) * sizeof(uint32_t)));
-    uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
+    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nscan * sizeof(uint32_t)));
+    uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nscan + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
-    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nbins) * sizeof(uint32_t)));
+    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nscan) * sizeof(uint32_t)));
+    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (nbins + 8) * sizeof(uint32_t)));
+    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
@@ -394,21 +432,28 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     rec.vy = static_cast<float*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float)));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 8, 0.0f);
-    unsigned blocks;
-    if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinThreads - 1) / kBinThreads);
-    else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
-    if (in.from_snapshot) k_win_count<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, totals + 4);
-    else k_win_count<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, totals + 4);
+    if (in.from_snapshot)
+        k_win_count<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+    else
+        k_win_count<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
     SVDI_LAUNCH_CHECK();
-    launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
+    if (wg.use_lds) launch_exclusive_scan(ctx, blockhist, boff, nullptr, sums, nscan);
+    else launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
     const int waves = kWaves * std::max(1, ctx->cu_count);
-    k_win_items<<<1, 1024, 0, ctx->stream>>>(boff, wg, items, static_cast<int>(max_items), waves, totals);
+    const unsigned bin_blocks = static_cast<unsigned>((nbins + 255) / 256);
+    k_win_item_counts<<<bin_blocks, 256, 0, ctx->stream>>>(boff, stride, wg, waves, icount, totals);
     SVDI_LAUNCH_CHECK();
-    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
+    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, nbins);
+    k_win_item_write<<<bin_blocks, 256, 0, ctx->stream>>>(boff, stride, ioff, wg, items,
+                                                          static_cast<uint32_t>(max_items), totals);
+    SVDI_LAUNCH_CHECK();
+    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, stride, totals, wg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     const uint32_t limit = static_cast<uint32_t>(max_records);
-    if (in.from_snapshot) k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, cursor, rec, limit);
-    else k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, cursor, rec, limit);
+    if (in.from_snapshot)
+        k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, boff, cursor, rec, limit);
+    else
+        k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, boff, cursor, rec, limit);
     SVDI_LAUNCH_CHECK();
 
     const size_t tok = stage_begin(ctx, 5);
