@@ -7,13 +7,14 @@ from raw device-resident visibilities (uvw, frequencies, XX/YY, flags, WEIGHT_SP
 dirty image in HBM: FP64 rotation, planning, ES gridding of the Chebyshev moments, 2D inverse
 FFT, image-plane w-correction, SIN splat.  Prints one JSON line.
 
-The headline gridder is the MFMA one (runs of same-position records contracted on f16 hi/lo
-MFMA, FP32 accumulation); the plain FP32 tiled imager is timed in the same run on the same
-snapshot.  The accuracy of both against an FP64 direct DFT is measured (untimed) at 64 seeded
+The headline gridder is the MFMA one (each 16 x 16 grid window contracted over its samples on
+f16 hi/lo MFMA with FP32 accumulation); the plain FP32 tiled imager is timed in the same run on
+the same snapshot.  The accuracy of both against an FP64 direct DFT is measured (untimed) at 64 seeded
 pixels plus the peak pixel of the signed, pre-reprojection image.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--gridder mfma|tiled|atomic|auto]
                   [--no-compare] [--no-accuracy] [--no-cpu-baseline] [--cpu-samples N]
+  python bench.py --copy-calibration MIB     (plain copy, the PMC byte counters' reference)
 """
 from __future__ import annotations
 
@@ -73,6 +74,23 @@ def cpu_baseline(snap, cfg, n_snapshots):
                       f"({cfg.num_samples} samples, {cfg.image_size}^2), median "
                       f"{t:.2f} s each: FP32 preprocess + gridding + FP64 FFT + image pass",
             "mvis_per_s": cfg.num_samples / t / 1e6}
+
+
+def measured_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` on `workload` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, each scaled by the reading of a plain copy of known
+    size), or None when no such measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = table.get(workload, {}).get(kernel)
+    if not rec:
+        return None
+    return {"hbm_bytes_per_dispatch": rec["hbm_bytes_per_dispatch"],
+            "source": f"profiles/traffic.json ({rec.get('summary', '')})"}
 
 
 def time_gridder(S, ctx, dsnap, cfg, image, gridder, steps, warmup, label):
@@ -178,7 +196,20 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=1)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (three times) and exit: the "
+                         "reference reading for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "dispatches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(snapshots shard across GPUs with no collective, see the design notes)",
@@ -219,13 +250,17 @@ def main():
     achieved_tflops = alg["F_grid"] / launches / (kernel_ms * 1e-3) / 1e12
     mfma = args.gridder == "mfma"
     peak = FP16_PEAK_TFLOPS if mfma else FP32_PEAK_TFLOPS
-    kernel = {"mfma": "k_tile_grid (MFMA)", "tiled": "k_tile_grid (FP32)",
-              "auto": "k_tile_grid (FP32)", "atomic": "k_grid_atomic"}[args.gridder]
+    kernel = {"mfma": "k_win_grid", "tiled": "k_tile_grid",
+              "auto": "k_tile_grid", "atomic": "k_grid_atomic"}[args.gridder]
     roofline = {"bound": "mfma", "achieved": achieved_tflops, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved_tflops / peak, "traffic": None,
                 "kernel": kernel, "kernel_ms": kernel_ms,
                 "algorithmic": "F_grid = 4 * 8^2 * K * S_unflagged per launch",
                 "peak_kind": "dense FP16 MFMA" if mfma else "FP32 matrix"}
+    traffic = measured_traffic(kernel, cfg.name)
+    if traffic is not None:
+        roofline["traffic"] = traffic["hbm_bytes_per_dispatch"]
+        roofline["traffic_source"] = traffic["source"]
     hbm_bytes = alg["B_vis"] + alg["B_grid"] + alg["B_fft"] + alg["B_img"]
 
     out = {

@@ -60,6 +60,7 @@ __global__ void k_conv_corr(int N, int G, GL28Table gl, float* __restrict__ conv
     conv[t] = static_cast<float>(acc * kSupport / 2.4937047051153827);
 }
 
+template <int KMAX>
 __global__ void __launch_bounds__(256)
 k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restrict__ vin,
         const float* __restrict__ conv, float* __restrict__ image) {
@@ -77,8 +78,8 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     if (ra < 0.0) return;
     const double t3 = -v33 + (v33 >= 0.0 ? 1.0 : -1.0) * sqrt(ra);
     const double alpha = 2.0 * kPi * t3 * static_cast<double>(ip.slab_half);
-    double jn[kMaxTerms];
-    bessel_sequence_f64(alpha, ip.K, jn);
+    double jn[KMAX];
+    bessel_sequence_fixed<KMAX>(alpha, jn);
     // Plane element read by this pixel: linear index G*G/2 + G/2 + idyr*G + idxr of a G x G
     // plane (SVDIntegratedImager.cu:1397), located in the transformed window.
     const long G = ip.G;
@@ -92,7 +93,9 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
                                         : ip.r3_min + (static_cast<float>(ip.slab_start + ls) + 0.5f) * ip.slab_width;
         double sb, cb;
         sincos(2.0 * kPi * t3 * static_cast<double>(centre), &sb, &cb);
-        for (int k = 0; k < ip.K; ++k) {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            if (k >= ip.K) break;
             const float2 m = px[static_cast<size_t>(ls * ip.K + k) * plane_stride];
             const double a = (k == 0 ? 1.0 : 2.0) * jn[k];
             double cr, ci;
@@ -143,7 +146,10 @@ void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev,
                   const ImageParams& ip, float* image) {
     dim3 block(16, 16);
     dim3 grid(static_cast<unsigned>((ip.N + 15) / 16), static_cast<unsigned>((ip.N + 15) / 16));
-    k_image<<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
+    if (ip.K <= 4) k_image<4><<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
+    else if (ip.K <= 8) k_image<8><<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
+    else if (ip.K <= 16) k_image<16><<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
+    else k_image<32><<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);
     SVDI_LAUNCH_CHECK();
 }
 

@@ -162,12 +162,14 @@ k_term_errors(DevPlan* __restrict__ plan, int archive) {
     const double alpha = static_cast<double>(alpha_f);
     if (threadIdx.x == 0) {
         double j[kMaxTerms];
-        bessel_sequence_f64(alpha, kMaxTerms, j);
+        bessel_sequence_fixed<kMaxTerms>(alpha, j);
+#pragma unroll
         for (int n = 0; n < kMaxTerms; ++n) jn[n] = j[n];
     }
     __syncthreads();
-    float err[kMaxTerms];
-    for (int n = 0; n < kMaxTerms; ++n) err[n] = 0.0f;
+    double err2[kMaxTerms];                  // squared error; the root is taken once at the end
+#pragma unroll
+    for (int n = 0; n < kMaxTerms; ++n) err2[n] = 0.0;
     for (int si = threadIdx.x; si < ns; si += kThreads) {
         double s;
         if (archive) {
@@ -178,6 +180,7 @@ k_term_errors(DevPlan* __restrict__ plan, int archive) {
         double er, ei;
         sincos(alpha * s, &ei, &er);
         double ar = 0.0, ai = 0.0, tm2 = 1.0, tm1 = s;
+#pragma unroll
         for (int n = 0; n < kMaxTerms; ++n) {
             double tn;
             if (n == 0) tn = 1.0;
@@ -199,12 +202,13 @@ k_term_errors(DevPlan* __restrict__ plan, int archive) {
                 }
             }
             const double dr = er - ar, di = ei - ai;
-            err[n] = fmaxf(err[n], static_cast<float>(sqrt(dr * dr + di * di)));
+            err2[n] = fmax(err2[n], dr * dr + di * di);
         }
     }
     // wave-level max, then across the block's waves
+#pragma unroll
     for (int n = 0; n < kMaxTerms; ++n) {
-        float v = err[n];
+        float v = static_cast<float>(sqrt(err2[n]));
         for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
         if ((threadIdx.x & 63) == 0) red[n][threadIdx.x >> 6] = v;
     }

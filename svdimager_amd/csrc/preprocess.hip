@@ -51,7 +51,7 @@ k_geometry_finalize(const double* __restrict__ partials, int nblocks,
                     const double* __restrict__ freq, size_t channels, size_t rows,
                     double array_center_x, DevGeometry* __restrict__ geo) {
     __shared__ double fs[4][kThreads];
-    __shared__ double mom[9];
+    __shared__ double red[9][kThreads];
     double s1 = 0.0, s2 = 0.0, fmn = INFINITY, fmx = -INFINITY;
     for (size_t c = threadIdx.x; c < channels; c += kThreads) {
         const double f = freq[c];
@@ -64,11 +64,12 @@ k_geometry_finalize(const double* __restrict__ partials, int nblocks,
     fs[1][threadIdx.x] = s2;
     fs[2][threadIdx.x] = fmn;
     fs[3][threadIdx.x] = fmx;
-    if (threadIdx.x < 9) {
-        double m = 0.0;
-        for (int b = 0; b < nblocks; ++b) m += partials[b * 9 + threadIdx.x];
-        mom[threadIdx.x] = m;
-    }
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nblocks; b += kThreads)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) acc[i] += partials[b * 9 + i];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) red[i][threadIdx.x] = acc[i];
     __syncthreads();
     for (int stride = kThreads / 2; stride > 0; stride >>= 1) {
         if (threadIdx.x < stride) {
@@ -76,10 +77,15 @@ k_geometry_finalize(const double* __restrict__ partials, int nblocks,
             fs[1][threadIdx.x] += fs[1][threadIdx.x + stride];
             fs[2][threadIdx.x] = fmin(fs[2][threadIdx.x], fs[2][threadIdx.x + stride]);
             fs[3][threadIdx.x] = fmax(fs[3][threadIdx.x], fs[3][threadIdx.x + stride]);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + stride];
         }
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
+    double mom[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) mom[i] = red[i][0];
     const double c = kSpeedOfLight;
     const double S = static_cast<double>(rows) * static_cast<double>(channels);
     const double sf = fs[0][0], sf2 = fs[1][0];

@@ -92,6 +92,68 @@ SVDI_HD void bessel_j01_asymptotic(double x, double* j0, double* j1) {
 // recurrence from the asymptotic J_0, J_1 (stable while n < x); otherwise Miller's backward
 // recurrence normalised by J_0 + 2 sum_k J_{2k} = 1.  Replaces the FP32 power series of
 // bessel_sequence (SVDIntegratedImager.cu:901-920), which loses digits to cancellation near x = 10.
+// J_0..J_{COUNT-1}(x) with every index known at compile time, so the sequence stays in
+// registers on the device: Miller's backward recurrence runs from m down to COUNT + 1 without
+// stores, then through the COUNT stored terms unrolled.  Same operations, in the same order, as
+// the runtime-count form below.
+template <int COUNT>
+SVDI_HD void bessel_sequence_fixed(double x, double* j) {
+    const double ax = fabs(x);
+    if (ax < 1.0) {
+#pragma unroll
+        for (int n = 0; n < COUNT; ++n) j[n] = bessel_series(n, ax);
+    } else if (ax >= 1000.0) {
+        double j0, j1;
+        bessel_j01_asymptotic(ax, &j0, &j1);
+        j[0] = j0;
+        if (COUNT > 1) j[1] = j1;
+#pragma unroll
+        for (int n = 1; n + 1 < COUNT; ++n) j[n + 1] = (2.0 * n / ax) * j[n] - j[n - 1];
+    } else {
+        const double big = 1e150, small = 1e-150;
+        const int top = static_cast<int>(fmax(static_cast<double>(COUNT), ax));
+        const int m = 2 * ((top + 20 + static_cast<int>(sqrt(60.0 * top))) / 2);   // > COUNT
+        double jp = 0.0, jc = 1.0, norm = 0.0;
+#pragma unroll
+        for (int n = 0; n < COUNT; ++n) j[n] = 0.0;
+        for (int k = m; k > COUNT; --k) {
+            const double jm = (2.0 * k / ax) * jc - jp;   // J_{k-1}
+            jp = jc;
+            jc = jm;
+            if (fabs(jc) > big) {
+                jc *= small;
+                jp *= small;
+                norm *= small;
+            }
+            const int km1 = k - 1;
+            if ((km1 & 1) == 0) norm += 2.0 * jc;
+        }
+#pragma unroll
+        for (int k = COUNT; k > 0; --k) {
+            const double jm = (2.0 * k / ax) * jc - jp;
+            jp = jc;
+            jc = jm;
+            if (fabs(jc) > big) {
+                jc *= small;
+                jp *= small;
+                norm *= small;
+#pragma unroll
+                for (int n = 0; n < COUNT; ++n) j[n] *= small;
+            }
+            j[k - 1] = jc;
+            if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2.0 * jc;
+        }
+        norm += jc;                                        // + J_0
+        const double inv = 1.0 / norm;
+#pragma unroll
+        for (int n = 0; n < COUNT; ++n) j[n] *= inv;
+    }
+    if (x < 0.0) {
+#pragma unroll
+        for (int n = 1; n < COUNT; n += 2) j[n] = -j[n];
+    }
+}
+
 SVDI_HD void bessel_sequence_f64(double x, int count, double* j) {
     const double ax = fabs(x);
     if (ax < 1.0) {
@@ -155,7 +217,14 @@ SVDI_HD void eigen_sym3(const double* a_in, double* evals, double* evecs_colmajo
         for (int p = 0; p < 2; ++p)
             for (int q = p + 1; q < 3; ++q) {
                 const double apq = a[p][q];
-                if (apq == 0.0) continue;
+                // An off-diagonal element below the rounding level of its diagonal pair is set
+                // to zero, so the sweep loop ends once the matrix is diagonal to working
+                // precision (relative eigenvalue change <= 1e-17).
+                if (fabs(apq) <= 1e-17 * (fabs(a[p][p]) + fabs(a[q][q]))) {
+                    a[p][q] = 0.0;
+                    a[q][p] = 0.0;
+                    continue;
+                }
                 const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
                 const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                 const double c = 1.0 / sqrt(t * t + 1.0);

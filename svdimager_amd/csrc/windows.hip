@@ -16,10 +16,10 @@
 // a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
 // nothing overflows f16; the scales are undone exactly at write-out.
 //
-// Binning: samples are counted per (slab, window) with block-local LDS histograms; one scan over
-// the (window, block) histogram entries gives each block its slots, and one more pass over the
-// samples writes the records (position relative to the window, s, v), a sample meeting several
-// windows (at most 2 x 2) writing one record per window.
+// Binning: samples are counted per (slab, window) with block-local LDS histograms, kept per block;
+// after a scan of the window totals each block reserves its ranges with one atomic per window
+// and one more pass over its samples writes the records (position relative to the window, s, v),
+// a sample meeting several windows (at most 2 x 2) writing one record per window.
 #include "sample.hpp"
 #include "scan.hpp"
 
@@ -27,8 +27,10 @@ namespace svdi {
 namespace {
 
 constexpr int kWin = 16;                 // window side in cells
-constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 64;        // samples per thread, preprocessed input
+constexpr int kBinThreads = 1024;        // binning blocks: 256 rows x 4 channel groups
+constexpr int kBinRows = 256;
+constexpr int kBinChanGroups = kBinThreads / kBinRows;
+constexpr int kBinPerThread = 16;        // samples per thread, preprocessed input
 constexpr int kLdsBins = 12288;          // block-local histogram size (48 KB)
 constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
 constexpr int kChunk = 32;               // samples per MFMA reduction step
@@ -60,12 +62,16 @@ struct WRecords {
     float* vy;
 };
 
+// Samples of this block.  Snapshot input: 256 consecutive rows, each split over 4 threads that
+// take every fourth channel (a wavefront reads 64 consecutive rows of one channel: coalesced).
+// Preprocessed input: 16384 consecutive samples, 16 per thread at a stride of the block size.
 template <bool kSnap, class Fn>
 __device__ __forceinline__ void for_block_samples(const SampleInput& in, Fn&& fn) {
     if constexpr (kSnap) {
-        const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
+        const size_t row = blockIdx.x * (size_t)kBinRows + (threadIdx.x % kBinRows);
         if (row >= in.num_rows) return;
-        for (size_t chan = 0; chan < in.num_channels; ++chan) fn(row + chan * in.num_rows, row, chan);
+        for (size_t chan = threadIdx.x / kBinRows; chan < in.num_channels; chan += kBinChanGroups)
+            fn(row + chan * in.num_rows, row, chan);
     } else {
         const size_t base = blockIdx.x * (size_t)kBinThreads * kBinPerThread + threadIdx.x;
         for (int j = 0; j < kBinPerThread; ++j) {
@@ -104,10 +110,9 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
-// Window counts.  With LDS histograms each block writes its whole histogram to
-// blockhist[bin * nblocks + block], so that one exclusive scan of that array gives every
-// (window, block) pair its first record slot and the scatter needs one pass over the samples;
-// otherwise the counts go straight to counts[bin].
+// Window counts: per block in an LDS histogram, added to counts[bin] and also kept per block
+// in blockhist[block * nbins + bin] for the scatter (without LDS histograms the counts go
+// straight to counts[bin]).
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
@@ -125,20 +130,23 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     });
     __syncthreads();
     if (wg.use_lds) {
-        const size_t nblocks = gridDim.x;
-        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads)
-            blockhist[static_cast<size_t>(b) * nblocks + blockIdx.x] = hist[b];
+        uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
+        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
+            const uint32_t c = hist[b];
+            row[b] = c;
+            if (c != 0) atomicAdd(&counts[b], c);
+        }
     }
     block_max_to(m, vmax_bits);
 }
 
-// Records of this block's samples.  With LDS histograms the block's first slot per window comes
-// from the scanned block histograms (`starts`, stride nblocks) and the slots are handed out by LDS
-// cursors; otherwise by a global cursor per window.  `limit` (the record buffer's capacity) only
-// bounds the writes.
+// Records of this block's samples, in one pass: with LDS histograms the block reserves each
+// window's range with one atomic on the window's cursor (its count read back from blockhist)
+// and hands out slots with LDS cursors; otherwise every record takes a global cursor slot.
+// `limit` (the record buffer's capacity) only bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
-k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ starts,
+k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ blockhist,
               uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit) {
     __shared__ uint32_t hist[kLdsBins];
     const int off = gp.G / 2;
@@ -161,9 +169,11 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         });
         return;
     }
-    const size_t nblocks = gridDim.x;
-    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads)
-        hist[b] = starts[static_cast<size_t>(b) * nblocks + blockIdx.x];
+    const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
+    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
+        const uint32_t c = row[b];
+        hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
+    }
     __syncthreads();
     for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
         emit(atomicAdd(&hist[bin], 1u), f, wa, wb);
@@ -172,7 +182,7 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
 
 // Work items over the windows, at most `cap` records each, cap = clamp(total / (4 x wavefronts),
 // kMinItem, kMaxItem); every window gets at least one item so every window is written.  Window
-// b's records are [boff[b * stride], boff[(b + 1) * stride]).
+// b's records are [boff[b], boff[b + 1]).
 // totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
 __device__ __forceinline__ uint32_t item_cap(uint32_t total, int waves) {
     const uint32_t fair = (total + 4u * waves - 1) / (4u * waves);
@@ -180,48 +190,53 @@ __device__ __forceinline__ uint32_t item_cap(uint32_t total, int waves) {
 }
 
 __global__ void __launch_bounds__(256)
-k_win_item_counts(const uint32_t* __restrict__ boff, size_t stride, WinGeom wg, int waves,
+k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, int waves,
                   uint32_t* __restrict__ icount, uint32_t* __restrict__ totals) {
     const int b = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t total = boff[static_cast<size_t>(wg.nbins) * stride];
+    const uint32_t total = boff[wg.nbins];
     const uint32_t cap = item_cap(total, waves);
     if (b == 0) {
         totals[0] = total;
         totals[3] = cap;
     }
     if (b >= wg.nbins) return;
-    const uint32_t c = boff[static_cast<size_t>(b + 1) * stride] - boff[static_cast<size_t>(b) * stride];
+    const uint32_t c = boff[b + 1] - boff[b];
     icount[b] = max(1u, (c + cap - 1) / cap);
 }
 
+// One thread per item: the window is found by binary search over the item offsets.
 __global__ void __launch_bounds__(256)
-k_win_item_write(const uint32_t* __restrict__ boff, size_t stride, const uint32_t* __restrict__ ioff,
+k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
                  WinGeom wg, WItem* __restrict__ items, uint32_t max_items, uint32_t* __restrict__ totals) {
-    const int b = blockIdx.x * 256 + threadIdx.x;
-    if (b == 0) totals[1] = min(ioff[wg.nbins], max_items);
-    if (b >= wg.nbins) return;
-    const uint32_t cap = totals[3];
-    const uint32_t start = boff[static_cast<size_t>(b) * stride];
-    const uint32_t c = boff[static_cast<size_t>(b + 1) * stride] - start;
-    const uint32_t n = ioff[b + 1] - ioff[b];
-    for (uint32_t q = 0; q < n; ++q) {
-        const uint32_t slot = ioff[b] + q;
-        if (slot >= max_items) break;
-        WItem t;
-        t.bin = b;
-        t.start = static_cast<int>(start + q * cap);
-        t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
-        t.split = n > 1;
-        items[slot] = t;
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t nitems = min(ioff[wg.nbins], max_items);
+    if (i == 0) totals[1] = nitems;
+    if (i >= nitems) return;
+    int lo = 0, hi = wg.nbins;                  // largest b with ioff[b] <= i
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (ioff[mid] <= i) lo = mid;
+        else hi = mid;
     }
+    const int b = lo;
+    const uint32_t cap = totals[3];
+    const uint32_t start = boff[b];
+    const uint32_t c = boff[b + 1] - start;
+    const uint32_t q = i - ioff[b];
+    WItem t;
+    t.bin = b;
+    t.start = static_cast<int>(start + q * cap);
+    t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
+    t.split = (ioff[b + 1] - ioff[b]) > 1;
+    items[i] = t;
 }
 
 // Clear the windows accumulated by several items (their items add atomically).
 __global__ void __launch_bounds__(256)
-k_win_zero_split(const uint32_t* __restrict__ boff, size_t stride, const uint32_t* __restrict__ totals,
+k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ totals,
                  WinGeom wg, int G, int K, float2* __restrict__ planes) {
     const int bin = blockIdx.x;
-    if (boff[static_cast<size_t>(bin + 1) * stride] - boff[static_cast<size_t>(bin) * stride] <= totals[3]) return;
+    if (boff[bin + 1] - boff[bin] <= totals[3]) return;
     const int ls = bin / wg.nwin, w = bin - ls * wg.nwin;
     const int wa = w / wg.nw, wb = w - wa * wg.nw;
     const size_t plane_size = static_cast<size_t>(G) * G;
@@ -410,19 +425,16 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const size_t nbins = static_cast<size_t>(wg.nbins);
     const size_t max_items = nbins + (max_records + kMinItem - 1) / kMinItem;
     unsigned blocks;
-    if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinThreads - 1) / kBinThreads);
+    if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinRows - 1) / kBinRows);
     else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
-    // Offsets: with LDS histograms, over (window, block) pairs (stride = blocks); else per window.
-    const size_t stride = wg.use_lds ? blocks : 1;
-    const size_t nscan = nbins * stride;
-    if (nscan >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
+    const size_t nhist = wg.use_lds ? static_cast<size_t>(blocks) * nbins : 1;
 
     // counts[0, nbins) | counts[nbins + 2, +6) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 8) * sizeof(uint32_t)));
-    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nscan * sizeof(uint32_t)));
-    uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nscan + 8) * sizeof(uint32_t)));
+    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
+    uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
-    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nscan) * sizeof(uint32_t)));
+    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nbins) * sizeof(uint32_t)));
     uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* totals = counts + nbins + 2;
@@ -437,23 +449,22 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     else
         k_win_count<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
     SVDI_LAUNCH_CHECK();
-    if (wg.use_lds) launch_exclusive_scan(ctx, blockhist, boff, nullptr, sums, nscan);
-    else launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
+    launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
     const int waves = kWaves * std::max(1, ctx->cu_count);
     const unsigned bin_blocks = static_cast<unsigned>((nbins + 255) / 256);
-    k_win_item_counts<<<bin_blocks, 256, 0, ctx->stream>>>(boff, stride, wg, waves, icount, totals);
+    k_win_item_counts<<<bin_blocks, 256, 0, ctx->stream>>>(boff, wg, waves, icount, totals);
     SVDI_LAUNCH_CHECK();
     launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, nbins);
-    k_win_item_write<<<bin_blocks, 256, 0, ctx->stream>>>(boff, stride, ioff, wg, items,
-                                                          static_cast<uint32_t>(max_items), totals);
+    k_win_item_write<<<static_cast<unsigned>((max_items + 255) / 256), 256, 0, ctx->stream>>>(
+        boff, ioff, wg, items, static_cast<uint32_t>(max_items), totals);
     SVDI_LAUNCH_CHECK();
-    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, stride, totals, wg, gp.G, gp.K, planes);
+    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot)
-        k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, boff, cursor, rec, limit);
+        k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
     else
-        k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, boff, cursor, rec, limit);
+        k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
     SVDI_LAUNCH_CHECK();
 
     const size_t tok = stage_begin(ctx, 5);
