@@ -18,11 +18,31 @@ constexpr int kThreads = 256;
 constexpr int kMaxStages = 24;
 constexpr int kMaxRadix = 16;
 
+// Quotient q = n / d for 0 <= n < 2^27 and d <= 2^13 as (n * ceil(2^40 / d)) >> 40: the error
+// term n / 2^40 stays below 1 / d, so the floor is exact without an integer divide.
+struct FastDiv {
+    uint64_t mul;
+    __host__ __device__ uint32_t div(uint32_t n) const { return static_cast<uint32_t>((n * mul) >> 40); }
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.mul = ((1ull << 40) + d - 1) / d;
+    return f;
+}
+
 struct FftStages {
     int n;
     int count;
     int radix[kMaxStages];
+    FastDiv by_m[kMaxStages];      // m = n / R of the stage
+    FastDiv by_ns[kMaxStages];     // ns = product of the earlier radices
 };
+
+// LDS index of element i of a transform: one float2 of padding per 32 keeps the radix-R
+// Stockham writes (stride R) off shared banks.
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
+__host__ __device__ inline size_t padded_len(size_t n) { return n + n / 32 + 1; }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -116,67 +136,131 @@ __device__ __forceinline__ void dft_fixed(float2* v) {
     else dft8(v);
 }
 
-// One Stockham stage of radix R over ntrans transforms of length n (LDS src -> LDS dst):
-// butterfly j reads src[j + r m], twiddles by e^{+2 pi i (j mod ns) r / (ns R)}, and writes
+// One Stockham stage of radix R over ntrans transforms of length n (LDS src -> LDS dst, each
+// transform `pn` float2 apart, padded): butterfly j reads src[j + r m], twiddles by
+// e^{+2 pi i (j mod ns) r / (ns R)} (table tw in LDS: tw[t] = e^{2 pi i t / n}), and writes
 // dst[(j / ns) ns R + (j mod ns) + r ns].
 template <int R>
-__device__ __forceinline__ void stage_fixed(const float2* src, float2* dst, int ntrans, int n, int ns,
-                                            const float2* __restrict__ tw) {
+__device__ __forceinline__ void stage_fixed(const float2* src, float2* dst, int ntrans, int n, int pn,
+                                            int ns, FastDiv by_m, FastDiv by_ns, const float2* tw) {
     const int m = n / R;
     const int tstep = n / (ns * R);
     const int total = ntrans * m;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int tr = t / m;
+        const int tr = static_cast<int>(by_m.div(t));
         const int j = t - tr * m;
-        const float2* in = src + tr * n;
-        float2* out = dst + tr * n;
-        const int jm = j % ns;
+        const int jq = static_cast<int>(by_ns.div(j));
+        const int jm = j - jq * ns;
+        const float2* in = src + tr * pn;
+        float2* out = dst + tr * pn;
         float2 v[R];
-        v[0] = in[j];
+        v[0] = in[pidx(j)];
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[r] = cmul(in[j + r * m], tw[jm * r * tstep]);
+        for (int r = 1; r < R; ++r) v[r] = cmul(in[pidx(j + r * m)], tw[jm * r * tstep]);
         dft_fixed<R>(v);
-        const int d = (j / ns) * ns * R + jm;
+        const int d = jq * ns * R + jm;
 #pragma unroll
-        for (int r = 0; r < R; ++r) out[d + r * ns] = v[r];
+        for (int r = 0; r < R; ++r) out[pidx(d + r * ns)] = v[r];
     }
 }
 
-__device__ void stage_generic(const float2* src, float2* dst, int ntrans, int n, int ns, int R,
-                              const float2* __restrict__ tw) {
+__device__ void stage_generic(const float2* src, float2* dst, int ntrans, int n, int pn, int ns, int R,
+                              FastDiv by_m, FastDiv by_ns, const float2* tw) {
     const int m = n / R;
     const int tstep = n / (ns * R);
     const int total = ntrans * m;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int tr = t / m;
+        const int tr = static_cast<int>(by_m.div(t));
         const int j = t - tr * m;
-        const float2* in = src + tr * n;
-        float2* out = dst + tr * n;
-        const int jm = j % ns;
+        const int jq = static_cast<int>(by_ns.div(j));
+        const int jm = j - jq * ns;
+        const float2* in = src + tr * pn;
+        float2* out = dst + tr * pn;
         float2 v[kMaxRadix];
-        v[0] = in[j];
-        for (int r = 1; r < R; ++r) v[r] = cmul(in[j + r * m], tw[jm * r * tstep]);
+        v[0] = in[pidx(j)];
+        for (int r = 1; r < R; ++r) v[r] = cmul(in[pidx(j + r * m)], tw[jm * r * tstep]);
         dft_generic(v, R, tw, n);
-        const int d = (j / ns) * ns * R + jm;
-        for (int r = 0; r < R; ++r) out[d + r * ns] = v[r];
+        const int d = jq * ns * R + jm;
+        for (int r = 0; r < R; ++r) out[pidx(d + r * ns)] = v[r];
     }
 }
 
-// ntrans transforms of length n, stored back to back in src (LDS).  Returns the buffer that
-// holds the result (src or dst).
-__device__ float2* stockham(float2* src, float2* dst, int ntrans, const FftStages& st,
-                            const float2* __restrict__ tw) {
+// In-place form of a radix-R stage: every butterfly of the stage (at most kMaxB per thread) is
+// read and transformed into registers, then, after a barrier, written back to the same buffer.
+// Half the LDS of the ping-pong form, so twice the transforms per workgroup.
+constexpr int kMaxB = 4;
+constexpr int kInplaceThreads = 512;
+
+template <int R>
+__device__ __forceinline__ void stage_inplace(float2* buf, int ntrans, int n, int pn, int ns,
+                                              FastDiv by_m, FastDiv by_ns, const float2* tw) {
+    const int m = n / R;
+    const int tstep = n / (ns * R);
+    const int total = ntrans * m;
+    float2 v[kMaxB][R];
+    int obase[kMaxB];
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        const int t = threadIdx.x + b * kInplaceThreads;
+        obase[b] = -1;
+        if (t < total) {
+            const int tr = static_cast<int>(by_m.div(t));
+            const int j = t - tr * m;
+            const int jq = static_cast<int>(by_ns.div(j));
+            const int jm = j - jq * ns;
+            const float2* in = buf + tr * pn;
+            v[b][0] = in[pidx(j)];
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[b][r] = cmul(in[pidx(j + r * m)], tw[jm * r * tstep]);
+            dft_fixed<R>(v[b]);
+            obase[b] = (tr << 20) | (jq * ns * R + jm);     // tr < 16, index < 2^20
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxB; ++b) {
+        if (obase[b] < 0) continue;
+        float2* out = buf + (obase[b] >> 20) * pn;
+        const int d = obase[b] & 0xfffff;
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[pidx(d + r * ns)] = v[b][r];
+    }
+    __syncthreads();
+}
+
+__device__ void stockham_inplace(float2* buf, int ntrans, int pn, const FftStages& st, const float2* tw) {
     const int n = st.n;
     int ns = 1;
     for (int stage = 0; stage < st.count; ++stage) {
         const int R = st.radix[stage];
+        const FastDiv bm = st.by_m[stage], bn = st.by_ns[stage];
         switch (R) {
-            case 2: stage_fixed<2>(src, dst, ntrans, n, ns, tw); break;
-            case 3: stage_fixed<3>(src, dst, ntrans, n, ns, tw); break;
-            case 4: stage_fixed<4>(src, dst, ntrans, n, ns, tw); break;
-            case 5: stage_fixed<5>(src, dst, ntrans, n, ns, tw); break;
-            case 8: stage_fixed<8>(src, dst, ntrans, n, ns, tw); break;
-            default: stage_generic(src, dst, ntrans, n, ns, R, tw); break;
+            case 2: stage_inplace<2>(buf, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 3: stage_inplace<3>(buf, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 4: stage_inplace<4>(buf, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 5: stage_inplace<5>(buf, ntrans, n, pn, ns, bm, bn, tw); break;
+            default: stage_inplace<8>(buf, ntrans, n, pn, ns, bm, bn, tw); break;
+        }
+        ns *= R;
+    }
+}
+
+// ntrans transforms of length n, `pn` float2 apart in src (LDS, padded).  Returns the buffer
+// that holds the result (src or dst).
+__device__ float2* stockham(float2* src, float2* dst, int ntrans, int pn, const FftStages& st,
+                            const float2* tw) {
+    const int n = st.n;
+    int ns = 1;
+    for (int stage = 0; stage < st.count; ++stage) {
+        const int R = st.radix[stage];
+        const FastDiv bm = st.by_m[stage], bn = st.by_ns[stage];
+        switch (R) {
+            case 2: stage_fixed<2>(src, dst, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 3: stage_fixed<3>(src, dst, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 4: stage_fixed<4>(src, dst, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 5: stage_fixed<5>(src, dst, ntrans, n, pn, ns, bm, bn, tw); break;
+            case 8: stage_fixed<8>(src, dst, ntrans, n, pn, ns, bm, bn, tw); break;
+            default: stage_generic(src, dst, ntrans, n, pn, ns, R, bm, bn, tw); break;
         }
         __syncthreads();
         float2* tmp = src;
@@ -187,46 +271,90 @@ __device__ float2* stockham(float2* src, float2* dst, int ntrans, const FftStage
     return src;
 }
 
+// LDS of a pass: the twiddle table (n float2), then two ping-pong buffers of ntrans padded
+// transforms.
+__device__ __forceinline__ const float2* load_twiddles(float2* lds, const float2* __restrict__ tw, int n) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = tw[t];
+    return lds;
+}
+
 // Row pass: planes [P][G][G] -> row_out [P][G][cols] (output columns col_lo .. col_lo+cols-1).
-__global__ void __launch_bounds__(kThreads)
+// kInplace: radices 2/3/4/5/8 only, one LDS buffer, kInplaceThreads threads.
+template <bool kInplace>
+__global__ void __launch_bounds__(kInplace ? kInplaceThreads : kThreads)
 k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, FftStages st,
-           const float2* __restrict__ tw, int batch, int col_lo, int cols) {
+           const float2* __restrict__ tw_g, int batch, int col_lo, int cols) {
     extern __shared__ float2 lds[];
     const int G = st.n;
+    const int pn = static_cast<int>(padded_len(G));
+    const float2* tw = load_twiddles(lds, tw_g, G);
+    float2* buf0 = lds + G;
+    float2* buf1 = kInplace ? buf0 : buf0 + batch * pn;
     const size_t plane = blockIdx.y;
     const int row0 = blockIdx.x * batch;
     const int nrows = min(batch, G - row0);
     const float2* src = in + (plane * G + row0) * static_cast<size_t>(G);
-    for (int t = threadIdx.x; t < nrows * G; t += blockDim.x) lds[t] = src[t];
+    for (int r = 0; r < nrows; ++r)
+        for (int c = threadIdx.x; c < G; c += blockDim.x) buf0[r * pn + pidx(c)] = src[static_cast<size_t>(r) * G + c];
     __syncthreads();
-    const float2* res = stockham(lds, lds + batch * G, nrows, st, tw);
-    float2* dst = out + (plane * G + row0) * static_cast<size_t>(cols);
-    for (int t = threadIdx.x; t < nrows * cols; t += blockDim.x) {
-        const int r = t / cols, c = t - r * cols;
-        dst[static_cast<size_t>(r) * cols + c] = res[r * G + col_lo + c];
+    const float2* res;
+    if constexpr (kInplace) {
+        stockham_inplace(buf0, nrows, pn, st, tw);
+        res = buf0;
+    } else {
+        res = stockham(buf0, buf1, nrows, pn, st, tw);
     }
+    float2* dst = out + (plane * G + row0) * static_cast<size_t>(cols);
+    for (int r = 0; r < nrows; ++r)
+        for (int c = threadIdx.x; c < cols; c += blockDim.x)
+            dst[static_cast<size_t>(r) * cols + c] = res[r * pn + pidx(col_lo + c)];
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin in launch order, each XCD with its own L2.
+// Remapped so that consecutive logical tiles (neighbouring column strips, which share cache
+// lines) run on the same XCD at about the same time.
+// A bijection of [0, total): XCD x takes the contiguous tiles [x q + min(x, r), ...), q + 1 of
+// them for x < r and q otherwise (q = total / 8, r = total % 8).
+__device__ __forceinline__ int xcd_local_tile(int linear, int total) {
+    constexpr int kXcds = 8;
+    const int q = total / kXcds, r = total % kXcds;
+    const int x = linear % kXcds, local = linear / kXcds;
+    return x * q + min(x, r) + local;
 }
 
 // Column pass: row_out [P][G][cols] -> out [P][rows][cols] (output rows row_lo .. row_lo+rows-1).
-__global__ void __launch_bounds__(kThreads)
+// One workgroup per (plane, strip of `width` columns), strips ordered XCD-locally.
+template <bool kInplace>
+__global__ void __launch_bounds__(kInplace ? kInplaceThreads : kThreads)
 k_fft_cols(const float2* __restrict__ in, float2* __restrict__ out, FftStages st,
-           const float2* __restrict__ tw, int width, int cols, int row_lo, int rows) {
+           const float2* __restrict__ tw_g, int width, int cols, int row_lo, int rows, int strips) {
     extern __shared__ float2 lds[];
     const int G = st.n;
-    const size_t plane = blockIdx.y;
-    const int c0 = blockIdx.x * width;
+    const int pn = static_cast<int>(padded_len(G));
+    const float2* tw = load_twiddles(lds, tw_g, G);
+    float2* buf0 = lds + G;
+    float2* buf1 = kInplace ? buf0 : buf0 + width * pn;
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    const size_t plane = tile / strips;
+    const int c0 = (tile - static_cast<int>(plane) * strips) * width;
     const int nc = min(width, cols - c0);
     const float2* src = in + plane * G * static_cast<size_t>(cols) + c0;
     for (int t = threadIdx.x; t < G * nc; t += blockDim.x) {
         const int r = t / nc, c = t - r * nc;
-        lds[c * G + r] = src[static_cast<size_t>(r) * cols + c];
+        buf0[c * pn + pidx(r)] = src[static_cast<size_t>(r) * cols + c];
     }
     __syncthreads();
-    const float2* res = stockham(lds, lds + width * G, nc, st, tw);
+    const float2* res;
+    if constexpr (kInplace) {
+        stockham_inplace(buf0, nc, pn, st, tw);
+        res = buf0;
+    } else {
+        res = stockham(buf0, buf1, nc, pn, st, tw);
+    }
     float2* dst = out + plane * rows * static_cast<size_t>(cols) + c0;
     for (int t = threadIdx.x; t < rows * nc; t += blockDim.x) {
         const int r = t / nc, c = t - r * nc;
-        dst[static_cast<size_t>(r) * cols + c] = res[c * G + row_lo + r];
+        dst[static_cast<size_t>(r) * cols + c] = res[c * pn + pidx(row_lo + r)];
     }
 }
 
@@ -271,7 +399,13 @@ bool factorize(int n, FftStages& st) {
             m /= r;
         }
     }
-    return m == 1;
+    int ns = 1;
+    for (int i = 0; i < st.count; ++i) {
+        st.by_m[i] = make_fastdiv(static_cast<uint32_t>(n / st.radix[i]));
+        st.by_ns[i] = make_fastdiv(static_cast<uint32_t>(ns));
+        ns *= st.radix[i];
+    }
+    return m == 1 && n <= 8192;
 }
 
 }  // namespace
@@ -295,7 +429,9 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
     fft_prepare(ctx, G);
     const float2* tw = static_cast<const float2*>(ctx->ws.get(Workspace::kTwiddles, G * sizeof(float2)));
     FftStages st;
-    const bool ok = factorize(static_cast<int>(G), st) && G * 2 * sizeof(float2) <= 160 * 1024;
+    const size_t per = 2 * padded_len(G) * sizeof(float2);     // one transform, ping-pong
+    const size_t twb = G * sizeof(float2);
+    const bool ok = factorize(static_cast<int>(G), st) && twb + per <= 160 * 1024;
     if (!ok) {
         dim3 gr(static_cast<unsigned>((cols + 127) / 128), static_cast<unsigned>(G), static_cast<unsigned>(P));
         k_dft_rows_naive<<<gr, 128, 0, ctx->stream>>>(planes, row_out, static_cast<int>(G), tw,
@@ -308,21 +444,42 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
         SVDI_LAUNCH_CHECK();
         return;
     }
-    const size_t per = 2 * G * sizeof(float2);
-    const int batch = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per)));
-    const int width = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, (96 * 1024) / per)));
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_rows),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(batch * per)));
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(width * per)));
+    // In-place form when every radix has a fixed kernel and a stage's butterflies for at least
+    // one transform fit kMaxB per thread: transforms per workgroup = min over stages of
+    // kMaxB * threads / m, capped by 64 KB of LDS.
+    bool inplace = true;
+    int fit = 16;
+    for (int i = 0; i < st.count; ++i) {
+        const int R = st.radix[i];
+        if (R != 2 && R != 3 && R != 4 && R != 5 && R != 8) inplace = false;
+        fit = std::min(fit, kMaxB * kInplaceThreads / (static_cast<int>(G) / R));
+    }
+    const size_t one = padded_len(G) * sizeof(float2);
+    if (inplace && fit >= 1) {
+        fit = static_cast<int>(std::max<size_t>(1, std::min<size_t>(fit, (64 * 1024 > twb ? (64 * 1024 - twb) / one : 1))));
+    } else {
+        inplace = false;
+        const size_t budget = 128 * 1024;
+        const size_t room = budget > twb ? (budget - twb) / per : 0;
+        fit = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, room)));
+    }
+    const int batch = fit, width = fit;
+    const size_t lds = twb + static_cast<size_t>(fit) * (inplace ? one : per);
+    const int threads = inplace ? kInplaceThreads : kThreads;
+    auto rows_fn = inplace ? &k_fft_rows<true> : &k_fft_rows<false>;
+    auto cols_fn = inplace ? &k_fft_cols<true> : &k_fft_cols<false>;
+    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rows_fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cols_fn),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     dim3 gr(static_cast<unsigned>((G + batch - 1) / batch), static_cast<unsigned>(P));
-    k_fft_rows<<<gr, kThreads, batch * per, ctx->stream>>>(planes, row_out, st, tw, batch,
-                                                           static_cast<int>(col_lo), static_cast<int>(cols));
+    rows_fn<<<gr, threads, lds, ctx->stream>>>(planes, row_out, st, tw, batch, static_cast<int>(col_lo),
+                                               static_cast<int>(cols));
     SVDI_LAUNCH_CHECK();
-    dim3 gc(static_cast<unsigned>((cols + width - 1) / width), static_cast<unsigned>(P));
-    k_fft_cols<<<gc, kThreads, width * per, ctx->stream>>>(row_out, out, st, tw, width,
-                                                           static_cast<int>(cols), static_cast<int>(row_lo),
-                                                           static_cast<int>(rows));
+    const int strips = static_cast<int>((cols + width - 1) / width);
+    const unsigned gc = static_cast<unsigned>(strips * P);
+    cols_fn<<<gc, threads, lds, ctx->stream>>>(row_out, out, st, tw, width, static_cast<int>(cols),
+                                               static_cast<int>(row_lo), static_cast<int>(rows), strips);
     SVDI_LAUNCH_CHECK();
 }
 

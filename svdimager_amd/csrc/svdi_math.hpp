@@ -88,7 +88,11 @@ SVDI_HD void bessel_j01_asymptotic(double x, double* j0, double* j1) {
     *j1 = amp * (p1 * cos(c1) - q1 * sin(c1));
 }
 
-// J_0 .. J_{count-1}(x) in FP64.  |x| < 1: the power series per order; |x| >= 1000: forward
+// Below this |x| the power series converges in a few terms (|x/2|^2 <= 2.5e-5 per term); above
+// it Miller's recurrence is cheaper and as accurate.
+constexpr double kBesselSeriesMax = 1e-2;
+
+// J_0 .. J_{count-1}(x) in FP64.  |x| < 1e-2: the power series per order; |x| >= 1000: forward
 // recurrence from the asymptotic J_0, J_1 (stable while n < x); otherwise Miller's backward
 // recurrence normalised by J_0 + 2 sum_k J_{2k} = 1.  Replaces the FP32 power series of
 // bessel_sequence (SVDIntegratedImager.cu:901-920), which loses digits to cancellation near x = 10.
@@ -99,7 +103,7 @@ SVDI_HD void bessel_j01_asymptotic(double x, double* j0, double* j1) {
 template <int COUNT>
 SVDI_HD void bessel_sequence_fixed(double x, double* j) {
     const double ax = fabs(x);
-    if (ax < 1.0) {
+    if (ax < kBesselSeriesMax) {
 #pragma unroll
         for (int n = 0; n < COUNT; ++n) j[n] = bessel_series(n, ax);
     } else if (ax >= 1000.0) {
@@ -113,11 +117,12 @@ SVDI_HD void bessel_sequence_fixed(double x, double* j) {
         const double big = 1e150, small = 1e-150;
         const int top = static_cast<int>(fmax(static_cast<double>(COUNT), ax));
         const int m = 2 * ((top + 20 + static_cast<int>(sqrt(60.0 * top))) / 2);   // > COUNT
+        const double two_over_x = 2.0 / ax;
         double jp = 0.0, jc = 1.0, norm = 0.0;
 #pragma unroll
         for (int n = 0; n < COUNT; ++n) j[n] = 0.0;
         for (int k = m; k > COUNT; --k) {
-            const double jm = (2.0 * k / ax) * jc - jp;   // J_{k-1}
+            const double jm = (k * two_over_x) * jc - jp;   // J_{k-1}
             jp = jc;
             jc = jm;
             if (fabs(jc) > big) {
@@ -130,7 +135,7 @@ SVDI_HD void bessel_sequence_fixed(double x, double* j) {
         }
 #pragma unroll
         for (int k = COUNT; k > 0; --k) {
-            const double jm = (2.0 * k / ax) * jc - jp;
+            const double jm = (k * two_over_x) * jc - jp;
             jp = jc;
             jc = jm;
             if (fabs(jc) > big) {
@@ -156,7 +161,7 @@ SVDI_HD void bessel_sequence_fixed(double x, double* j) {
 
 SVDI_HD void bessel_sequence_f64(double x, int count, double* j) {
     const double ax = fabs(x);
-    if (ax < 1.0) {
+    if (ax < kBesselSeriesMax) {
         for (int n = 0; n < count; ++n) j[n] = bessel_series(n, ax);
     } else if (ax >= 1000.0) {
         double j0, j1;
@@ -168,10 +173,11 @@ SVDI_HD void bessel_sequence_f64(double x, int count, double* j) {
         const double big = 1e150, small = 1e-150;
         const int top = static_cast<int>(fmax(static_cast<double>(count), ax));
         int m = 2 * ((top + 20 + static_cast<int>(sqrt(60.0 * top))) / 2);
+        const double two_over_x = 2.0 / ax;
         double jp = 0.0, jc = 1.0, norm = 0.0;
         for (int n = 0; n < count; ++n) j[n] = 0.0;
         for (int k = m; k > 0; --k) {
-            const double jm = (2.0 * k / ax) * jc - jp;   // J_{k-1}
+            const double jm = (k * two_over_x) * jc - jp;   // J_{k-1}
             jp = jc;
             jc = jm;
             if (fabs(jc) > big) {

@@ -36,8 +36,8 @@ constexpr int kWaves = 12;               // wavefronts per workgroup of the wind
 constexpr int kChunk = 32;               // samples per MFMA reduction step
 constexpr int kStride = 36;              // floats per staging row (32 samples + pad)
 constexpr int kStageFloats = 48 * kStride;   // ku[16], kv[16], c[16] rows per wavefront
-constexpr int kMinItem = 1024;
-constexpr int kMaxItem = 8192;
+constexpr int kMinItem = 2048;
+constexpr int kMaxItem = 16384;
 constexpr float kEsScale = 128.0f;       // 2^7 per axis
 constexpr int kEsScaleLog2Sq = 14;
 
@@ -180,12 +180,14 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
     });
 }
 
-// Work items over the windows, at most `cap` records each, cap = clamp(total / (4 x wavefronts),
-// kMinItem, kMaxItem); every window gets at least one item so every window is written.  Window
-// b's records are [boff[b], boff[b + 1]).
+// Work items over the windows: a window with at most `cap` records is one item; a larger one is
+// split into items of `cap` records that add their partial windows atomically.  cap =
+// clamp(total / wavefronts, kMinItem, kMaxItem).  The split windows' items come first in the
+// queue, so the largest items start early and the single-item windows fill in behind them.
+// icount[b] = items of split window b, icount[nbins + b] = 1 for an unsplit window b.
 // totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
 __device__ __forceinline__ uint32_t item_cap(uint32_t total, int waves) {
-    const uint32_t fair = (total + 4u * waves - 1) / (4u * waves);
+    const uint32_t fair = (total + waves - 1) / waves;
     return min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
 }
 
@@ -201,33 +203,36 @@ k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, int waves,
     }
     if (b >= wg.nbins) return;
     const uint32_t c = boff[b + 1] - boff[b];
-    icount[b] = max(1u, (c + cap - 1) / cap);
+    icount[b] = c > cap ? (c + cap - 1) / cap : 0u;
+    icount[wg.nbins + b] = c > cap ? 0u : 1u;
 }
 
-// One thread per item: the window is found by binary search over the item offsets.
+// One thread per item: its entry of the item counts is found by binary search over the offsets.
 __global__ void __launch_bounds__(256)
 k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
                  WinGeom wg, WItem* __restrict__ items, uint32_t max_items, uint32_t* __restrict__ totals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t nitems = min(ioff[wg.nbins], max_items);
+    const int n = 2 * wg.nbins;
+    const uint32_t nitems = min(ioff[n], max_items);
     if (i == 0) totals[1] = nitems;
     if (i >= nitems) return;
-    int lo = 0, hi = wg.nbins;                  // largest b with ioff[b] <= i
+    int lo = 0, hi = n;                         // largest j with ioff[j] <= i
     while (hi - lo > 1) {
         const int mid = (lo + hi) / 2;
         if (ioff[mid] <= i) lo = mid;
         else hi = mid;
     }
-    const int b = lo;
+    const bool split = lo < wg.nbins;
+    const int b = split ? lo : lo - wg.nbins;
     const uint32_t cap = totals[3];
     const uint32_t start = boff[b];
     const uint32_t c = boff[b + 1] - start;
-    const uint32_t q = i - ioff[b];
+    const uint32_t q = i - ioff[lo];
     WItem t;
     t.bin = b;
     t.start = static_cast<int>(start + q * cap);
     t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
-    t.split = (ioff[b + 1] - ioff[b]) > 1;
+    t.split = split ? 1 : 0;
     items[i] = t;
 }
 
@@ -434,9 +439,9 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
-    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(nbins) * sizeof(uint32_t)));
-    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (nbins + 8) * sizeof(uint32_t)));
-    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (nbins + 8) * sizeof(uint32_t)));
+    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(2 * nbins) * sizeof(uint32_t)));
+    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (2 * nbins + 8) * sizeof(uint32_t)));
+    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (2 * nbins + 8) * sizeof(uint32_t)));
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
@@ -454,7 +459,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const unsigned bin_blocks = static_cast<unsigned>((nbins + 255) / 256);
     k_win_item_counts<<<bin_blocks, 256, 0, ctx->stream>>>(boff, wg, waves, icount, totals);
     SVDI_LAUNCH_CHECK();
-    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, nbins);
+    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, 2 * nbins);
     k_win_item_write<<<static_cast<unsigned>((max_items + 255) / 256), 256, 0, ctx->stream>>>(
         boff, ioff, wg, items, static_cast<uint32_t>(max_items), totals);
     SVDI_LAUNCH_CHECK();

@@ -116,6 +116,8 @@ PROBE = r"""
 #include "svdi_math.hpp"
 extern "C" {
 void p_bessel(double x, int n, double* j) { svdi::bessel_sequence_f64(x, n, j); }
+void p_bessel_fixed8(double x, double* j) { svdi::bessel_sequence_fixed<8>(x, j); }
+void p_bessel_fixed32(double x, double* j) { svdi::bessel_sequence_fixed<32>(x, j); }
 void p_eigen(const double* a, double* ev, double* vec) { svdi::eigen_sym3(a, ev, vec); }
 void p_basis(const double* vec, double acx, double* basis) { svdi::oriented_basis(vec, acx, basis); }
 float p_es(float x) { return svdi::es_kernel(x); }
@@ -139,6 +141,8 @@ def probe(tmp_path_factory):
     dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     P.p_bessel.argtypes = [ctypes.c_double, ctypes.c_int, dp]
+    P.p_bessel_fixed8.argtypes = [ctypes.c_double, dp]
+    P.p_bessel_fixed32.argtypes = [ctypes.c_double, dp]
     P.p_eigen.argtypes = [dp, dp, dp]
     P.p_basis.argtypes = [dp, ctypes.c_double, dp]
     P.p_es.argtypes = [ctypes.c_float]
@@ -148,12 +152,24 @@ def probe(tmp_path_factory):
     return P
 
 
-@pytest.mark.parametrize("x", [0.0, 1e-9, 0.3, 0.999, 1.0, 4.2, 10.0, 37.5, -6.5, 999.0, 1000.0, 2.5e4])
+BESSEL_X = [0.0, 1e-9, 0.005, 0.00999, 0.01, 0.02, 0.3, 0.999, 1.0, 4.2, 10.0, 37.5, -6.5, -0.05,
+            999.0, 1000.0, 2.5e4]
+
+
+@pytest.mark.parametrize("x", BESSEL_X)
 def test_device_bessel_matches_scipy(probe, x):
-    j = np.zeros(32)
-    probe.p_bessel(x, 32, j)
+    """J_0..J_31 against scipy: absolute error (the image pass weights J_k against moments of
+    comparable size), plus relative error for the orders above 1e-30."""
     ref = special.jv(np.arange(32), x)
-    assert np.abs(j - ref).max() < 1e-12 + 1e-10 * np.abs(ref).max()
+    for fn, n in ((lambda j: probe.p_bessel(x, 32, j), 32), (probe.p_bessel_fixed32, 32),
+                  (probe.p_bessel_fixed8, 8)):
+        j = np.zeros(n)
+        fn(x, j) if fn in (probe.p_bessel_fixed32, probe.p_bessel_fixed8) else fn(j)
+        r = ref[:n]
+        assert np.abs(j - r).max() < 1e-12 + 1e-10 * np.abs(r).max(), (x, n)
+        big = np.abs(r) > 1e-30
+        if abs(x) < 1000:
+            assert np.all(np.abs(j[big] - r[big]) <= 1e-10 * np.abs(r[big])), (x, n)
 
 
 def test_device_eigen_and_basis_match_oracle(probe):
