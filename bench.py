@@ -1,11 +1,13 @@
 #!/usr/bin/env python3
 """Throughput of the snapshot imager on one MI355X.
 
-One step = one SKA-Low-like snapshot (512 stations log-radial within 35 km -> 130,816 rows,
-64 channels 50-350 MHz -> 8,372,224 samples, 10% flags, 1024^2 image, cell 2e-5 rad) imaged
-from raw device-resident visibilities (uvw, frequencies, XX/YY, flags, WEIGHT_SPECTRUM) to the
-dirty image in HBM: FP64 rotation, planning, ES gridding of the Chebyshev moments, 2D inverse
-FFT, image-plane w-correction, SIN splat.  Prints one JSON line.
+Workload: SKA-Low-like snapshots (512 stations log-radial within 35 km -> 130,816 rows, 64
+channels 50-350 MHz -> 8,372,224 samples, 10% flags, 1024^2 image, cell 2e-5 rad) imaged from
+raw device-resident visibilities (uvw, frequencies, XX/YY, flags, WEIGHT_SPECTRUM) to the dirty
+image in HBM: FP64 rotation, planning, ES gridding of the Chebyshev moments, 2D inverse FFT,
+image-plane w-correction, SIN splat.  One step = one batched call imaging `--batch` (8)
+consecutive snapshots of the observation, 10 s apart, on concurrent lanes; `value` counts
+snapshots.  Single-snapshot calls are timed too.  Prints one JSON line.
 
 The headline gridder is the MFMA one (each 16 x 16 grid window contracted over its samples on
 f16 hi/lo MFMA with FP32 accumulation); the plain FP32 tiled imager is timed in the same run on
@@ -13,7 +15,8 @@ the same snapshot.  The accuracy of both against an FP64 direct DFT is measured 
 pixels plus the peak pixel of the signed, pre-reprojection image.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--gridder mfma|tiled|atomic|auto]
-                  [--no-compare] [--no-accuracy] [--no-cpu-baseline] [--cpu-samples N]
+                  [--batch B] [--no-compare] [--no-accuracy] [--no-cpu-baseline]
+                  [--cpu-samples N]
   python bench.py --copy-calibration MIB     (plain copy, the PMC byte counters' reference)
 """
 from __future__ import annotations
@@ -196,6 +199,9 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8,
+                    help="snapshots per batched call (consecutive 10 s snapshots of the same "
+                         "observation); 0 = time single-snapshot calls only")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the "
                          "reference reading for the FETCH_SIZE / WRITE_SIZE counters")
@@ -238,8 +244,34 @@ def main():
     elapsed, stages, plan = time_gridder(S, ctx, dsnap, cfg, image, gridder, args.steps,
                                          args.warmup, args.gridder)
     log(f"bench: plan {plan}")
+    single_snaps_per_s = args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
-    snaps_per_s = args.steps / elapsed
+    snaps_per_s = single_snaps_per_s
+    batch = max(0, args.batch)
+    batched = None
+    if batch > 0:
+        # The snapshot above plus batch - 1 followers, 10 s apart in hour angle.
+        dsnaps = [dsnap]
+        for k in range(1, batch):
+            s_k = synth.make_snapshot(cfg, ha_hours=cfg.ha_hours + k * 10.0 / 3600.0, device="cuda")
+            dsnaps.append(S.DeviceSnapshot(s_k, "cuda"))
+            del s_k
+        log(f"bench: {batch} snapshots resident")
+        images = [torch.empty_like(image) for _ in range(batch)]
+        for i in range(args.warmup):
+            S.image_snapshot_batch(ctx, dsnaps, cfg.image_size, cfg.cell, images=images, gridder=gridder)
+            log(f"bench[batch {batch}]: warmup {i + 1}/{args.warmup} done")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            S.image_snapshot_batch(ctx, dsnaps, cfg.image_size, cfg.cell, images=images, gridder=gridder)
+        torch.cuda.synchronize()
+        el_b = time.perf_counter() - t0
+        log(f"bench[batch {batch}]: {args.steps} steps in {el_b:.3f} s")
+        batched = {"snapshots_per_s": args.steps * batch / el_b, "ms_per_step": el_b * 1e3 / args.steps}
+        snaps_per_s = batched["snapshots_per_s"]
+        ms_per_step = batched["ms_per_step"]
+        del dsnaps, images
     K, P = plan["num_terms"], plan["num_planes"]
     alg = algorithmic(cfg, S_eff, K, P)
 
@@ -273,8 +305,10 @@ def main():
                    "channels": cfg.n_channels, "samples": cfg.num_samples,
                    "image_size": cfg.image_size, "grid_size": cfg.grid_size, "cell": cfg.cell,
                    "num_terms": K, "num_slabs": plan["num_slabs"],
-                   "gridder": args.gridder, "parallelism": "single-gpu"},
+                   "gridder": args.gridder, "batch": batch, "parallelism": "single-gpu"},
         "mvis_per_s": cfg.num_samples * snaps_per_s / 1e6,
+        "single_snapshot": {"snapshots_per_s": single_snaps_per_s,
+                            "ms_per_snapshot": elapsed * 1e3 / args.steps},
         "stages_ms": stages,
         "records": plan["records"],
         "path_hbm_bytes_algorithmic": hbm_bytes,
@@ -289,6 +323,7 @@ def main():
                              "mvis_per_s": cfg.num_samples * args.steps / el2 / 1e6,
                              "stages_ms": st2}
         out["speedup_vs_plain_fp32"] = snaps_per_s / (args.steps / el2)
+        out["speedup_single_vs_plain_fp32"] = single_snaps_per_s / (args.steps / el2)
     if not args.no_accuracy:
         log("bench: accuracy against the FP64 DFT ...")
         gl = {args.gridder: gridder}

@@ -152,6 +152,16 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snapshot, float*
                         size_t image_size, float cell_size, const svdi_options* opts,
                         svdi_plan_info* info);
 
+/* Batched svdi_image_snapshot (no counterpart in the reference, which images one snapshot per
+ * process): `count` device-resident snapshots imaged into images[i] (device pointers,
+ * image_size^2 floats each), up to four at a time on concurrent streams with their own
+ * workspaces, so that one snapshot's latency-bound phases overlap another's.  Work starts after
+ * the caller's earlier work on the context stream and the call returns when every image is
+ * done.  infos (count entries) may be NULL; their elapsed_ms and records are 0. */
+int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snapshots, size_t count,
+                              float* const* images, size_t image_size, float cell_size,
+                              const svdi_options* opts, svdi_plan_info* infos);
+
 /* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
  * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

@@ -139,31 +139,60 @@ void finish_call(svdi_context* ctx) {
     }
 }
 
-// The imaging core shared by every entry point.  `in` describes the samples; the image
-// (N x N floats, device) is overwritten.
-void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
-                const svdi_options& o, svdi_plan_info* info, float* image) {
+// Concurrent imaging lanes of a batched call.
+constexpr size_t kBatchLanes = 4;
+
+// Host-side state of one imaging call between its phases.
+struct CallState {
+    HostPlan hp;
+    size_t N = 0;
+    float cell = 0.0f;
+    size_t records = 0;
+    const uint32_t* tile_totals = nullptr;
+};
+
+// Phase 1 of an imaging call: validate, enqueue the planning kernels and the read-back of the
+// plan and the rotation rows into page-locked memory, and record ev_plan.  `in` describes the
+// samples.
+void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell, const svdi_options& o,
+                 CallState& cs) {
     if (N == 0) throw ArgError("Image_Size must be greater than zero");
     if (!(cell > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
     if (in.num_samples == 0) throw ArgError("Measurement set is empty");
     if (o.num_terms < 0 || o.num_terms > kMaxTerms) throw ArgError("num_terms must be in 0..32");
     if (N > 46340) throw ArgError("Image_Size too large");
+    if (o.gridder != SVDI_GRIDDER_AUTO && o.gridder != SVDI_GRIDDER_ATOMIC && o.gridder != SVDI_GRIDDER_TILED &&
+        o.gridder != SVDI_GRIDDER_MFMA)
+        throw ArgError("unknown gridder");
     const size_t G = (N * 3 + 1) / 2;                      // ceiling_divide(image_size * 3, 2)
-    HostPlan hp;
-    hp.semantics = o.semantics == SVDI_SEMANTICS_ARCHIVE ? Semantics::Archive : Semantics::Integrated;
-    hp.image_size = N;
-    hp.grid_size = G;
-    hp.cell = cell;
+    cs = CallState();
+    cs.N = N;
+    cs.cell = cell;
+    cs.hp.semantics = o.semantics == SVDI_SEMANTICS_ARCHIVE ? Semantics::Archive : Semantics::Integrated;
+    cs.hp.image_size = N;
+    cs.hp.grid_size = G;
+    cs.hp.cell = cell;
     const float target_alpha = o.target_alpha > 0.0f ? o.target_alpha : env_target_alpha();
 
     DevPlan* dplan = static_cast<DevPlan*>(ctx->ws.get(Workspace::kPlan, sizeof(DevPlan)));
     SVDI_HIP(hipEventRecord(ctx->ev_start, ctx->stream));
-    size_t tok = stage_begin(ctx, 1);
-    launch_plan(ctx, in, hp.semantics, N, cell, target_alpha, dplan);
+    const size_t tok = stage_begin(ctx, 1);
+    launch_plan(ctx, in, cs.hp.semantics, N, cell, target_alpha, dplan);
     stage_end(ctx, tok);
-    SVDI_HIP(hipMemcpyAsync(ctx->pinned_plan, dplan, sizeof(DevPlan), hipMemcpyDeviceToHost, ctx->stream));
-    SVDI_HIP(hipStreamSynchronize(ctx->stream));
-    hp.dev = *ctx->pinned_plan;
+    const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
+    SVDI_HIP(hipMemcpyAsync(&ctx->pinned->plan, dplan, sizeof(DevPlan), hipMemcpyDeviceToHost, ctx->stream));
+    SVDI_HIP(hipMemcpyAsync(ctx->pinned->vin, vin_dev, sizeof ctx->pinned->vin, hipMemcpyDeviceToHost, ctx->stream));
+    SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
+}
+
+// Phase 2, once ev_plan has completed: choose K and the slab batches on the host, then enqueue
+// gridding, FFT and image pass per slab batch, and record ev_stop.  The image (N x N floats,
+// device) is overwritten.
+void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+    HostPlan& hp = cs.hp;
+    const size_t N = cs.N, G = hp.grid_size;
+    const float cell = cs.cell;
+    hp.dev = ctx->pinned->plan;
 
     // K: bucket rule (SVDIntegratedImager.cu:1577-1603) or smallest passing K (FIPkernels.cu:301-318).
     if (hp.semantics == Semantics::Integrated) {
@@ -230,18 +259,8 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     gp.r3_centre = hp.dev.r3_centre;
     gp.scale_f32 = cell * static_cast<float>(G);
     gp.scale_f64 = static_cast<double>(cell) * static_cast<double>(G);
-    float det;
-    if (in.from_snapshot) {
-        DevGeometry g;
-        SVDI_HIP(hipMemcpyAsync(&g, in.geo, sizeof g, hipMemcpyDeviceToHost, ctx->stream));
-        SVDI_HIP(hipStreamSynchronize(ctx->stream));
-        det = g.det;
-    } else {
-        float v[9];
-        SVDI_HIP(hipMemcpyAsync(v, in.vin_dev, sizeof v, hipMemcpyDeviceToHost, ctx->stream));
-        SVDI_HIP(hipStreamSynchronize(ctx->stream));
-        det = std::fabs(v[0] * v[4] - v[1] * v[3]);
-    }
+    const float* v = ctx->pinned->vin;
+    const float det = std::fabs(v[0] * v[4] - v[1] * v[3]);
     gp.det = det;
     gp.inv_det = 1.0f / det;
 
@@ -263,20 +282,16 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     ip.signed_out = o.output == SVDI_OUTPUT_SIGNED;
     const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
 
-    const uint32_t* tile_totals = nullptr;
     const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_TILED : o.gridder;
-    if (gridder != SVDI_GRIDDER_ATOMIC && gridder != SVDI_GRIDDER_TILED && gridder != SVDI_GRIDDER_MFMA)
-        throw ArgError("unknown gridder");
-
     for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
         gp.nslabs = static_cast<int>(nb);
-        tok = stage_begin(ctx, 2);
+        size_t tok = stage_begin(ctx, 2);
         if (gridder == SVDI_GRIDDER_MFMA) {
-            tile_totals = launch_grid_windows(ctx, in, gp, planes);
+            cs.tile_totals = launch_grid_windows(ctx, in, gp, planes);
         } else if (gridder == SVDI_GRIDDER_TILED) {
-            tile_totals = launch_grid_tiled(ctx, in, gp, planes);
+            cs.tile_totals = launch_grid_tiled(ctx, in, gp, planes);
         } else {
             launch_grid_atomic(ctx, in, gp, planes);
         }
@@ -293,38 +308,54 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
         stage_end(ctx, tok);
     }
     SVDI_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+}
+
+void fill_info(const CallState& cs, float ms, svdi_plan_info* info) {
+    if (!info) return;
+    const HostPlan& hp = cs.hp;
+    info->r3_min = hp.dev.r3_min;
+    info->r3_max = hp.dev.r3_max;
+    info->max_abs_t3 = hp.dev.max_abs_t3;
+    info->valid_pixels = hp.dev.valid_pixels;
+    info->alpha_max = hp.dev.alpha_max;
+    info->target_alpha = hp.dev.target_alpha;
+    info->num_slabs = hp.dev.num_slabs;
+    info->slab_width = hp.dev.slab_width;
+    info->slab_half_range = hp.dev.slab_half_range;
+    info->slab_alpha_max = hp.dev.slab_alpha_max;
+    info->num_terms = hp.num_terms;
+    info->term_error = hp.term_error;
+    for (int i = 0; i < 4; ++i) info->bucket_errors[i] = bits_to_float(hp.dev.bucket_err_bits[i]);
+    info->batch_slabs = hp.batch_slabs;
+    info->grid_size = hp.grid_size;
+    info->num_planes = static_cast<size_t>(hp.dev.num_slabs) * static_cast<size_t>(hp.num_terms);
+    info->records = cs.records;
+    info->elapsed_ms = ms;
+}
+
+// Phase 3: wait for the call's work, then report.
+void image_complete(svdi_context* ctx, const svdi_options& o, CallState& cs, svdi_plan_info* info) {
     SVDI_HIP(hipEventSynchronize(ctx->ev_stop));
     finish_call(ctx);
     float ms = 0.0f;
     SVDI_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    size_t records = 0;
-    if (tile_totals != nullptr) {
+    if (cs.tile_totals != nullptr) {
         uint32_t tot[2] = {0, 0};
-        SVDI_HIP(hipMemcpy(tot, tile_totals, sizeof tot, hipMemcpyDeviceToHost));
-        records = tot[0];                                  // last slab batch
+        SVDI_HIP(hipMemcpy(tot, cs.tile_totals, sizeof tot, hipMemcpyDeviceToHost));
+        cs.records = tot[0];                               // last slab batch
     }
     if (o.verbose) std::cout << "Time elapsed: " << ms << " ms" << std::endl;
+    fill_info(cs, ms, info);
+}
 
-    if (info) {
-        info->r3_min = hp.dev.r3_min;
-        info->r3_max = hp.dev.r3_max;
-        info->max_abs_t3 = hp.dev.max_abs_t3;
-        info->valid_pixels = hp.dev.valid_pixels;
-        info->alpha_max = hp.dev.alpha_max;
-        info->target_alpha = hp.dev.target_alpha;
-        info->num_slabs = hp.dev.num_slabs;
-        info->slab_width = hp.dev.slab_width;
-        info->slab_half_range = hp.dev.slab_half_range;
-        info->slab_alpha_max = hp.dev.slab_alpha_max;
-        info->num_terms = K;
-        info->term_error = hp.term_error;
-        for (int i = 0; i < 4; ++i) info->bucket_errors[i] = bits_to_float(hp.dev.bucket_err_bits[i]);
-        info->batch_slabs = hp.batch_slabs;
-        info->grid_size = G;
-        info->num_planes = total_slabs * static_cast<size_t>(K);
-        info->records = records;
-        info->elapsed_ms = ms;
-    }
+// One imaging call on one context: the three phases back to back.
+void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
+                const svdi_options& o, svdi_plan_info* info, float* image) {
+    CallState cs;
+    image_begin(ctx, in, N, cell, o, cs);
+    SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
+    image_enqueue(ctx, in, o, cs, image);
+    image_complete(ctx, o, cs, info);
 }
 
 svdi_options resolve(const svdi_options* o, int default_semantics) {
@@ -433,7 +464,40 @@ size_t Workspace::bytes_held() const {
     return t;
 }
 
+std::unique_ptr<svdi_context> create_context(int device) {
+    int count = 0;
+    SVDI_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw ArgError("no such HIP device");
+    SVDI_HIP(hipSetDevice(device));
+    std::unique_ptr<svdi_context> c(new svdi_context());
+    c->device = device;
+    SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
+    SVDI_HIP(hipEventCreate(&c->ev_start));
+    SVDI_HIP(hipEventCreate(&c->ev_stop));
+    SVDI_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
+    SVDI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), sizeof(PinnedPlan), hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    SVDI_HIP(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    return c;
+}
+
 }  // namespace svdi
+
+svdi_context::~svdi_context() {
+    (void)hipSetDevice(device);
+    lanes.clear();
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (own_stream && own_stream != stream) (void)hipStreamSynchronize(own_stream);
+    ws.release();
+    if (pinned) (void)hipHostFree(pinned);
+    if (ev_start) (void)hipEventDestroy(ev_start);
+    if (ev_stop) (void)hipEventDestroy(ev_stop);
+    if (ev_plan) (void)hipEventDestroy(ev_plan);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+}
 
 using namespace svdi;
 
@@ -455,35 +519,13 @@ int svdi_create(svdi_context** out, int device) {
     return guarded([&] {
         if (!out) throw ArgError("ctx is null");
         *out = nullptr;
-        int count = 0;
-        SVDI_HIP(hipGetDeviceCount(&count));
-        if (device < 0 || device >= count) throw ArgError("no such HIP device");
-        SVDI_HIP(hipSetDevice(device));
-        std::unique_ptr<svdi_context> c(new svdi_context());
-        c->device = device;
-        SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
-        c->stream = c->own_stream;
-        SVDI_HIP(hipEventCreate(&c->ev_start));
-        SVDI_HIP(hipEventCreate(&c->ev_stop));
-        SVDI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned_plan), sizeof(DevPlan), hipHostMallocDefault));
-        hipDeviceProp_t prop;
-        SVDI_HIP(hipGetDeviceProperties(&prop, device));
-        c->cu_count = prop.multiProcessorCount;
-        *out = c.release();
+        *out = create_context(device).release();
     });
 }
 
 int svdi_destroy(svdi_context* ctx) {
     return guarded([&] {
         if (!ctx) return;
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);
-        ctx->ws.release();
-        if (ctx->pinned_plan) (void)hipHostFree(ctx->pinned_plan);
-        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
-        if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
-        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-        for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
         delete ctx;
     });
 }
@@ -576,6 +618,56 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snap, float* ima
         begin_call(ctx);
         SampleInput in = snapshot_input(ctx, *snap);
         image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_INTEGRATED), info, image);
+    });
+}
+
+int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
+                              float* const* images, size_t image_size, float cell_size,
+                              const svdi_options* opts, svdi_plan_info* infos) {
+    return guarded([&] {
+        if (!ctx || !snaps || !images) throw ArgError("ctx / snapshots / images is null");
+        for (size_t i = 0; i < count; ++i) check_device_pointer(images[i], "image");
+        if (count == 0) return;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        const size_t L = std::min(count, kBatchLanes);
+        while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
+        // The lanes start after the caller's earlier work on the context stream, and the context
+        // stream continues after the last lane's work.
+        hipEvent_t ready = nullptr;
+        SVDI_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        std::unique_ptr<void, void (*)(void*)> ready_guard(ready, [](void* e) {
+            (void)hipEventDestroy(static_cast<hipEvent_t>(e));
+        });
+        SVDI_HIP(hipEventRecord(ready, ctx->stream));
+        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ready, 0));
+        std::vector<CallState> cs(count);
+        std::vector<SampleInput> in(count);
+        // Rounds of L snapshots, one per lane: every lane's planning is enqueued first, then each
+        // lane's gridding as soon as its plan is back, while the other lanes keep the GPU busy.
+        for (size_t r0 = 0; r0 < count; r0 += L) {
+            const size_t n = std::min(L, count - r0);
+            for (size_t j = 0; j < n; ++j) {
+                svdi_context* lane = ctx->lanes[j].get();
+                begin_call(lane);
+                in[r0 + j] = snapshot_input(lane, snaps[r0 + j]);
+                image_begin(lane, in[r0 + j], image_size, cell_size, o, cs[r0 + j]);
+            }
+            for (size_t j = 0; j < n; ++j) {
+                svdi_context* lane = ctx->lanes[j].get();
+                SVDI_HIP(hipEventSynchronize(lane->ev_plan));
+                image_enqueue(lane, in[r0 + j], o, cs[r0 + j], images[r0 + j]);
+            }
+        }
+        for (size_t j = 0; j < L; ++j) {
+            svdi_context* lane = ctx->lanes[j].get();
+            SVDI_HIP(hipEventRecord(lane->ev_stop, lane->stream));
+            SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_stop, 0));
+        }
+        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamSynchronize(ctx->lanes[j]->stream));
+        if (infos)
+            for (size_t i = 0; i < count; ++i) fill_info(cs[i], 0.0f, &infos[i]);
     });
 }
 

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -129,13 +130,24 @@ class Workspace {
 
 }  // namespace svdi
 
+namespace svdi {
+// What the host reads back between planning and gridding, in one page-locked block.
+struct PinnedPlan {
+    DevPlan plan;
+    float vin[9];           // the rotation rows in FP32 (|det| of the first two rows is the vis scale)
+};
+}  // namespace svdi
+
 struct svdi_context {
     int device = 0;
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_plan = nullptr;
     svdi::Workspace ws;
-    svdi::DevPlan* pinned_plan = nullptr;   // page-locked mirror of the device plan
+    svdi::PinnedPlan* pinned = nullptr;      // page-locked read-back of the plan
+    // Batched calls: child contexts, each with its own stream and workspace, that image
+    // snapshots concurrently.
+    std::vector<std::unique_ptr<svdi_context>> lanes;
     std::mutex mu;
     size_t twiddle_n = 0;
     int cu_count = 0;
@@ -145,6 +157,10 @@ struct svdi_context {
     size_t events_used = 0;
     std::vector<std::pair<int, size_t>> stage_marks;   // (stage, index of start event)
     float stage_ms[SVDI_NUM_STAGES] = {};
+    svdi_context() = default;
+    svdi_context(const svdi_context&) = delete;
+    svdi_context& operator=(const svdi_context&) = delete;
+    ~svdi_context();                         // waits for the context's work, frees everything
 };
 
 namespace svdi {

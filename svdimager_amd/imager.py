@@ -14,7 +14,7 @@ Every call goes to the HIP library; there is no CPU path.
 from __future__ import annotations
 
 import ctypes
-from typing import Dict, Optional, Tuple
+from typing import Dict, List, Optional, Tuple
 
 import numpy as np
 
@@ -131,6 +131,30 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
     check(lib().svdi_image_snapshot(ctx.handle, ctypes.byref(s), _ptr(image), N,
                                     float(cell_size), ctypes.byref(opts), ctypes.byref(info)))
     return image, info.as_dict()
+
+
+def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size: int,
+                         cell_size: float, images=None, **options) -> Tuple[List, List[Dict]]:
+    """Image several device-resident snapshots in one call (concurrent lanes); returns (images,
+    plan dicts).  `images`: optional list of (N, N) float32 device tensors."""
+    torch = _torch()
+    n = len(dsnaps)
+    if n == 0:
+        return [], []
+    dev = dsnaps[0].tensors["uvw"].device
+    N = int(image_size)
+    if images is None:
+        images = [torch.empty((N, N), dtype=torch.float32, device=dev) for _ in range(n)]
+    if len(images) != n:
+        raise ValueError("one image per snapshot")
+    opts = default_options(**options)
+    infos = (PlanInfo * n)()
+    structs = (Snapshot * n)(*[d.struct() for d in dsnaps])
+    ptrs = (ctypes.c_void_p * n)(*[_ptr(im) for im in images])
+    _bind_stream(ctx, dev)
+    check(lib().svdi_image_snapshot_batch(ctx.handle, structs, n, ptrs, N, float(cell_size),
+                                          ctypes.byref(opts), infos))
+    return images, [infos[i].as_dict() for i in range(n)]
 
 
 def fipipe_host(ctx: Context, vis_re: np.ndarray, vis_im: np.ndarray, bin_: np.ndarray,
