@@ -16,10 +16,16 @@
 // a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
 // nothing overflows f16; the scales are undone exactly at write-out.
 //
-// Binning: samples are counted per (slab, window) with block-local LDS histograms, kept per block;
-// after a scan of the window totals each block reserves its ranges with one atomic per window
-// and one more pass over its samples writes the records (position relative to the window, s, v),
-// a sample meeting several windows (at most 2 x 2) writing one record per window.
+// Binning by row class: a sample whose (clipped) first footprint row a0 lies in [8h, 8h + 8) is
+// in row class h, and its footprint rows lie in [8h, 8h + 16).  Window row wa (rows
+// [16 wa, 16 wa + 16)) therefore reads the classes 2 wa - 1 (its rows 0-7 only), 2 wa (all rows)
+// and 2 wa + 1 (rows 8-15 only).  Records are binned by (slab, window column, row class) with
+// the class fastest, so a window's records are one contiguous range ordered by class, each
+// record is written once per window column it meets, and a chunk of 32 records drawn from one
+// or two classes skips the window half that none of them touches.  Counting uses block-local
+// LDS histograms kept per block; after a scan of the bin totals each block reserves its ranges
+// with one atomic per bin and one more pass over its samples writes the records (position
+// relative to the class / window-column origin, s, v).
 #include "sample.hpp"
 #include "scan.hpp"
 
@@ -27,18 +33,21 @@ namespace svdi {
 namespace {
 
 constexpr int kWin = 16;                 // window side in cells
+constexpr int kClass = 8;                // row-class height in cells
 constexpr int kBinThreads = 1024;        // binning blocks: 256 rows x 4 channel groups
 constexpr int kBinRows = 256;
 constexpr int kBinChanGroups = kBinThreads / kBinRows;
 constexpr int kBinPerThread = 16;        // samples per thread, preprocessed input
-constexpr int kLdsBins = 12288;          // block-local histogram size (48 KB)
+constexpr int kLdsBins = 36864;          // largest block-local histogram (144 KB of LDS)
 constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
 constexpr int kChunk = 32;               // samples per MFMA reduction step
 constexpr int kStride = 36;              // floats per staging row (32 samples + pad)
-constexpr int kStageFloats = 48 * kStride;   // ku[16], kv[16], c[16] rows per wavefront
+constexpr int kPad = 8;                  // staging rows beyond each window edge (-8 .. 23)
+constexpr int kStageRows = kWin + 2 * kPad;
+constexpr int kStageFloats = (2 * kStageRows + kWin) * kStride;   // ku, kv padded; c[16]
 constexpr int kMinItem = 2048;
 constexpr int kMaxItem = 16384;
-constexpr float kEsScale = 128.0f;       // 2^7 per axis
+constexpr float kEsScaleLog2 = 7.0f;     // ES factors scaled by 2^7 per axis
 constexpr int kEsScaleLog2Sq = 14;
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -46,16 +55,20 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct WinGeom {
     int nw;          // windows per grid axis
+    int nh;          // row classes: ceil(G / 8)
     int nwin;        // nw * nw
-    int nbins;       // nslabs * nwin
+    int nwin_total;  // nslabs * nwin
+    int nbins;       // nslabs * nw * nh, ordered (slab, window column, row class)
     int use_lds;     // histogram in LDS (nbins <= kLdsBins)
 };
 
+// A work item: records [start, start + count) of window `win`; records before b1 are of class
+// 2 wa - 1, records from b2 on of class 2 wa + 1, the rest of class 2 wa.
 struct WItem {
-    int bin, start, count, split;
+    int win, start, count, split, b1, b2;
 };
 
-// Records, structure of arrays: q = (p1 - window row origin, p2 - window column origin, s, Re v)
+// Records, structure of arrays: q = (p1 - row-class origin, p2 - window column origin, s, Re v)
 // in cells, vy = Im v.
 struct WRecords {
     float4* q;
@@ -82,17 +95,18 @@ __device__ __forceinline__ void for_block_samples(const SampleInput& in, Fn&& fn
     }
 }
 
-// Every (sample, window) pair of this block: the clipped footprint [a0, a0 + na) x [b0, b0 + nb)
-// meets windows a0 / 16 .. (a0 + na - 1) / 16 (at most two per axis, na <= 9).
+// Every (sample, bin) pair of this block: one row class h = a0 / 8, and the window columns
+// b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
 template <bool kSnap, class Fn>
-__device__ __forceinline__ void for_sample_windows(const SampleInput& in, const GridParams& gp,
-                                                   const WinGeom& wg, Fn&& fn) {
+__device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
+                                                const WinGeom& wg, Fn&& fn) {
     for_block_samples<kSnap>(in, [&](size_t idx, size_t row, size_t chan) {
         Footprint f;
         if (!setup_sample<kSnap>(in, gp, idx, row, chan, f)) return;
-        for (int wa = f.a0 / kWin; wa <= (f.a0 + f.na - 1) / kWin; ++wa)
-            for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
-                fn(f, f.local_slab * wg.nwin + wa * wg.nw + wb, wa, wb);
+        const int h = f.a0 / kClass;
+        const int col0 = f.local_slab * wg.nw;
+        for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
+            fn(f, (col0 + wb) * wg.nh + h, h, wb);
     });
 }
 
@@ -110,20 +124,20 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
-// Window counts: per block in an LDS histogram, added to counts[bin] and also kept per block
-// in blockhist[block * nbins + bin] for the scatter (without LDS histograms the counts go
-// straight to counts[bin]).
+// Bin counts: per block in an LDS histogram (dynamic LDS, nbins words), added to counts[bin]
+// and also kept per block in blockhist[block * nbins + bin] for the scatter (without LDS
+// histograms the counts go straight to counts[bin]).
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
             uint32_t* __restrict__ blockhist, uint32_t* __restrict__ vmax_bits) {
-    __shared__ uint32_t hist[kLdsBins];
+    extern __shared__ uint32_t hist[];
     if (wg.use_lds) {
         for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
         __syncthreads();
     }
     float m = 0.0f;
-    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int, int) {
+    for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int, int) {
         if (wg.use_lds) atomicAdd(&hist[bin], 1u);
         else atomicAdd(&counts[bin], 1u);
         m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
@@ -141,31 +155,31 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
 }
 
 // Records of this block's samples, in one pass: with LDS histograms the block reserves each
-// window's range with one atomic on the window's cursor (its count read back from blockhist)
-// and hands out slots with LDS cursors; otherwise every record takes a global cursor slot.
+// bin's range with one atomic on the bin's cursor (its count read back from blockhist) and
+// hands out slots with LDS cursors; otherwise every record takes a global cursor slot.
 // `limit` (the record buffer's capacity) only bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ blockhist,
               uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit) {
-    __shared__ uint32_t hist[kLdsBins];
+    extern __shared__ uint32_t hist[];
     const int off = gp.G / 2;
-    auto emit = [&](uint32_t slot, const Footprint& f, int wa, int wb) {
+    auto emit = [&](uint32_t slot, const Footprint& f, int h, int wb) {
         if (slot >= limit) return;
         float p1r, p2r;
         if constexpr (kSnap) {
-            p1r = f.q1 + static_cast<float>(f.a0 - wa * kWin);
+            p1r = f.q1 + static_cast<float>(f.a0 - h * kClass);
             p2r = f.q2 + static_cast<float>(f.b0 - wb * kWin);
         } else {
-            p1r = f.q1 - static_cast<float>(wa * kWin - off);
+            p1r = f.q1 - static_cast<float>(h * kClass - off);
             p2r = f.q2 - static_cast<float>(wb * kWin - off);
         }
         rec.q[slot] = make_float4(p1r, p2r, f.s, f.vw.x);
         rec.vy[slot] = f.vw.y;
     };
     if (!wg.use_lds) {
-        for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
-            emit(atomicAdd(&cursor[bin], 1u), f, wa, wb);
+        for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+            emit(atomicAdd(&cursor[bin], 1u), f, h, wb);
         });
         return;
     }
@@ -175,36 +189,57 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
     }
     __syncthreads();
-    for_sample_windows<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int wa, int wb) {
-        emit(atomicAdd(&hist[bin], 1u), f, wa, wb);
+    for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+        emit(atomicAdd(&hist[bin], 1u), f, h, wb);
     });
+}
+
+// Record range of window w (slab, wa, wb): the classes 2 wa - 1 .. 2 wa + 1 of column wb that
+// exist, with b1 / b2 the first records of classes 2 wa and 2 wa + 1.  2 wa <= nh - 1 always
+// (nh = ceil(G / 8) >= 2 ceil(G / 16) - 1).
+struct WinRange {
+    uint32_t start, end, b1, b2;
+};
+__device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ boff, const WinGeom& wg, int w) {
+    const int ls = w / wg.nwin, r = w - ls * wg.nwin;
+    const int wa = r / wg.nw, wb = r - wa * wg.nw;
+    const int base = (ls * wg.nw + wb) * wg.nh;
+    WinRange x;
+    x.start = boff[base + max(2 * wa - 1, 0)];
+    x.b1 = boff[base + 2 * wa];
+    x.b2 = boff[base + min(2 * wa + 1, wg.nh)];
+    x.end = boff[base + min(2 * wa + 2, wg.nh)];
+    return x;
 }
 
 // Work items over the windows: a window with at most `cap` records is one item; a larger one is
 // split into items of `cap` records that add their partial windows atomically.  cap =
-// clamp(total / wavefronts, kMinItem, kMaxItem).  The split windows' items come first in the
-// queue, so the largest items start early and the single-item windows fill in behind them.
-// icount[b] = items of split window b, icount[nbins + b] = 1 for an unsplit window b.
+// clamp(work / wavefronts, kMinItem, kMaxItem), the work estimated as 1.5 window reads per
+// record.  The split windows' items come first in the queue, so the largest items start early
+// and the single-item windows fill in behind them.
+// icount[w] = items of split window w, icount[nwin_total + w] = 1 for an unsplit window w.
 // totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
-__device__ __forceinline__ uint32_t item_cap(uint32_t total, int waves) {
-    const uint32_t fair = (total + waves - 1) / waves;
+__device__ __forceinline__ uint32_t item_cap(uint32_t records, int waves) {
+    const uint32_t work = records + records / 2;
+    const uint32_t fair = (work + waves - 1) / waves;
     return min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
 }
 
 __global__ void __launch_bounds__(256)
 k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, int waves,
                   uint32_t* __restrict__ icount, uint32_t* __restrict__ totals) {
-    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int w = blockIdx.x * 256 + threadIdx.x;
     const uint32_t total = boff[wg.nbins];
     const uint32_t cap = item_cap(total, waves);
-    if (b == 0) {
+    if (w == 0) {
         totals[0] = total;
         totals[3] = cap;
     }
-    if (b >= wg.nbins) return;
-    const uint32_t c = boff[b + 1] - boff[b];
-    icount[b] = c > cap ? (c + cap - 1) / cap : 0u;
-    icount[wg.nbins + b] = c > cap ? 0u : 1u;
+    if (w >= wg.nwin_total) return;
+    const WinRange x = window_range(boff, wg, w);
+    const uint32_t c = x.end - x.start;
+    icount[w] = c > cap ? (c + cap - 1) / cap : 0u;
+    icount[wg.nwin_total + w] = c > cap ? 0u : 1u;
 }
 
 // One thread per item: its entry of the item counts is found by binary search over the offsets.
@@ -212,7 +247,7 @@ __global__ void __launch_bounds__(256)
 k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
                  WinGeom wg, WItem* __restrict__ items, uint32_t max_items, uint32_t* __restrict__ totals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const int n = 2 * wg.nbins;
+    const int n = 2 * wg.nwin_total;
     const uint32_t nitems = min(ioff[n], max_items);
     if (i == 0) totals[1] = nitems;
     if (i >= nitems) return;
@@ -222,17 +257,19 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
         if (ioff[mid] <= i) lo = mid;
         else hi = mid;
     }
-    const bool split = lo < wg.nbins;
-    const int b = split ? lo : lo - wg.nbins;
+    const bool split = lo < wg.nwin_total;
+    const int w = split ? lo : lo - wg.nwin_total;
     const uint32_t cap = totals[3];
-    const uint32_t start = boff[b];
-    const uint32_t c = boff[b + 1] - start;
+    const WinRange x = window_range(boff, wg, w);
+    const uint32_t c = x.end - x.start;
     const uint32_t q = i - ioff[lo];
     WItem t;
-    t.bin = b;
-    t.start = static_cast<int>(start + q * cap);
+    t.win = w;
+    t.start = static_cast<int>(x.start + q * cap);
     t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
     t.split = split ? 1 : 0;
+    t.b1 = static_cast<int>(x.b1);
+    t.b2 = static_cast<int>(x.b2);
     items[i] = t;
 }
 
@@ -240,10 +277,11 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
 __global__ void __launch_bounds__(256)
 k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ totals,
                  WinGeom wg, int G, int K, float2* __restrict__ planes) {
-    const int bin = blockIdx.x;
-    if (boff[bin + 1] - boff[bin] <= totals[3]) return;
-    const int ls = bin / wg.nwin, w = bin - ls * wg.nwin;
-    const int wa = w / wg.nw, wb = w - wa * wg.nw;
+    const int w = blockIdx.x;
+    const WinRange x = window_range(boff, wg, w);
+    if (x.end - x.start <= totals[3]) return;
+    const int ls = w / wg.nwin, r = w - ls * wg.nwin;
+    const int wa = r / wg.nw, wb = r - wa * wg.nw;
     const size_t plane_size = static_cast<size_t>(G) * G;
     for (int e = threadIdx.x; e < K * kWin * kWin; e += 256) {
         const int k = e / (kWin * kWin), c = e - k * kWin * kWin;
@@ -276,6 +314,21 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One window row a: B_a[r][b] = ku_r[a] kv_r[b] split into hi / lo, D_a += C B_a as
+// hi*hi + lo*hi + hi*lo.
+__device__ __forceinline__ void window_row(floatx4& d, const float* __restrict__ kurow, const float4& kv0,
+                                           const float4& kv1, const half8& a_hi, const half8& a_lo) {
+    const float4 ku0 = *reinterpret_cast<const float4*>(kurow);
+    const float4 ku1 = *reinterpret_cast<const float4*>(kurow + 4);
+    const float4 x0 = make_float4(ku0.x * kv0.x, ku0.y * kv0.y, ku0.z * kv0.z, ku0.w * kv0.w);
+    const float4 x1 = make_float4(ku1.x * kv1.x, ku1.y * kv1.y, ku1.z * kv1.z, ku1.w * kv1.w);
+    half8 b_hi, b_lo;
+    split_hilo(x0, x1, b_hi, b_lo);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d, 0, 0, 0);
+}
+
 // Persistent wavefronts take items from a queue; each item is one window's record range.  The
 // terms are done eight at a time (one 16-row block of m = 2k + ri).
 __global__ void __launch_bounds__(kWaves * 64)
@@ -283,9 +336,9 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
            WinGeom wg, float2* __restrict__ planes) {
     extern __shared__ __align__(16) float stage_all[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* s_ku = stage_all + wave * kStageFloats;        // [16 rows a][32 samples]
-    float* s_kv = s_ku + 16 * kStride;                     // [16 columns b][32 samples]
-    float* s_c = s_kv + 16 * kStride;                      // [16 m][32 samples]
+    float* s_ku = stage_all + wave * kStageFloats;        // [32 rows a = -8 .. 23][32 samples]
+    float* s_kv = s_ku + kStageRows * kStride;             // [32 columns b = -8 .. 23][32 samples]
+    float* s_c = s_kv + kStageRows * kStride;              // [16 m][32 samples]
 
     // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
     const float vmax = __uint_as_float(totals[4]);
@@ -294,21 +347,27 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     e = min(max(e, -100), 100);
     const float vscale = ldexpf(1.0f, 14 - e);
     const float inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
+    // ES(x) 2^7 = exp2(c (sqrt(1 - x^2) - 1) + 7), c = beta log2(e).
+    constexpr float kEsC = kBeta * 1.4426950408889634f;
 
     const int K = gp.K, G = gp.G;
     const size_t plane_size = static_cast<size_t>(G) * G;
     const int rl = lane & 31, axis = lane >> 5;           // staging: lane = (sample, axis)
     const int g8 = 8 * (lane >> 4), n16 = lane & 15;      // MFMA operand layout
+    float* s_k = (axis ? s_kv : s_ku) + rl;               // this lane's staging column
     for (;;) {
         uint32_t ticket = 0;
         if (lane == 0) ticket = atomicAdd(&totals[2], 1u);
-        const int item = __shfl(static_cast<int>(ticket), 0, 64);
+        // Lane 0's ticket, in a scalar register: the item and everything derived from it are
+        // wave-uniform, so the window-half tests below are scalar branches.
+        const int item = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
         if (item >= static_cast<int>(totals[1])) break;
         const WItem it = items[item];
-        const int ls = it.bin / wg.nwin, w = it.bin - ls * wg.nwin;
+        const int ls = it.win / wg.nwin, w = it.win - ls * wg.nwin;
         const int wa = w / wg.nw, wb = w - wa * wg.nw;
         const float4* q_g = rec.q + it.start;
         const float* vy_g = rec.vy + it.start;
+        const int end = it.start + it.count;
         for (int kg = 0; kg < K; kg += 8) {
             floatx4 d[kWin];
 #pragma unroll
@@ -328,36 +387,48 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     q_next = q_g[c0 + kChunk + rl];
                     vy_next = vy_g[c0 + kChunk + rl];
                 }
-                // ES factors along this lane's axis: zero the 16 window rows / columns, then
-                // evaluate the at most 9 cells of the support |a - p| <= 4 that lie in the window.
-                const float pr = axis ? q.y : q.x;
-                float* s_k = axis ? s_kv : s_ku;
+                // Window halves touched by the chunk's classes (wave-uniform).
+                const int i0 = it.start + c0, i1 = min(end, i0 + kChunk);
+                const bool top = i0 < it.b2, bottom = i1 > it.b1;
+                // ES factors along this lane's axis: zero the window's 16 rows / columns, then
+                // write the 9 cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside
+                // the window; the ninth is nonzero only for an integer p).
 #pragma unroll
-                for (int a = 0; a < kWin; ++a) s_k[a * kStride + rl] = 0.0f;
+                for (int a = 0; a < kWin; ++a) s_k[(a + kPad) * kStride] = 0.0f;
                 if (valid) {
-                    const int first = static_cast<int>(ceilf(pr - 4.0f));
+                    const int i = i0 + rl;
+                    const float shift = i < it.b1 ? -static_cast<float>(kClass)
+                                                  : (i < it.b2 ? 0.0f : static_cast<float>(kClass));
+                    const float pr = axis ? q.y : q.x + shift;
+                    const float fl = ceilf(pr - 4.0f);
+                    const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
+                    const float x0 = (fl - pr) * 0.25f;
+                    float* dst = s_k + (first + kPad) * kStride;
 #pragma unroll
                     for (int j = 0; j < 9; ++j) {
-                        const int a = first + j;
-                        const float x = (static_cast<float>(a) - pr) * 0.25f;
-                        if (a >= 0 && a < kWin && fabsf(x) <= 1.0f)
-                            s_k[a * kStride + rl] = es_kernel_fast(x) * kEsScale;
+                        const float x = x0 + 0.25f * static_cast<float>(j);
+                        const float t = fmaf(-x, x, 1.0f);
+                        const float v = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
+                        dst[j * kStride] = t >= 0.0f ? v : 0.0f;
                     }
                 }
                 // c rows 2k + axis for the terms kg .. kg + 7.
-                const float vs = valid ? (axis ? vy : q.w) * vscale : 0.0f;
-                const float s = q.z;
-                float tkm2 = 1.0f, tkm1 = s;
-                for (int k = 0; k < kg + 8; ++k) {
-                    float tk;
-                    if (k == 0) tk = 1.0f;
-                    else if (k == 1) tk = s;
-                    else {
-                        tk = 2.0f * s * tkm1 - tkm2;
-                        tkm2 = tkm1;
-                        tkm1 = tk;
+                {
+                    const float vs = valid ? (axis ? vy : q.w) * vscale : 0.0f;
+                    const float s = q.z, s2 = 2.0f * s;
+                    float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
+                    for (int k = 0; k < kg; ++k) {
+                        const float t2 = s2 * t1 - t0;
+                        t0 = t1;
+                        t1 = t2;
                     }
-                    if (k >= kg) s_c[(2 * (k - kg) + axis) * kStride + rl] = k < K ? tk * vs : 0.0f;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        s_c[(2 * j + axis) * kStride + rl] = kg + j < K ? t0 * vs : 0.0f;
+                        const float t2 = s2 * t1 - t0;
+                        t0 = t1;
+                        t1 = t2;
+                    }
                 }
                 wave_lds_sync();
 
@@ -369,21 +440,17 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4),
                                a_hi, a_lo);
                 }
-                const float* kvrow = s_kv + n16 * kStride + g8;
+                const float* kvrow = s_kv + (kPad + n16) * kStride + g8;
                 const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
                 const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
+                const float* kub = s_ku + kPad * kStride + g8;
+                if (top) {
 #pragma unroll
-                for (int a = 0; a < kWin; ++a) {
-                    const float* kurow = s_ku + a * kStride + g8;
-                    const float4 ku0 = *reinterpret_cast<const float4*>(kurow);
-                    const float4 ku1 = *reinterpret_cast<const float4*>(kurow + 4);
-                    const float4 x0 = make_float4(ku0.x * kv0.x, ku0.y * kv0.y, ku0.z * kv0.z, ku0.w * kv0.w);
-                    const float4 x1 = make_float4(ku1.x * kv1.x, ku1.y * kv1.y, ku1.z * kv1.z, ku1.w * kv1.w);
-                    half8 b_hi, b_lo;
-                    split_hilo(x0, x1, b_hi, b_lo);
-                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d[a], 0, 0, 0);
-                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d[a], 0, 0, 0);
-                    d[a] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d[a], 0, 0, 0);
+                    for (int a = 0; a < kWin / 2; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+                }
+                if (bottom) {
+#pragma unroll
+                    for (int a = kWin / 2; a < kWin; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
                 }
                 wave_lds_sync();
             }
@@ -414,6 +481,11 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     }
 }
 
+// Dynamic LDS above the default limit for a kernel.
+void allow_lds(const void* fn, size_t bytes) {
+    SVDI_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+}
+
 }  // namespace
 
 const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
@@ -421,27 +493,34 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
     WinGeom wg;
     wg.nw = (gp.G + kWin - 1) / kWin;
+    wg.nh = (gp.G + kClass - 1) / kClass;
     wg.nwin = wg.nw * wg.nw;
-    wg.nbins = wg.nwin * gp.nslabs;
+    wg.nwin_total = wg.nwin * gp.nslabs;
+    const size_t nbins_sz = static_cast<size_t>(gp.nslabs) * wg.nw * wg.nh;
+    if (nbins_sz >= (1ull << 30)) throw UnsupportedError("grid too large for the window gridder");
+    wg.nbins = static_cast<int>(nbins_sz);
     wg.use_lds = wg.nbins <= kLdsBins ? 1 : 0;
     const size_t S = in.num_samples;
-    const size_t max_records = 4 * S;                      // a footprint meets at most 2 x 2 windows
+    const size_t max_records = 2 * S;                      // one row class, at most two window columns
     if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
-    const size_t nbins = static_cast<size_t>(wg.nbins);
-    const size_t max_items = nbins + (max_records + kMinItem - 1) / kMinItem;
+    const size_t nbins = nbins_sz;
+    const size_t nwt = static_cast<size_t>(wg.nwin_total);
+    const size_t max_items = nwt + (2 * max_records + kMinItem - 1) / kMinItem;
     unsigned blocks;
     if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinRows - 1) / kBinRows);
     else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
     const size_t nhist = wg.use_lds ? static_cast<size_t>(blocks) * nbins : 1;
+    const size_t bin_lds = wg.use_lds ? nbins * sizeof(uint32_t) : 0;
 
     // counts[0, nbins) | counts[nbins + 2, +6) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
-    uint32_t* sums = static_cast<uint32_t*>(ctx->ws.get(Workspace::kScanSums, scan_blocks(2 * nbins) * sizeof(uint32_t)));
-    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (2 * nbins + 8) * sizeof(uint32_t)));
-    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (2 * nbins + 8) * sizeof(uint32_t)));
+    uint32_t* sums = static_cast<uint32_t*>(
+        ctx->ws.get(Workspace::kScanSums, scan_blocks(std::max(nbins, 2 * nwt)) * sizeof(uint32_t)));
+    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (2 * nwt + 8) * sizeof(uint32_t)));
+    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (2 * nwt + 8) * sizeof(uint32_t)));
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
@@ -449,33 +528,37 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     rec.vy = static_cast<float*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float)));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 8, 0.0f);
-    if (in.from_snapshot)
-        k_win_count<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
-    else
-        k_win_count<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+    if (in.from_snapshot) {
+        allow_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
+        k_win_count<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+    } else {
+        allow_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
+        k_win_count<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+    }
     SVDI_LAUNCH_CHECK();
     launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
     const int waves = kWaves * std::max(1, ctx->cu_count);
-    const unsigned bin_blocks = static_cast<unsigned>((nbins + 255) / 256);
-    k_win_item_counts<<<bin_blocks, 256, 0, ctx->stream>>>(boff, wg, waves, icount, totals);
+    k_win_item_counts<<<static_cast<unsigned>((nwt + 255) / 256), 256, 0, ctx->stream>>>(boff, wg, waves, icount, totals);
     SVDI_LAUNCH_CHECK();
-    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, 2 * nbins);
+    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, 2 * nwt);
     k_win_item_write<<<static_cast<unsigned>((max_items + 255) / 256), 256, 0, ctx->stream>>>(
         boff, ioff, wg, items, static_cast<uint32_t>(max_items), totals);
     SVDI_LAUNCH_CHECK();
-    k_win_zero_split<<<wg.nbins, 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
+    k_win_zero_split<<<static_cast<unsigned>(nwt), 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     const uint32_t limit = static_cast<uint32_t>(max_records);
-    if (in.from_snapshot)
-        k_win_scatter<true><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
-    else
-        k_win_scatter<false><<<blocks, kBinThreads, 0, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
+    if (in.from_snapshot) {
+        allow_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
+        k_win_scatter<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
+    } else {
+        allow_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
+        k_win_scatter<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
+    }
     SVDI_LAUNCH_CHECK();
 
     const size_t tok = stage_begin(ctx, 5);
     constexpr size_t lds = sizeof(float) * kStageFloats * kWaves;
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_win_grid),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    allow_lds(reinterpret_cast<const void*>(&k_win_grid), lds);
     const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
     k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes);
     SVDI_LAUNCH_CHECK();
