@@ -63,9 +63,11 @@ struct WinGeom {
 };
 
 // A work item: records [start, start + count) of window `win`; records before b1 are of class
-// 2 wa - 1, records from b2 on of class 2 wa + 1, the rest of class 2 wa.
+// 2 wa - 1, records from b2 on of class 2 wa + 1, the rest of class 2 wa.  slot >= 0: the item is
+// one piece of a window split over several workgroup items and writes its partial window to
+// partial slot `slot`; slot < 0: it writes the window to the planes.
 struct WItem {
-    int win, start, count, split, b1, b2;
+    int win, start, count, slot, b1, b2;
 };
 
 // Records, structure of arrays: q = (p1 - row-class origin, p2 - window column origin, s, Re v)
@@ -212,44 +214,71 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
     return x;
 }
 
-// Work items over the windows: a window with at most `cap` records is one item; a larger one is
-// split into items of `cap` records that add their partial windows atomically.  cap =
-// clamp(work / wavefronts, kMinItem, kMaxItem), the work estimated as 1.5 window reads per
-// record.  The split windows' items come first in the queue, so the largest items start early
-// and the single-item windows fill in behind them.
-// icount[w] = items of split window w, icount[nwin_total + w] = 1 for an unsplit window w.
-// totals[0] = records, [1] = items, [2] = queue head, [3] = cap, [4] = max |v| bits.
-__device__ __forceinline__ uint32_t item_cap(uint32_t records, int waves) {
-    const uint32_t work = records + records / 2;
-    const uint32_t fair = (work + waves - 1) / waves;
-    return min(static_cast<uint32_t>(kMaxItem), max(static_cast<uint32_t>(kMinItem), fair));
+// Work items.  A window with at most kCoopMin records is one wavefront item.  A larger one is
+// taken by whole workgroups: pieces of at most kCoopCap records whose chunks the workgroup's
+// wavefronts share, their accumulators summed in LDS.  A window with several pieces writes one
+// partial window per piece, and k_win_reduce adds them in piece order (so the result does not
+// depend on timing).  Both queues run largest first, by octave classes: class c holds items
+// with count * 2^c <= cap < count * 2^(c+1) (cap = kCoopMin for wavefront items, kCoopCap for
+// workgroup items; the last class also the smaller and empty ones), so that the wavefronts and
+// workgroups finish together.
+// icount: [kItemClasses][nwin_total] wavefront items, [kItemClasses][nwin_total] workgroup items,
+// [nwin_total] partial slots; one exclusive scan gives the queue positions and slot numbers.
+// totals[0] = records, [1] = wavefront items, [2] = wavefront queue head, [3] = workgroup items,
+// [4] = max |v| bits, [5] = workgroup queue head, [6] = windows with several pieces.
+constexpr int kItemClasses = 8;
+constexpr uint32_t kCoopMin = 1024;
+constexpr uint32_t kCoopCap = 12288;
+
+__device__ __forceinline__ int item_class(uint32_t count, uint32_t cap) {
+    if (count == 0) return kItemClasses - 1;
+    int c = 0;
+    while (c < kItemClasses - 1 && (static_cast<uint64_t>(count) << (c + 1)) <= cap) ++c;
+    return c;
 }
+
+__device__ __forceinline__ uint32_t coop_pieces(uint32_t c) { return (c + kCoopCap - 1) / kCoopCap; }
 
 __global__ void __launch_bounds__(256)
-k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, int waves,
-                  uint32_t* __restrict__ icount, uint32_t* __restrict__ totals) {
+k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, uint32_t* __restrict__ icount,
+                  uint32_t* __restrict__ multi, uint32_t* __restrict__ totals) {
     const int w = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t total = boff[wg.nbins];
-    const uint32_t cap = item_cap(total, waves);
-    if (w == 0) {
-        totals[0] = total;
-        totals[3] = cap;
-    }
-    if (w >= wg.nwin_total) return;
+    const int nwt = wg.nwin_total;
+    if (w == 0) totals[0] = boff[wg.nbins];
+    if (w >= nwt) return;
     const WinRange x = window_range(boff, wg, w);
     const uint32_t c = x.end - x.start;
-    icount[w] = c > cap ? (c + cap - 1) / cap : 0u;
-    icount[wg.nwin_total + w] = c > cap ? 0u : 1u;
+    for (int k = 0; k <= 2 * kItemClasses; ++k) icount[k * nwt + w] = 0u;
+    if (c <= kCoopMin) {
+        icount[item_class(c, kCoopMin) * nwt + w] = 1u;
+        return;
+    }
+    const uint32_t full = c / kCoopCap, rem = c - full * kCoopCap;
+    uint32_t* gcount = icount + kItemClasses * nwt;
+    gcount[w] += full;
+    if (rem > 0) gcount[item_class(rem, kCoopCap) * nwt + w] += 1u;
+    const uint32_t pieces = coop_pieces(c);
+    if (pieces > 1) {
+        icount[2 * kItemClasses * nwt + w] = pieces;
+        multi[atomicAdd(&totals[6], 1u)] = static_cast<uint32_t>(w);
+    }
 }
 
-// One thread per item: its entry of the item counts is found by binary search over the offsets.
+// One thread per item: its (queue, class, window) entry of the counts is found by binary search
+// over the offsets.  Workgroup piece q of class 0 below the window's full count is full piece q;
+// any other is the remainder (piece `full`).
 __global__ void __launch_bounds__(256)
 k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
                  WinGeom wg, WItem* __restrict__ items, uint32_t max_items, uint32_t* __restrict__ totals) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const int n = 2 * wg.nwin_total;
+    const int nwt = wg.nwin_total;
+    const int n = 2 * kItemClasses * nwt;
+    const uint32_t nwave = ioff[kItemClasses * nwt];
     const uint32_t nitems = min(ioff[n], max_items);
-    if (i == 0) totals[1] = nitems;
+    if (i == 0) {
+        totals[1] = min(nwave, nitems);
+        totals[3] = nitems - min(nwave, nitems);
+    }
     if (i >= nitems) return;
     int lo = 0, hi = n;                         // largest j with ioff[j] <= i
     while (hi - lo > 1) {
@@ -257,38 +286,70 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
         if (ioff[mid] <= i) lo = mid;
         else hi = mid;
     }
-    const bool split = lo < wg.nwin_total;
-    const int w = split ? lo : lo - wg.nwin_total;
-    const uint32_t cap = totals[3];
+    const bool coop = lo >= kItemClasses * nwt;
+    const int j = coop ? lo - kItemClasses * nwt : lo;
+    const int cls = j / nwt;
+    const int w = j - cls * nwt;
     const WinRange x = window_range(boff, wg, w);
     const uint32_t c = x.end - x.start;
-    const uint32_t q = i - ioff[lo];
     WItem t;
     t.win = w;
-    t.start = static_cast<int>(x.start + q * cap);
-    t.count = static_cast<int>(c > q * cap ? min(cap, c - q * cap) : 0u);
-    t.split = split ? 1 : 0;
     t.b1 = static_cast<int>(x.b1);
     t.b2 = static_cast<int>(x.b2);
+    t.start = static_cast<int>(x.start);
+    t.count = static_cast<int>(c);
+    t.slot = -1;
+    if (coop) {
+        const uint32_t full = c / kCoopCap;
+        const uint32_t q = i - ioff[lo];
+        const uint32_t piece = (cls == 0 && q < full) ? q : full;
+        t.start = static_cast<int>(x.start + piece * kCoopCap);
+        t.count = static_cast<int>(piece < full ? kCoopCap : c - full * kCoopCap);
+        if (coop_pieces(c) > 1) t.slot = static_cast<int>(ioff[n + w] - ioff[n] + piece);
+    }
     items[i] = t;
 }
 
-// Clear the windows accumulated by several items (their items add atomically).
+// Windows with several pieces: the sum of their partial windows, in piece order, into the
+// planes.  One unit = (listed window, term); 256 threads = the window's cells.
 __global__ void __launch_bounds__(256)
-k_win_zero_split(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ totals,
-                 WinGeom wg, int G, int K, float2* __restrict__ planes) {
-    const int w = blockIdx.x;
-    const WinRange x = window_range(boff, wg, w);
-    if (x.end - x.start <= totals[3]) return;
-    const int ls = w / wg.nwin, r = w - ls * wg.nwin;
-    const int wa = r / wg.nw, wb = r - wa * wg.nw;
+k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
+             const uint32_t* __restrict__ multi, const uint32_t* __restrict__ totals, WinGeom wg,
+             int G, int K, uint32_t max_slots, const float2* __restrict__ partial, float2* __restrict__ planes) {
+    const int n = 2 * kItemClasses * wg.nwin_total;
+    const uint32_t units = totals[6] * static_cast<uint32_t>(K);
     const size_t plane_size = static_cast<size_t>(G) * G;
-    for (int e = threadIdx.x; e < K * kWin * kWin; e += 256) {
-        const int k = e / (kWin * kWin), c = e - k * kWin * kWin;
-        const int a = wa * kWin + c / kWin, b = wb * kWin + c % kWin;
+    const size_t piece_size = static_cast<size_t>(K) * kWin * kWin;
+    for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const int w = static_cast<int>(multi[u / K]);
+        const int k = static_cast<int>(u % K);
+        const WinRange x = window_range(boff, wg, w);
+        const uint32_t pieces = coop_pieces(x.end - x.start);
+        const uint32_t first = ioff[n + w] - ioff[n];
+        if (first + pieces > max_slots) continue;          // never: the slot count bounds the pieces
+        const int ls = w / wg.nwin, r = w - ls * wg.nwin;
+        const int wa = r / wg.nw, wb = r - wa * wg.nw;
+        const int cell = threadIdx.x;
+        const int a = wa * kWin + cell / kWin, b = wb * kWin + cell % kWin;
+        const float2* src = partial + first * piece_size + static_cast<size_t>(k) * kWin * kWin + cell;
+        float2 acc[4] = {};
+        uint32_t p = 0;
+        for (; p + 4 <= pieces; p += 4) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float2 v = src[(p + j) * piece_size];
+                acc[j].x += v.x;
+                acc[j].y += v.y;
+            }
+        }
+        for (; p < pieces; ++p) {
+            const float2 v = src[p * piece_size];
+            acc[0].x += v.x;
+            acc[0].y += v.y;
+        }
         if (a < G && b < G)
             planes[(static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(a) * G + b] =
-                make_float2(0.f, 0.f);
+                make_float2((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x), (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y));
     }
 }
 
@@ -329,155 +390,321 @@ __device__ __forceinline__ void window_row(floatx4& d, const float* __restrict__
     d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d, 0, 0, 0);
 }
 
-// Persistent wavefronts take items from a queue; each item is one window's record range.  The
-// terms are done eight at a time (one 16-row block of m = 2k + ri).
-__global__ void __launch_bounds__(kWaves * 64)
-k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
-           WinGeom wg, float2* __restrict__ planes) {
-    extern __shared__ __align__(16) float stage_all[];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float* s_ku = stage_all + wave * kStageFloats;        // [32 rows a = -8 .. 23][32 samples]
-    float* s_kv = s_ku + kStageRows * kStride;             // [32 columns b = -8 .. 23][32 samples]
-    float* s_c = s_kv + kStageRows * kStride;              // [16 m][32 samples]
+// Per-wavefront constants of the window kernel.
+struct WinLane {
+    float* s_ku;        // [32 rows a = -8 .. 23][32 samples]
+    float* s_kv;        // [32 columns b = -8 .. 23][32 samples]
+    float* s_c;         // [16 m][32 samples]
+    float* s_k;         // this lane's staging column (ku or kv)
+    int lane, rl, axis, g8, n16;
+    float vscale, inv;
+};
 
-    // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
-    const float vmax = __uint_as_float(totals[4]);
-    int e = 0;
-    if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
-    e = min(max(e, -100), 100);
-    const float vscale = ldexpf(1.0f, 14 - e);
-    const float inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
+// One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q, vy) contracted
+// into the accumulators d for the terms kg .. kg + 7.
+__device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
+                                             const float4& q, float vy, floatx4 (&d)[kWin]) {
     // ES(x) 2^7 = exp2(c (sqrt(1 - x^2) - 1) + 7), c = beta log2(e).
     constexpr float kEsC = kBeta * 1.4426950408889634f;
+    const bool valid = c0 + L.rl < it.count;
+    // Window halves touched by the chunk's classes (wave-uniform).
+    const int i0 = it.start + c0, i1 = min(it.start + it.count, i0 + kChunk);
+    const bool top = i0 < it.b2, bottom = i1 > it.b1;
+    // ES factors along this lane's axis: zero the window's 16 rows / columns, then write the 9
+    // cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside the window; the ninth
+    // is nonzero only for an integer p).
+#pragma unroll
+    for (int a = 0; a < kWin; ++a) L.s_k[(a + kPad) * kStride] = 0.0f;
+    if (valid) {
+        const int i = i0 + L.rl;
+        const float shift = i < it.b1 ? -static_cast<float>(kClass)
+                                      : (i < it.b2 ? 0.0f : static_cast<float>(kClass));
+        const float pr = L.axis ? q.y : q.x + shift;
+        const float fl = ceilf(pr - 4.0f);
+        const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
+        const float x0 = (fl - pr) * 0.25f;
+        float* dst = L.s_k + (first + kPad) * kStride;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const float x = x0 + 0.25f * static_cast<float>(j);
+            const float t = fmaf(-x, x, 1.0f);
+            const float v = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
+            dst[j * kStride] = t >= 0.0f ? v : 0.0f;
+        }
+    }
+    // c rows 2k + axis for the terms kg .. kg + 7.
+    {
+        const float vs = valid ? (L.axis ? vy : q.w) * L.vscale : 0.0f;
+        const float s = q.z, s2 = 2.0f * s;
+        float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
+        for (int k = 0; k < kg; ++k) {
+            const float t2 = s2 * t1 - t0;
+            t0 = t1;
+            t1 = t2;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            L.s_c[(2 * j + L.axis) * kStride + L.rl] = kg + j < K ? t0 * vs : 0.0f;
+            const float t2 = s2 * t1 - t0;
+            t0 = t1;
+            t1 = t2;
+        }
+    }
+    wave_lds_sync();
 
-    const int K = gp.K, G = gp.G;
+    // A (16 x 32): lane holds A[n16][g8 + i]; B (32 x 16): lane holds B[g8 + i][n16];
+    // D (16 x 16): lane holds D[4 (lane / 16) + i][n16].
+    half8 a_hi, a_lo;
+    {
+        const float* row = L.s_c + L.n16 * kStride + L.g8;
+        split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4), a_hi, a_lo);
+    }
+    const float* kvrow = L.s_kv + (kPad + L.n16) * kStride + L.g8;
+    const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
+    const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
+    const float* kub = L.s_ku + kPad * kStride + L.g8;
+    if (top) {
+#pragma unroll
+        for (int a = 0; a < kWin / 2; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+    }
+    if (bottom) {
+#pragma unroll
+        for (int a = kWin / 2; a < kWin; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+    }
+    wave_lds_sync();
+}
+
+// Destination of the window values of (row a, term k, column n16): the planes for a whole
+// window, the partial slot ([k][16][16]) for a piece; out-of-grid cells and k >= K go to the
+// scratch line `trash` so that every lane issues the same stores.
+struct WinOut {
+    float2* base;
+    int kstride, astride;
+    bool piece;
+};
+__device__ __forceinline__ WinOut window_out(const WItem& it, const WinGeom& wg, int K, int G, float2* planes,
+                                             float2* partial, int n16) {
+    const int ls = it.win / wg.nwin, w = it.win - ls * wg.nwin;
+    const int wa = w / wg.nw, wb = w - wa * wg.nw;
     const size_t plane_size = static_cast<size_t>(G) * G;
-    const int rl = lane & 31, axis = lane >> 5;           // staging: lane = (sample, axis)
-    const int g8 = 8 * (lane >> 4), n16 = lane & 15;      // MFMA operand layout
-    float* s_k = (axis ? s_kv : s_ku) + rl;               // this lane's staging column
+    WinOut o;
+    o.piece = it.slot >= 0;
+    o.base = o.piece ? partial + static_cast<size_t>(it.slot) * K * kWin * kWin + n16
+                     : planes + static_cast<size_t>(ls) * K * plane_size + static_cast<size_t>(wa) * kWin * G + wb * kWin + n16;
+    // Offsets from base fit in 32 bits: k * G^2 + 15 G < 2^31 for K <= 32, G <= 8192.
+    o.kstride = o.piece ? kWin * kWin : static_cast<int>(plane_size);
+    o.astride = o.piece ? kWin : G;
+    return o;
+}
+
+__device__ __forceinline__ void window_store(const WinOut& o, const WItem& it, const WinGeom& wg, int G, int K,
+                                             int a, int k, int n16, float inv, float2 v, float2* trash, int lane) {
+    const int w = it.win % wg.nwin;
+    const int ag = (w / wg.nw) * kWin + a, bg = (w % wg.nw) * kWin + n16;
+    const float sg = ((ag + bg) & 1) ? -inv : inv;
+    const bool ok = k < K && (o.piece || (ag < G && bg < G));
+    float2* dst = ok ? o.base + (k * o.kstride + a * o.astride) : trash + lane;
+    *dst = make_float2(v.x * sg, v.y * sg);
+}
+
+// Persistent workgroups, one per CU.  First the workgroup items: all wavefronts of the workgroup
+// claim the item's chunks from a shared counter, then sum their accumulators in LDS (over the
+// staging area) and the workgroup writes the window once.  Then the wavefront items, each
+// wavefront on its own:
+// the next item is claimed while the current one is contracted and its first records are
+// requested before the current window is written, so the write-out overlaps the next item's
+// loads (vector memory operations complete in order).  The terms are done eight at a time
+// (one 16-row block of m = 2k + ri).
+__global__ void __launch_bounds__(kWaves * 64)
+k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
+           WinGeom wg, float2* __restrict__ planes, float2* __restrict__ partial, float2* __restrict__ trash) {
+    extern __shared__ __align__(16) float stage_all[];
+    __shared__ int s_item, s_chunk;
+    WinLane L;
+    const int wave = threadIdx.x >> 6;
+    L.lane = threadIdx.x & 63;
+    L.s_ku = stage_all + wave * kStageFloats;
+    L.s_kv = L.s_ku + kStageRows * kStride;
+    L.s_c = L.s_kv + kStageRows * kStride;
+    L.rl = L.lane & 31;
+    L.axis = L.lane >> 5;
+    L.g8 = 8 * (L.lane >> 4);
+    L.n16 = L.lane & 15;
+    L.s_k = (L.axis ? L.s_kv : L.s_ku) + L.rl;
+    // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
+    {
+        const float vmax = __uint_as_float(totals[4]);
+        int e = 0;
+        if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
+        e = min(max(e, -100), 100);
+        L.vscale = ldexpf(1.0f, 14 - e);
+        L.inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
+    }
+    const int K = gp.K, G = gp.G;
+    const int lane = L.lane;
+    const int nwave_items = static_cast<int>(totals[1]);
+    const int ncoop = static_cast<int>(totals[3]);
+    const WItem* coop_items = items + nwave_items;
+
+    // Workgroup items.  The claimed index is read back into a scalar register, so the loop exit is
+    // wave-uniform by construction and every wavefront runs the same barriers: were it left in a
+    // vector register, the compiler could treat the exit as divergent and let lanes other than
+    // thread 0 iterate without the claim.  Within an item the wavefronts claim chunks from an LDS
+    // counter (the SIMDs issue their oldest wavefront first, so an even split would leave the
+    // item waiting for the youngest); each claims its next chunk before contracting the current
+    // one, and requests that chunk's records at once.
+    const int wave_id = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+    if (threadIdx.x == 0) s_chunk = 0;
+    auto claim_chunk = [&]() -> int {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(&s_chunk, kChunk);
+        return __builtin_amdgcn_readfirstlane(v);
+    };
     for (;;) {
-        uint32_t ticket = 0;
-        if (lane == 0) ticket = atomicAdd(&totals[2], 1u);
-        // Lane 0's ticket, in a scalar register: the item and everything derived from it are
-        // wave-uniform, so the window-half tests below are scalar branches.
-        const int item = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
-        if (item >= static_cast<int>(totals[1])) break;
-        const WItem it = items[item];
-        const int ls = it.win / wg.nwin, w = it.win - ls * wg.nwin;
-        const int wa = w / wg.nw, wb = w - wa * wg.nw;
-        const float4* q_g = rec.q + it.start;
-        const float* vy_g = rec.vy + it.start;
-        const int end = it.start + it.count;
+        if (wave_id == 0) {
+            uint32_t t = 0;
+            if (lane == 0) t = atomicAdd(&totals[5], 1u);
+            s_item = __builtin_amdgcn_readfirstlane(static_cast<int>(t));
+        }
+        __syncthreads();
+        const int gi = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (gi >= ncoop) break;
+        const WItem it = coop_items[gi];
         for (int kg = 0; kg < K; kg += 8) {
             floatx4 d[kWin];
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
-            // Records of the next chunk are loaded while the current one is contracted.
+            int c0 = claim_chunk();
             float4 q_next = make_float4(0.f, 0.f, 0.f, 0.f);
             float vy_next = 0.0f;
-            if (rl < it.count) {
-                q_next = q_g[rl];
-                vy_next = vy_g[rl];
+            if (c0 < it.count) {
+                const int i = it.start + min(c0 + L.rl, it.count - 1);
+                q_next = rec.q[i];
+                vy_next = rec.vy[i];
             }
-            for (int c0 = 0; c0 < it.count; c0 += kChunk) {
-                const bool valid = c0 + rl < it.count;
+            while (c0 < it.count) {
                 const float4 q = q_next;
                 const float vy = vy_next;
-                if (c0 + kChunk + rl < it.count) {
-                    q_next = q_g[c0 + kChunk + rl];
-                    vy_next = vy_g[c0 + kChunk + rl];
+                const int c1 = claim_chunk();
+                if (c1 < it.count) {
+                    const int i = it.start + min(c1 + L.rl, it.count - 1);
+                    q_next = rec.q[i];
+                    vy_next = rec.vy[i];
                 }
-                // Window halves touched by the chunk's classes (wave-uniform).
-                const int i0 = it.start + c0, i1 = min(end, i0 + kChunk);
-                const bool top = i0 < it.b2, bottom = i1 > it.b1;
-                // ES factors along this lane's axis: zero the window's 16 rows / columns, then
-                // write the 9 cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside
-                // the window; the ninth is nonzero only for an integer p).
-#pragma unroll
-                for (int a = 0; a < kWin; ++a) s_k[(a + kPad) * kStride] = 0.0f;
-                if (valid) {
-                    const int i = i0 + rl;
-                    const float shift = i < it.b1 ? -static_cast<float>(kClass)
-                                                  : (i < it.b2 ? 0.0f : static_cast<float>(kClass));
-                    const float pr = axis ? q.y : q.x + shift;
-                    const float fl = ceilf(pr - 4.0f);
-                    const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
-                    const float x0 = (fl - pr) * 0.25f;
-                    float* dst = s_k + (first + kPad) * kStride;
-#pragma unroll
-                    for (int j = 0; j < 9; ++j) {
-                        const float x = x0 + 0.25f * static_cast<float>(j);
-                        const float t = fmaf(-x, x, 1.0f);
-                        const float v = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
-                        dst[j * kStride] = t >= 0.0f ? v : 0.0f;
-                    }
-                }
-                // c rows 2k + axis for the terms kg .. kg + 7.
-                {
-                    const float vs = valid ? (axis ? vy : q.w) * vscale : 0.0f;
-                    const float s = q.z, s2 = 2.0f * s;
-                    float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
-                    for (int k = 0; k < kg; ++k) {
-                        const float t2 = s2 * t1 - t0;
-                        t0 = t1;
-                        t1 = t2;
-                    }
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        s_c[(2 * j + axis) * kStride + rl] = kg + j < K ? t0 * vs : 0.0f;
-                        const float t2 = s2 * t1 - t0;
-                        t0 = t1;
-                        t1 = t2;
-                    }
-                }
-                wave_lds_sync();
-
-                // A (16 x 32): lane holds A[n16][g8 + i]; B (32 x 16): lane holds B[g8 + i][n16];
-                // D (16 x 16): lane holds D[4 (lane / 16) + i][n16].
-                half8 a_hi, a_lo;
-                {
-                    const float* row = s_c + n16 * kStride + g8;
-                    split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4),
-                               a_hi, a_lo);
-                }
-                const float* kvrow = s_kv + (kPad + n16) * kStride + g8;
-                const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
-                const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
-                const float* kub = s_ku + kPad * kStride + g8;
-                if (top) {
-#pragma unroll
-                    for (int a = 0; a < kWin / 2; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
-                }
-                if (bottom) {
-#pragma unroll
-                    for (int a = kWin / 2; a < kWin; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
-                }
-                wave_lds_sync();
+                window_chunk(L, it, c0, kg, K, q, vy, d);
+                c0 = c1;
             }
-            // Write the window: rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16)
-            // and kg + 2 (lane / 16) + 1; column n16.
-            const int bg = wb * kWin + n16;
+            // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
+            // wavefronts 0-5 store theirs in slots 0-5, wavefronts 6-11 add theirs to slot
+            // w - 6, and the write sums the six slots.  Slot layout red4[slot][a][lane] (float4
+            // = the lane's D[4 (lane / 16) + i][n16], i = 0..3).  The chunk counter is reset for
+            // the next term block or item once every wavefront is done with it.
+            constexpr int kSlots = kWaves / 2;
+            static_assert(kSlots * kWin * 64 * 4 <= kStageFloats * kWaves, "reduction slots exceed the staging area");
+            float4* red4 = reinterpret_cast<float4*>(stage_all);
+            __syncthreads();
+            if (wave_id < kSlots) {
 #pragma unroll
-            for (int a = 0; a < kWin; ++a) {
-                const int ag = wa * kWin + a;
-                if (ag >= G || bg >= G) continue;
-                const float sg = ((ag + bg) & 1) ? -inv : inv;
+                for (int a = 0; a < kWin; ++a)
+                    red4[(wave_id * kWin + a) * 64 + lane] = make_float4(d[a][0], d[a][1], d[a][2], d[a][3]);
+            }
+            if (threadIdx.x == 0) s_chunk = 0;
+            __syncthreads();
+            if (wave_id >= kSlots) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int k = kg + 2 * (lane >> 4) + h;
-                    if (k >= K) continue;
-                    const float2 val = make_float2(d[a][2 * h] * sg, d[a][2 * h + 1] * sg);
-                    float2* dst = planes + (static_cast<size_t>(ls) * K + k) * plane_size +
-                                  static_cast<size_t>(ag) * G + bg;
-                    if (it.split) {
-                        unsafeAtomicAdd(&dst->x, val.x);
-                        unsafeAtomicAdd(&dst->y, val.y);
-                    } else {
-                        *dst = val;
-                    }
+                for (int a = 0; a < kWin; ++a) {
+                    float4& r = red4[((wave_id - kSlots) * kWin + a) * 64 + lane];
+                    const float4 v = r;
+                    r = make_float4(v.x + d[a][0], v.y + d[a][1], v.z + d[a][2], v.w + d[a][3]);
                 }
             }
+            __syncthreads();
+            // Write: (row a, source lane sl) -> terms kg + 2 (sl / 16) + h, h = 0, 1, column sl % 16.
+            for (int v = threadIdx.x; v < kWin * 64; v += kWaves * 64) {
+                const int a = v >> 6, sl = v & 63;
+                float4 acc = red4[a * 64 + sl];
+#pragma unroll
+                for (int j = 1; j < kSlots; ++j) {
+                    const float4 x = red4[(j * kWin + a) * 64 + sl];
+                    acc = make_float4(acc.x + x.x, acc.y + x.y, acc.z + x.z, acc.w + x.w);
+                }
+                const WinOut os = window_out(it, wg, K, G, planes, partial, sl & 15);
+                window_store(os, it, wg, G, K, a, kg + 2 * (sl >> 4), sl & 15, L.inv, make_float2(acc.x, acc.y), trash, sl);
+                window_store(os, it, wg, G, K, a, kg + 2 * (sl >> 4) + 1, sl & 15, L.inv, make_float2(acc.z, acc.w), trash, sl);
+            }
+            __syncthreads();
         }
+    }
+
+    // Wavefront items.  The queue head is claimed by lane 0 through a pointer the compiler cannot
+    // prove uniform, so the claim stays a single returning atomic whose result is read only when
+    // needed.
+    uint32_t* head = &totals[2];
+    asm volatile("" : "+v"(head));
+    uint32_t ticket = 0;
+    if (lane == 0) ticket = atomicAdd(head, 1u);
+    int item = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
+    if (item >= nwave_items) return;
+    WItem it = items[item];
+    // Records of the next chunk, loaded while the current one is contracted.  Every lane loads
+    // (lanes past the end re-read the last record and ignore it), so the loads land in place.
+    float4 q_next;
+    float vy_next;
+    {
+        const int i = it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0;
+        q_next = rec.q[i];
+        vy_next = rec.vy[i];
+    }
+    for (;;) {
+        if (lane == 0) ticket = atomicAdd(head, 1u);
+        int next = nwave_items;
+        WItem it_next = it;
+        const WinOut o = window_out(it, wg, K, G, planes, partial, L.n16);
+        for (int kg = 0; kg < K; kg += 8) {
+            floatx4 d[kWin];
+#pragma unroll
+            for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+            for (int c0 = 0; c0 < it.count; c0 += kChunk) {
+                const float4 q = q_next;
+                const float vy = vy_next;
+                if (c0 + kChunk < it.count) {
+                    const int i = it.start + min(c0 + kChunk + L.rl, it.count - 1);
+                    q_next = rec.q[i];
+                    vy_next = rec.vy[i];
+                }
+                window_chunk(L, it, c0, kg, K, q, vy, d);
+            }
+            // What comes next: this item's next term block, or the next item.  Its first
+            // records are requested now, ahead of the stores below.
+            {
+                int pstart = it.start, pcount = it.count;
+                if (kg + 8 >= K) {
+                    next = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
+                    if (next < nwave_items) {
+                        it_next = items[next];
+                        pstart = it_next.start;
+                        pcount = it_next.count;
+                    } else {
+                        pcount = 0;
+                    }
+                }
+                const int i = pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0;
+                q_next = rec.q[i];
+                vy_next = rec.vy[i];
+            }
+            // Rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16) and
+            // kg + 2 (lane / 16) + 1; column n16.
+#pragma unroll
+            for (int a = 0; a < kWin; ++a)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+                    window_store(o, it, wg, G, K, a, kg + 2 * (lane >> 4) + h, L.n16, L.inv,
+                                 make_float2(d[a][2 * h], d[a][2 * h + 1]), trash, lane);
+        }
+        if (next >= nwave_items) break;
+        item = next;
+        it = it_next;
     }
 }
 
@@ -497,7 +724,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     wg.nwin = wg.nw * wg.nw;
     wg.nwin_total = wg.nwin * gp.nslabs;
     const size_t nbins_sz = static_cast<size_t>(gp.nslabs) * wg.nw * wg.nh;
-    if (nbins_sz >= (1ull << 30)) throw UnsupportedError("grid too large for the window gridder");
+    if (nbins_sz >= (1ull << 30) || gp.G > 8192) throw UnsupportedError("grid too large for the window gridder");
     wg.nbins = static_cast<int>(nbins_sz);
     wg.use_lds = wg.nbins <= kLdsBins ? 1 : 0;
     const size_t S = in.num_samples;
@@ -505,29 +732,40 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
     const size_t nbins = nbins_sz;
     const size_t nwt = static_cast<size_t>(wg.nwin_total);
-    const size_t max_items = nwt + (2 * max_records + kMinItem - 1) / kMinItem;
+    // Wavefront items: one per window at most.  Workgroup pieces: at most one per window plus
+    // reads / kCoopCap (a window reads each record at most twice).  Partial slots: only windows
+    // with two or more pieces have them, at most 2 reads / kCoopCap.
+    const size_t max_reads = 2 * max_records;
+    const size_t max_items = 2 * nwt + max_reads / kCoopCap + 1;
+    const size_t max_slots = 2 * max_reads / kCoopCap + 1;
     unsigned blocks;
     if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinRows - 1) / kBinRows);
     else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
     const size_t nhist = wg.use_lds ? static_cast<size_t>(blocks) * nbins : 1;
     const size_t bin_lds = wg.use_lds ? nbins * sizeof(uint32_t) : 0;
 
-    // counts[0, nbins) | counts[nbins + 2, +6) = totals.
-    uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 8) * sizeof(uint32_t)));
+    // counts[0, nbins) | counts[nbins + 2, +8) = totals.
+    uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 12) * sizeof(uint32_t)));
     uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
-    uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, nbins * sizeof(uint32_t)));
+    uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, (nbins + 8 + nwt) * sizeof(uint32_t)));
+    uint32_t* multi = cursor + nbins + 8;                  // windows with several pieces
+    const size_t ncounts = (2 * kItemClasses + 1) * nwt;
     uint32_t* sums = static_cast<uint32_t*>(
-        ctx->ws.get(Workspace::kScanSums, scan_blocks(std::max(nbins, 2 * nwt)) * sizeof(uint32_t)));
-    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (2 * nwt + 8) * sizeof(uint32_t)));
-    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (2 * nwt + 8) * sizeof(uint32_t)));
+        ctx->ws.get(Workspace::kScanSums, scan_blocks(std::max(nbins, ncounts)) * sizeof(uint32_t)));
+    uint32_t* icount = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemCounts, (ncounts + 8) * sizeof(uint32_t)));
+    uint32_t* ioff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kItemOffsets, (ncounts + 8) * sizeof(uint32_t)));
+    const size_t piece_floats2 = static_cast<size_t>(gp.K) * kWin * kWin;
+    float2* partial = static_cast<float2*>(
+        ctx->ws.get(Workspace::kWinPartial, (max_slots * piece_floats2 + 64) * sizeof(float2)));
+    float2* trash = partial + max_slots * piece_floats2;
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
     rec.q = static_cast<float4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(float4)));
     rec.vy = static_cast<float*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float)));
 
-    launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 8, 0.0f);
+    launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
         allow_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
         k_win_count<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
@@ -537,14 +775,11 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     }
     SVDI_LAUNCH_CHECK();
     launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
-    const int waves = kWaves * std::max(1, ctx->cu_count);
-    k_win_item_counts<<<static_cast<unsigned>((nwt + 255) / 256), 256, 0, ctx->stream>>>(boff, wg, waves, icount, totals);
+    k_win_item_counts<<<static_cast<unsigned>((nwt + 255) / 256), 256, 0, ctx->stream>>>(boff, wg, icount, multi, totals);
     SVDI_LAUNCH_CHECK();
-    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, 2 * nwt);
+    launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, ncounts);
     k_win_item_write<<<static_cast<unsigned>((max_items + 255) / 256), 256, 0, ctx->stream>>>(
         boff, ioff, wg, items, static_cast<uint32_t>(max_items), totals);
-    SVDI_LAUNCH_CHECK();
-    k_win_zero_split<<<static_cast<unsigned>(nwt), 256, 0, ctx->stream>>>(boff, totals, wg, gp.G, gp.K, planes);
     SVDI_LAUNCH_CHECK();
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot) {
@@ -560,7 +795,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     constexpr size_t lds = sizeof(float) * kStageFloats * kWaves;
     allow_lds(reinterpret_cast<const void*>(&k_win_grid), lds);
     const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
-    k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes);
+    k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
+    SVDI_LAUNCH_CHECK();
+    k_win_reduce<<<2048, 256, 0, ctx->stream>>>(boff, ioff, multi, totals, wg, gp.G, gp.K,
+                                                static_cast<uint32_t>(max_slots), partial, planes);
     SVDI_LAUNCH_CHECK();
     stage_end(ctx, tok);
     return totals;
