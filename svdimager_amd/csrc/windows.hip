@@ -227,7 +227,7 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
 // totals[0] = records, [1] = wavefront items, [2] = wavefront queue head, [3] = workgroup items,
 // [4] = max |v| bits, [5] = workgroup queue head, [6] = windows with several pieces.
 constexpr int kItemClasses = 8;
-constexpr uint32_t kCoopMin = 1024;
+constexpr uint32_t kCoopMin = 256;
 constexpr uint32_t kCoopCap = 12288;
 
 __device__ __forceinline__ int item_class(uint32_t count, uint32_t cap) {
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(kWaves * 64)
 k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
            WinGeom wg, float2* __restrict__ planes, float2* __restrict__ partial, float2* __restrict__ trash) {
     extern __shared__ __align__(16) float stage_all[];
-    __shared__ int s_item, s_chunk;
+    __shared__ int s_item, s_chunk, s_witem;
     WinLane L;
     const int wave = threadIdx.x >> 6;
     L.lane = threadIdx.x & 63;
@@ -554,22 +554,24 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     // item waiting for the youngest); each claims its next chunk before contracting the current
     // one, and requests that chunk's records at once.
     const int wave_id = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-    if (threadIdx.x == 0) s_chunk = 0;
+    if (threadIdx.x == 0) {
+        s_chunk = 0;
+        s_witem = 0;
+        s_item = static_cast<int>(atomicAdd(&totals[5], 1u));
+    }
     auto claim_chunk = [&]() -> int {
         int v = 0;
         if (lane == 0) v = atomicAdd(&s_chunk, kChunk);
         return __builtin_amdgcn_readfirstlane(v);
     };
     for (;;) {
-        if (wave_id == 0) {
-            uint32_t t = 0;
-            if (lane == 0) t = atomicAdd(&totals[5], 1u);
-            s_item = __builtin_amdgcn_readfirstlane(static_cast<int>(t));
-        }
         __syncthreads();
         const int gi = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (gi >= ncoop) break;
+        // The next item is claimed now; wavefront 0 publishes it after the last barrier of this one.
+        uint32_t gnext = 0;
+        if (wave_id == 0 && lane == 0) gnext = atomicAdd(&totals[5], 1u);
         const WItem it = coop_items[gi];
         for (int kg = 0; kg < K; kg += 8) {
             floatx4 d[kWin];
@@ -635,17 +637,25 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             }
             __syncthreads();
         }
+        if (wave_id == 0 && lane == 0) s_item = static_cast<int>(gnext);
     }
 
-    // Wavefront items.  The queue head is claimed by lane 0 through a pointer the compiler cannot
-    // prove uniform, so the claim stays a single returning atomic whose result is read only when
-    // needed.
-    uint32_t* head = &totals[2];
-    asm volatile("" : "+v"(head));
-    uint32_t ticket = 0;
-    if (lane == 0) ticket = atomicAdd(head, 1u);
-    int item = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
-    if (item >= nwave_items) return;
+    // Wavefront items.  Workgroup g takes items g, g + G, g + 2G, ... of the (largest first) list,
+    // G = the number of workgroups, so the CUs get equal shares without a global queue; its
+    // wavefronts claim them from an LDS counter, each claiming its next item while it contracts
+    // the current one.
+    const int nwg = static_cast<int>(gridDim.x);
+    const int wg_items = nwave_items > static_cast<int>(blockIdx.x)
+                             ? (nwave_items - static_cast<int>(blockIdx.x) + nwg - 1) / nwg : 0;
+    auto claim_item = [&]() -> int {
+        int v = 0;
+        if (lane == 0) v = atomicAdd(&s_witem, 1);
+        return v;
+    };
+    int ticket = claim_item();
+    int j = __builtin_amdgcn_readfirstlane(ticket);
+    if (j >= wg_items) return;
+    int item = static_cast<int>(blockIdx.x) + j * nwg;
     WItem it = items[item];
     // Records of the next chunk, loaded while the current one is contracted.  Every lane loads
     // (lanes past the end re-read the last record and ignore it), so the loads land in place.
@@ -657,7 +667,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
         vy_next = rec.vy[i];
     }
     for (;;) {
-        if (lane == 0) ticket = atomicAdd(head, 1u);
+        ticket = claim_item();
         int next = nwave_items;
         WItem it_next = it;
         const WinOut o = window_out(it, wg, K, G, planes, partial, L.n16);
@@ -680,7 +690,8 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             {
                 int pstart = it.start, pcount = it.count;
                 if (kg + 8 >= K) {
-                    next = __builtin_amdgcn_readfirstlane(static_cast<int>(ticket));
+                    const int jn = __builtin_amdgcn_readfirstlane(ticket);
+                    next = jn < wg_items ? static_cast<int>(blockIdx.x) + jn * nwg : nwave_items;
                     if (next < nwave_items) {
                         it_next = items[next];
                         pstart = it_next.start;
