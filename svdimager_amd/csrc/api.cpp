@@ -255,6 +255,7 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     gp.archive = hp.semantics == Semantics::Archive;
     gp.r3_min = hp.dev.r3_min;
     gp.slab_width = hp.dev.slab_width;
+    gp.inv_slab_width = 1.0 / static_cast<double>(hp.dev.slab_width);
     gp.inv_half = hp.dev.inv_slab_half_range;
     gp.r3_centre = hp.dev.r3_centre;
     gp.scale_f32 = cell * static_cast<float>(G);

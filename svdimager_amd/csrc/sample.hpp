@@ -16,76 +16,20 @@ struct Footprint {
     float2 vw;              // visibility / |det|
 };
 
-template <bool kSnap>
-__device__ __forceinline__ bool setup_sample(const SampleInput& in, const GridParams& gp,
-                                             size_t idx, size_t row, size_t chan, Footprint& f) {
+// Footprint of a sample at grid position (p1, p2) (centred coordinates, cells): the cells
+// ceil(p - 4) .. floor(p + 4) clipped to [-G/2, (G-1)/2] on each axis (SVDIntegratedImager.cu:
+// 1261-1275).  Non-finite positions convert to arbitrary integers: they are rejected, and so is
+// any footprint wider than the 9 cells a finite position can cover.
+__device__ __forceinline__ bool footprint_at(const GridParams& gp, double p1, double p2, Footprint& f) {
     const long gmin = -static_cast<long>(gp.G) / 2;
     const long gmax = (static_cast<long>(gp.G) - 1) / 2;
     const long off = static_cast<long>(gp.G) / 2;
-    float2 v;
-    long lo1, hi1, lo2, hi2;
-    float da, db, s;
-    int slab = 0;
-    if constexpr (kSnap) {
-        const double sc = in.freq[chan] / kSpeedOfLight;
-        const double p1 = in.rot[3 * row] * sc * gp.scale_f64;
-        const double p2 = in.rot[3 * row + 1] * sc * gp.scale_f64;
-        const double r3 = in.rot[3 * row + 2] * sc;
-        double centre;
-        if (gp.archive) {
-            centre = gp.r3_centre;
-        } else {
-            slab = static_cast<int>(floor((r3 - static_cast<double>(gp.r3_min)) / gp.slab_width));
-            slab = min(max(slab, 0), gp.total_slabs - 1);
-            if (slab < gp.slab_start || slab >= gp.slab_start + gp.nslabs) return false;
-            centre = gp.r3_min + (static_cast<float>(slab) + 0.5f) * gp.slab_width;
-        }
-        s = static_cast<float>(fmin(1.0, fmax(-1.0, (r3 - centre) * static_cast<double>(gp.inv_half))));
-        lo1 = max(static_cast<long>(ceil(p1 - 4.0)), gmin);
-        hi1 = min(static_cast<long>(floor(p1 + 4.0)), gmax);
-        lo2 = max(static_cast<long>(ceil(p2 - 4.0)), gmin);
-        hi2 = min(static_cast<long>(floor(p2 + 4.0)), gmax);
-        da = static_cast<float>(static_cast<double>(lo1) - p1);
-        db = static_cast<float>(static_cast<double>(lo2) - p2);
-        f.q1 = -da;
-        f.q2 = -db;
-        const float w0 = in.f0[idx] ? 0.0f : (in.w0 ? in.w0[idx] : 1.0f);
-        const float w3 = in.f3[idx] ? 0.0f : (in.w3 ? in.w3[idx] : 1.0f);
-        const float ws = w0 + w3;
-        if (ws > 0.0f) {
-            v.x = (in.v0r[idx] * w0 + in.v3r[idx] * w3) / ws;
-            v.y = (in.v0i[idx] * w0 + in.v3i[idx] * w3) / ws;
-        } else {
-            return false;                       // collapsed visibility is exactly zero
-        }
-    } else {
-        const float p1 = in.bin[3 * idx] * gp.scale_f32;
-        const float p2 = in.bin[3 * idx + 1] * gp.scale_f32;
-        const float r3 = in.bin[3 * idx + 2];
-        float centre;
-        if (gp.archive) {
-            centre = gp.r3_centre;
-        } else {
-            slab = static_cast<int>(floorf((r3 - gp.r3_min) / gp.slab_width));
-            slab = min(max(slab, 0), gp.total_slabs - 1);
-            if (slab < gp.slab_start || slab >= gp.slab_start + gp.nslabs) return false;
-            centre = gp.r3_min + (static_cast<float>(slab) + 0.5f) * gp.slab_width;
-        }
-        s = fminf(1.0f, fmaxf(-1.0f, (r3 - centre) * gp.inv_half));
-        lo1 = max(static_cast<long>(ceilf(p1 - 4)), gmin);
-        hi1 = min(static_cast<long>(floorf(p1 + 4)), gmax);
-        lo2 = max(static_cast<long>(ceilf(p2 - 4)), gmin);
-        hi2 = min(static_cast<long>(floorf(p2 + 4)), gmax);
-        da = static_cast<float>(lo1) - p1;
-        db = static_cast<float>(lo2) - p2;
-        f.q1 = p1;
-        f.q2 = p2;
-        v.x = in.vis_re[idx];
-        v.y = in.vis_im[idx];
-        if (v.x == 0.0f && v.y == 0.0f) return false;
-    }
-    // Non-finite coordinates convert to arbitrary integers: reject them, and any footprint
-    // wider than the 9 cells a finite position can cover.
+    const long lo1 = max(static_cast<long>(ceil(p1 - 4.0)), gmin);
+    const long hi1 = min(static_cast<long>(floor(p1 + 4.0)), gmax);
+    const long lo2 = max(static_cast<long>(ceil(p2 - 4.0)), gmin);
+    const long hi2 = min(static_cast<long>(floor(p2 + 4.0)), gmax);
+    const float da = static_cast<float>(static_cast<double>(lo1) - p1);
+    const float db = static_cast<float>(static_cast<double>(lo2) - p2);
     if (!(fabsf(da) <= 5.0f) || !(fabsf(db) <= 5.0f)) return false;
     if (lo1 > hi1 || lo2 > hi2 || hi1 - lo1 > 8 || hi2 - lo2 > 8) return false;
     f.a0 = static_cast<int>(lo1 + off);
@@ -94,11 +38,99 @@ __device__ __forceinline__ bool setup_sample(const SampleInput& in, const GridPa
     f.nb = static_cast<int>(hi2 - lo2 + 1);
     f.da = da;
     f.db = db;
-    f.local_slab = slab - (gp.archive ? 0 : gp.slab_start);
-    f.s = s;
-    f.vw.x = v.x / gp.det;
-    f.vw.y = v.y / gp.det;
     return true;
+}
+
+// Snapshot input, one sample in two steps: the loads (load_snapshot_sample), then the
+// arithmetic on them (finish_snapshot_sample), so that a thread can issue the loads of several
+// samples before it needs the first.  (ru, rv, rw) is the sample's rotated row in metres.
+struct SnapLoad {
+    double nu;              // channel frequency, Hz
+    float v0r, v0i, v3r, v3i, w0, w3;
+    uint8_t f0, f3;
+};
+
+__device__ __forceinline__ SnapLoad load_snapshot_sample(const SampleInput& in, size_t idx, size_t chan) {
+    SnapLoad s;
+    s.nu = in.freq[chan];
+    s.v0r = in.v0r[idx];
+    s.v0i = in.v0i[idx];
+    s.v3r = in.v3r[idx];
+    s.v3i = in.v3i[idx];
+    s.f0 = in.f0[idx];
+    s.f3 = in.f3[idx];
+    s.w0 = in.w0 ? in.w0[idx] : 1.0f;
+    s.w3 = in.w3 ? in.w3[idx] : 1.0f;
+    return s;
+}
+
+// Pol collapse (SVDIntegratedImager.cu:588-617): wp = flag_p ? 0 : weight_p, V = (w0 V0 +
+// w3 V3) / (w0 + w3); a sample with no weight is exactly zero and is skipped.  Position
+// b = uvw nu / c in FP64, in cells; slab and Chebyshev argument from r3 (:1249-1254).
+__device__ __forceinline__ bool finish_snapshot_sample(const GridParams& gp, double ru, double rv, double rw,
+                                                       const SnapLoad& s, Footprint& f) {
+    const float w0 = s.f0 ? 0.0f : s.w0;
+    const float w3 = s.f3 ? 0.0f : s.w3;
+    const float ws = w0 + w3;
+    if (!(ws > 0.0f)) return false;
+    const double sc = s.nu * (1.0 / kSpeedOfLight);
+    const double p1 = ru * sc * gp.scale_f64;
+    const double p2 = rv * sc * gp.scale_f64;
+    const double r3 = rw * sc;
+    double centre;
+    int slab = 0;
+    if (gp.archive) {
+        centre = gp.r3_centre;
+    } else {
+        if (gp.total_slabs > 1) {
+            slab = static_cast<int>(floor((r3 - static_cast<double>(gp.r3_min)) * gp.inv_slab_width));
+            slab = min(max(slab, 0), gp.total_slabs - 1);
+        }
+        if (slab < gp.slab_start || slab >= gp.slab_start + gp.nslabs) return false;
+        centre = gp.r3_min + (static_cast<float>(slab) + 0.5f) * gp.slab_width;
+    }
+    if (!footprint_at(gp, p1, p2, f)) return false;
+    f.q1 = -f.da;
+    f.q2 = -f.db;
+    f.local_slab = slab - (gp.archive ? 0 : gp.slab_start);
+    f.s = static_cast<float>(fmin(1.0, fmax(-1.0, (r3 - centre) * static_cast<double>(gp.inv_half))));
+    const float scale = gp.inv_det / ws;
+    f.vw.x = (s.v0r * w0 + s.v3r * w3) * scale;
+    f.vw.y = (s.v0i * w0 + s.v3i * w3) * scale;
+    return true;
+}
+
+template <bool kSnap>
+__device__ __forceinline__ bool setup_sample(const SampleInput& in, const GridParams& gp,
+                                             size_t idx, size_t row, size_t chan, Footprint& f) {
+    if constexpr (kSnap) {
+        return finish_snapshot_sample(gp, in.rot[3 * row], in.rot[3 * row + 1], in.rot[3 * row + 2],
+                                      load_snapshot_sample(in, idx, chan), f);
+    } else {
+        const float p1 = in.bin[3 * idx] * gp.scale_f32;
+        const float p2 = in.bin[3 * idx + 1] * gp.scale_f32;
+        const float r3 = in.bin[3 * idx + 2];
+        float centre;
+        int slab = 0;
+        if (gp.archive) {
+            centre = gp.r3_centre;
+        } else {
+            slab = static_cast<int>(floorf((r3 - gp.r3_min) / gp.slab_width));
+            slab = min(max(slab, 0), gp.total_slabs - 1);
+            if (slab < gp.slab_start || slab >= gp.slab_start + gp.nslabs) return false;
+            centre = gp.r3_min + (static_cast<float>(slab) + 0.5f) * gp.slab_width;
+        }
+        const float2 v = make_float2(in.vis_re[idx], in.vis_im[idx]);
+        if (v.x == 0.0f && v.y == 0.0f) return false;
+        if (!footprint_at(gp, p1, p2, f)) return false;
+        f.q1 = p1;
+        f.q2 = p2;
+        f.local_slab = slab - (gp.archive ? 0 : gp.slab_start);
+        f.s = fminf(1.0f, fmaxf(-1.0f, (r3 - centre) * gp.inv_half));
+        f.vw.x = v.x / gp.det;
+        f.vw.y = v.y / gp.det;
+        return true;
+    }
 }
 
 }  // namespace svdi

@@ -108,6 +108,7 @@ struct GridParams {
     double scale_f64;       // cell * G / c (snapshot input)
     float inv_det;          // 1 / |det| (vis weighting)
     float det;
+    double inv_slab_width;  // 1 / slab_width
 };
 
 // Grown-on-demand device buffers owned by a context.

@@ -77,39 +77,49 @@ struct WRecords {
     float* vy;
 };
 
-// Samples of this block.  Snapshot input: 256 consecutive rows, each split over 4 threads that
-// take every fourth channel (a wavefront reads 64 consecutive rows of one channel: coalesced).
-// Preprocessed input: 16384 consecutive samples, 16 per thread at a stride of the block size.
+// Every (sample, bin) pair of this block's samples: one row class h = a0 / 8, and the window
+// columns b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
+// Snapshot input: 256 consecutive rows, each split over 4 threads that take every fourth
+// channel (a wavefront reads 64 consecutive rows of one channel: coalesced), loading four
+// samples before working on the first.  Preprocessed input: 16384 consecutive samples, 16 per
+// thread at a stride of the block size.
 template <bool kSnap, class Fn>
-__device__ __forceinline__ void for_block_samples(const SampleInput& in, Fn&& fn) {
+__device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
+                                                const WinGeom& wg, Fn&& fn) {
+    auto bins_of = [&](const Footprint& f) {
+        const int h = f.a0 / kClass;
+        const int col0 = f.local_slab * wg.nw;
+        for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
+            fn(f, (col0 + wb) * wg.nh + h, h, wb);
+    };
     if constexpr (kSnap) {
+        constexpr int kBatch = 4;
         const size_t row = blockIdx.x * (size_t)kBinRows + (threadIdx.x % kBinRows);
         if (row >= in.num_rows) return;
-        for (size_t chan = threadIdx.x / kBinRows; chan < in.num_channels; chan += kBinChanGroups)
-            fn(row + chan * in.num_rows, row, chan);
+        const double ru = in.rot[3 * row], rv = in.rot[3 * row + 1], rw = in.rot[3 * row + 2];
+        for (size_t c0 = threadIdx.x / kBinRows; c0 < in.num_channels; c0 += kBatch * kBinChanGroups) {
+            SnapLoad ld[kBatch] = {};
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const size_t chan = c0 + u * kBinChanGroups;
+                if (chan < in.num_channels) ld[u] = load_snapshot_sample(in, row + chan * in.num_rows, chan);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const size_t chan = c0 + u * kBinChanGroups;
+                Footprint f;
+                if (chan < in.num_channels && finish_snapshot_sample(gp, ru, rv, rw, ld[u], f)) bins_of(f);
+            }
+        }
     } else {
         const size_t base = blockIdx.x * (size_t)kBinThreads * kBinPerThread + threadIdx.x;
         for (int j = 0; j < kBinPerThread; ++j) {
             const size_t idx = base + static_cast<size_t>(j) * kBinThreads;
             if (idx >= in.num_samples) return;
-            fn(idx, 0, 0);
+            Footprint f;
+            if (setup_sample<false>(in, gp, idx, 0, 0, f)) bins_of(f);
         }
     }
-}
-
-// Every (sample, bin) pair of this block: one row class h = a0 / 8, and the window columns
-// b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
-template <bool kSnap, class Fn>
-__device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
-                                                const WinGeom& wg, Fn&& fn) {
-    for_block_samples<kSnap>(in, [&](size_t idx, size_t row, size_t chan) {
-        Footprint f;
-        if (!setup_sample<kSnap>(in, gp, idx, row, chan, f)) return;
-        const int h = f.a0 / kClass;
-        const int col0 = f.local_slab * wg.nw;
-        for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
-            fn(f, (col0 + wb) * wg.nh + h, h, wb);
-    });
 }
 
 // Largest |Re v|, |Im v| of the block, folded into *vmax_bits (non-negative floats order as
