@@ -16,13 +16,15 @@
 // a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
 // nothing overflows f16; the scales are undone exactly at write-out.
 //
-// Binning by row class: a sample whose (clipped) first footprint row a0 lies in [8h, 8h + 8) is
-// in row class h, and its footprint rows lie in [8h, 8h + 16).  Window row wa (rows
-// [16 wa, 16 wa + 16)) therefore reads the classes 2 wa - 1 (its rows 0-7 only), 2 wa (all rows)
-// and 2 wa + 1 (rows 8-15 only).  Records are binned by (slab, window column, row class) with
-// the class fastest, so a window's records are one contiguous range ordered by class, each
-// record is written once per window column it meets, and a chunk of 32 records drawn from one
-// or two classes skips the window half that none of them touches.  Counting uses block-local
+// Binning by row class: with class height c (4, or 8 when the finer histogram does not fit in
+// LDS), a sample whose (clipped) first footprint row a0 lies in [c h, c h + c) is in row class h,
+// and its footprint rows lie in [c h, c h + c + 8).  Window row wa (rows [16 wa, 16 wa + 16))
+// therefore reads the 24 / c classes h = (16 / c) wa + d, d = -8 / c .. 16 / c - 1, class d
+// touching only the window rows [c d, c d + c + 7].  Records are binned by (slab, window
+// column, row class) with the class fastest, so a window's records are one contiguous range
+// ordered by class, each record is written once per window column it meets, and a chunk of 32
+// records drawn from neighbouring classes skips the 4-row quarters of the window that none of
+// them touches (with c = 4, 12 of 16 rows per class).  Counting uses block-local
 // LDS histograms kept per block; after a scan of the bin totals each block reserves its ranges
 // with one atomic per bin and one more pass over its samples writes the records (position
 // relative to the class / window-column origin, s, v).
@@ -33,10 +35,13 @@ namespace svdi {
 namespace {
 
 constexpr int kWin = 16;                 // window side in cells
-constexpr int kClass = 8;                // row-class height in cells
-constexpr int kBinThreads = 1024;        // binning blocks: 256 rows x 4 channel groups
-constexpr int kBinRows = 256;
-constexpr int kBinChanGroups = kBinThreads / kBinRows;
+constexpr int kMaxClasses = 6;           // classes per window: 24 / class height (4 or 8)
+constexpr int kBinThreads = 1024;        // binning blocks: 256 rows x 4 channel groups at a time
+constexpr int kBinRowSets = 2;           // row sets of 256 per block (fewer, larger blocks: the
+                                         // per-block histograms are written and read back whole)
+constexpr int kBinRowsPerSet = 256;
+constexpr int kBinRows = kBinRowSets * kBinRowsPerSet;
+constexpr int kBinChanGroups = kBinThreads / kBinRowsPerSet;
 constexpr int kBinPerThread = 16;        // samples per thread, preprocessed input
 constexpr int kLdsBins = 36864;          // largest block-local histogram (144 KB of LDS)
 constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
@@ -55,19 +60,22 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct WinGeom {
     int nw;          // windows per grid axis
-    int nh;          // row classes: ceil(G / 8)
+    int cls_log2;    // log2 of the row-class height (2 or 3)
+    int ncls;        // classes per window: 24 >> cls_log2
+    int nh;          // row classes: ceil(G / class height)
     int nwin;        // nw * nw
     int nwin_total;  // nslabs * nwin
     int nbins;       // nslabs * nw * nh, ordered (slab, window column, row class)
     int use_lds;     // histogram in LDS (nbins <= kLdsBins)
 };
 
-// A work item: records [start, start + count) of window `win`; records before b1 are of class
-// 2 wa - 1, records from b2 on of class 2 wa + 1, the rest of class 2 wa.  slot >= 0: the item is
-// one piece of a window split over several workgroup items and writes its partial window to
-// partial slot `slot`; slot < 0: it writes the window to the planes.
+// A work item: records [start, start + count) of window `win`; bnd[j] (j = 1 .. ncls - 1) is the
+// first record of the window's class j (d = j - 8 / c), records before bnd[1] are of class 0.
+// slot >= 0: the item is one piece of a window split over several workgroup items and writes its
+// partial window to partial slot `slot`; slot < 0: it writes the window to the planes.
 struct WItem {
-    int win, start, count, slot, b1, b2;
+    int win, start, count, slot;
+    int bnd[kMaxClasses];
 };
 
 // Records, structure of arrays: q = (p1 - row-class origin, p2 - window column origin, s, Re v)
@@ -79,25 +87,26 @@ struct WRecords {
 
 // Every (sample, bin) pair of this block's samples: one row class h = a0 / 8, and the window
 // columns b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
-// Snapshot input: 256 consecutive rows, each split over 4 threads that take every fourth
-// channel (a wavefront reads 64 consecutive rows of one channel: coalesced), loading four
-// samples before working on the first.  Preprocessed input: 16384 consecutive samples, 16 per
+// Snapshot input: 512 consecutive rows in two sets of 256, each row split over 4 threads that
+// take every fourth channel (a wavefront reads 64 consecutive rows of one channel: coalesced),
+// loading four samples before working on the first.  Preprocessed input: 16384 consecutive samples, 16 per
 // thread at a stride of the block size.
 template <bool kSnap, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
                                                 const WinGeom& wg, Fn&& fn) {
     auto bins_of = [&](const Footprint& f) {
-        const int h = f.a0 / kClass;
+        const int h = f.a0 >> wg.cls_log2;
         const int col0 = f.local_slab * wg.nw;
         for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
             fn(f, (col0 + wb) * wg.nh + h, h, wb);
     };
     if constexpr (kSnap) {
         constexpr int kBatch = 4;
-        const size_t row = blockIdx.x * (size_t)kBinRows + (threadIdx.x % kBinRows);
+        for (int set = 0; set < kBinRowSets; ++set) {
+        const size_t row = blockIdx.x * (size_t)kBinRows + set * kBinRowsPerSet + (threadIdx.x % kBinRowsPerSet);
         if (row >= in.num_rows) return;
         const double ru = in.rot[3 * row], rv = in.rot[3 * row + 1], rw = in.rot[3 * row + 2];
-        for (size_t c0 = threadIdx.x / kBinRows; c0 < in.num_channels; c0 += kBatch * kBinChanGroups) {
+        for (size_t c0 = threadIdx.x / kBinRowsPerSet; c0 < in.num_channels; c0 += kBatch * kBinChanGroups) {
             SnapLoad ld[kBatch] = {};
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
@@ -110,6 +119,7 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
                 Footprint f;
                 if (chan < in.num_channels && finish_snapshot_sample(gp, ru, rv, rw, ld[u], f)) bins_of(f);
             }
+        }
         }
     } else {
         const size_t base = blockIdx.x * (size_t)kBinThreads * kBinPerThread + threadIdx.x;
@@ -180,10 +190,10 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         if (slot >= limit) return;
         float p1r, p2r;
         if constexpr (kSnap) {
-            p1r = f.q1 + static_cast<float>(f.a0 - h * kClass);
+            p1r = f.q1 + static_cast<float>(f.a0 - (h << wg.cls_log2));
             p2r = f.q2 + static_cast<float>(f.b0 - wb * kWin);
         } else {
-            p1r = f.q1 - static_cast<float>(h * kClass - off);
+            p1r = f.q1 - static_cast<float>((h << wg.cls_log2) - off);
             p2r = f.q2 - static_cast<float>(wb * kWin - off);
         }
         rec.q[slot] = make_float4(p1r, p2r, f.s, f.vw.x);
@@ -206,21 +216,23 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
     });
 }
 
-// Record range of window w (slab, wa, wb): the classes 2 wa - 1 .. 2 wa + 1 of column wb that
-// exist, with b1 / b2 the first records of classes 2 wa and 2 wa + 1.  2 wa <= nh - 1 always
-// (nh = ceil(G / 8) >= 2 ceil(G / 16) - 1).
+// Record range of window w (slab, wa, wb): bnd[j] = first record of the window's class j
+// (h = (16 / c) wa - 8 / c + j, clamped to the column's classes [0, nh)), j = 0 .. ncls;
+// bnd[0] = start, bnd[ncls] = end.
 struct WinRange {
-    uint32_t start, end, b1, b2;
+    uint32_t bnd[kMaxClasses + 1];
+    __device__ uint32_t start() const { return bnd[0]; }
 };
-__device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ boff, const WinGeom& wg, int w) {
+__device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ boff, const WinGeom& wg, int w, uint32_t& end) {
     const int ls = w / wg.nwin, r = w - ls * wg.nwin;
     const int wa = r / wg.nw, wb = r - wa * wg.nw;
     const int base = (ls * wg.nw + wb) * wg.nh;
+    const int h0 = (wa << (4 - wg.cls_log2)) - (8 >> wg.cls_log2);
     WinRange x;
-    x.start = boff[base + max(2 * wa - 1, 0)];
-    x.b1 = boff[base + 2 * wa];
-    x.b2 = boff[base + min(2 * wa + 1, wg.nh)];
-    x.end = boff[base + min(2 * wa + 2, wg.nh)];
+#pragma unroll
+    for (int j = 0; j <= kMaxClasses; ++j)
+        x.bnd[j] = j <= wg.ncls ? boff[base + min(max(h0 + j, 0), wg.nh)] : 0u;
+    end = x.bnd[wg.ncls];
     return x;
 }
 
@@ -256,8 +268,9 @@ k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, uint32_t* __res
     const int nwt = wg.nwin_total;
     if (w == 0) totals[0] = boff[wg.nbins];
     if (w >= nwt) return;
-    const WinRange x = window_range(boff, wg, w);
-    const uint32_t c = x.end - x.start;
+    uint32_t end;
+    const WinRange x = window_range(boff, wg, w, end);
+    const uint32_t c = end - x.start();
     for (int k = 0; k <= 2 * kItemClasses; ++k) icount[k * nwt + w] = 0u;
     if (c <= kCoopMin) {
         icount[item_class(c, kCoopMin) * nwt + w] = 1u;
@@ -300,20 +313,21 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
     const int j = coop ? lo - kItemClasses * nwt : lo;
     const int cls = j / nwt;
     const int w = j - cls * nwt;
-    const WinRange x = window_range(boff, wg, w);
-    const uint32_t c = x.end - x.start;
+    uint32_t end;
+    const WinRange x = window_range(boff, wg, w, end);
+    const uint32_t c = end - x.start();
     WItem t;
     t.win = w;
-    t.b1 = static_cast<int>(x.b1);
-    t.b2 = static_cast<int>(x.b2);
-    t.start = static_cast<int>(x.start);
+#pragma unroll
+    for (int j = 0; j < kMaxClasses; ++j) t.bnd[j] = static_cast<int>(x.bnd[j]);
+    t.start = static_cast<int>(x.start());
     t.count = static_cast<int>(c);
     t.slot = -1;
     if (coop) {
         const uint32_t full = c / kCoopCap;
         const uint32_t q = i - ioff[lo];
         const uint32_t piece = (cls == 0 && q < full) ? q : full;
-        t.start = static_cast<int>(x.start + piece * kCoopCap);
+        t.start = static_cast<int>(x.start() + piece * kCoopCap);
         t.count = static_cast<int>(piece < full ? kCoopCap : c - full * kCoopCap);
         if (coop_pieces(c) > 1) t.slot = static_cast<int>(ioff[n + w] - ioff[n] + piece);
     }
@@ -333,8 +347,9 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const int w = static_cast<int>(multi[u / K]);
         const int k = static_cast<int>(u % K);
-        const WinRange x = window_range(boff, wg, w);
-        const uint32_t pieces = coop_pieces(x.end - x.start);
+        uint32_t end;
+        const WinRange x = window_range(boff, wg, w, end);
+        const uint32_t pieces = coop_pieces(end - x.start());
         const uint32_t first = ioff[n + w] - ioff[n];
         if (first + pieces > max_slots) continue;          // never: the slot count bounds the pieces
         const int ls = w / wg.nwin, r = w - ls * wg.nwin;
@@ -413,13 +428,24 @@ struct WinLane {
 // One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q, vy) contracted
 // into the accumulators d for the terms kg .. kg + 7.
 __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
-                                             const float4& q, float vy, floatx4 (&d)[kWin]) {
+                                             int cls_log2, int ncls, const float4& q, float vy, floatx4 (&d)[kWin]) {
     // ES(x) 2^7 = exp2(c (sqrt(1 - x^2) - 1) + 7), c = beta log2(e).
     constexpr float kEsC = kBeta * 1.4426950408889634f;
     const bool valid = c0 + L.rl < it.count;
-    // Window halves touched by the chunk's classes (wave-uniform).
+    // Classes of the chunk's first and last records (wave-uniform), and the 4-row quarters of
+    // the window that they touch: class j covers rows [c d, c d + c + 7], d = j - 8 / c.
     const int i0 = it.start + c0, i1 = min(it.start + it.count, i0 + kChunk);
-    const bool top = i0 < it.b2, bottom = i1 > it.b1;
+    int j_first = 0, j_last = 0;
+#pragma unroll
+    for (int j = 1; j < kMaxClasses; ++j) {
+        if (j < ncls) {
+            j_first += i0 >= it.bnd[j] ? 1 : 0;
+            j_last += i1 - 1 >= it.bnd[j] ? 1 : 0;
+        }
+    }
+    const int cls = 1 << cls_log2, dlo = -(8 >> cls_log2);
+    const int q_lo = max(0, ((j_first + dlo) * cls) >> 2);
+    const int q_hi = min(3, ((j_last + dlo) * cls + cls + 7) >> 2);
     // ES factors along this lane's axis: zero the window's 16 rows / columns, then write the 9
     // cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside the window; the ninth
     // is nonzero only for an integer p).
@@ -427,8 +453,11 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     for (int a = 0; a < kWin; ++a) L.s_k[(a + kPad) * kStride] = 0.0f;
     if (valid) {
         const int i = i0 + L.rl;
-        const float shift = i < it.b1 ? -static_cast<float>(kClass)
-                                      : (i < it.b2 ? 0.0f : static_cast<float>(kClass));
+        int jc = 0;
+#pragma unroll
+        for (int j = 1; j < kMaxClasses; ++j)
+            if (j < ncls) jc += i >= it.bnd[j] ? 1 : 0;
+        const float shift = static_cast<float>((jc + dlo) * cls);
         const float pr = L.axis ? q.y : q.x + shift;
         const float fl = ceilf(pr - 4.0f);
         const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
@@ -473,13 +502,12 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
     const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
     const float* kub = L.s_ku + kPad * kStride + L.g8;
-    if (top) {
 #pragma unroll
-        for (int a = 0; a < kWin / 2; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
-    }
-    if (bottom) {
+    for (int qd = 0; qd < 4; ++qd) {
+        if (qd >= q_lo && qd <= q_hi) {
 #pragma unroll
-        for (int a = kWin / 2; a < kWin; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+            for (int a = 4 * qd; a < 4 * qd + 4; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+        }
     }
     wave_lds_sync();
 }
@@ -604,7 +632,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     q_next = rec.q[i];
                     vy_next = rec.vy[i];
                 }
-                window_chunk(L, it, c0, kg, K, q, vy, d);
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
                 c0 = c1;
             }
             // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
@@ -693,7 +721,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     q_next = rec.q[i];
                     vy_next = rec.vy[i];
                 }
-                window_chunk(L, it, c0, kg, K, q, vy, d);
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
             }
             // What comes next: this item's next term block, or the next item.  Its first
             // records are requested now, ahead of the stores below.
@@ -741,7 +769,13 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
     WinGeom wg;
     wg.nw = (gp.G + kWin - 1) / kWin;
-    wg.nh = (gp.G + kClass - 1) / kClass;
+    // Row classes of 4 cells, or of 8 when only the coarser histogram fits in LDS.
+    const size_t cols = static_cast<size_t>(gp.nslabs) * wg.nw;
+    const bool fits4 = cols * ((gp.G + 3) / 4) <= static_cast<size_t>(kLdsBins);
+    const bool fits8 = cols * ((gp.G + 7) / 8) <= static_cast<size_t>(kLdsBins);
+    wg.cls_log2 = (!fits4 && fits8) ? 3 : 2;
+    wg.ncls = 24 >> wg.cls_log2;
+    wg.nh = (gp.G + (1 << wg.cls_log2) - 1) >> wg.cls_log2;
     wg.nwin = wg.nw * wg.nw;
     wg.nwin_total = wg.nwin * gp.nslabs;
     const size_t nbins_sz = static_cast<size_t>(gp.nslabs) * wg.nw * wg.nh;
