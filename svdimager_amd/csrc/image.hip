@@ -81,10 +81,19 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     double jn[KMAX];
     bessel_sequence_fixed<KMAX>(alpha, jn);
     // Plane element read by this pixel: linear index G*G/2 + G/2 + idyr*G + idxr of a G x G
-    // plane (SVDIntegratedImager.cu:1397), located in the transformed window.
-    const long G = ip.G;
-    const long lin = (G * G) / 2 + G / 2 + idyr * G + idxr;
-    const long gr = lin / G - ip.row_lo, gc = lin % G - ip.col_lo;
+    // plane (SVDIntegratedImager.cu:1397), located in the transformed window.  With
+    // G*G/2 + G/2 = q0 G + r0 (0 <= r0 < G) and |idxr| < G, the row and column are
+    // q0 + idyr and r0 + idxr with at most one carry.
+    const int G = ip.G;
+    int lr = ip.lin_q0 + static_cast<int>(idyr), lc = ip.lin_r0 + static_cast<int>(idxr);
+    if (lc < 0) {
+        lc += G;
+        --lr;
+    } else if (lc >= G) {
+        lc -= G;
+        ++lr;
+    }
+    const long gr = lr - ip.row_lo, gc = lc - ip.col_lo;
     const size_t plane_stride = static_cast<size_t>(ip.rows) * ip.cols;
     const float2* px = planes + static_cast<size_t>(gr) * ip.cols + gc;
     double sum = 0.0;
@@ -143,7 +152,12 @@ void launch_conv_corr(svdi_context* ctx, size_t N, size_t G, int /*archive_varia
 }
 
 void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev, const float* conv,
-                  const ImageParams& ip, float* image) {
+                  const ImageParams& ip_in, float* image) {
+    ImageParams ip = ip_in;
+    const long long G = ip.G;
+    const long long base = (G * G) / 2 + G / 2;
+    ip.lin_q0 = static_cast<int>(base / G);
+    ip.lin_r0 = static_cast<int>(base % G);
     dim3 block(16, 16);
     dim3 grid(static_cast<unsigned>((ip.N + 15) / 16), static_cast<unsigned>((ip.N + 15) / 16));
     if (ip.K <= 4) k_image<4><<<grid, block, 0, ctx->stream>>>(planes, ip, vin_dev, conv, image);

@@ -214,6 +214,7 @@ struct ImageParams {
     int rows, cols;         // window sizes of the transformed planes
     int row_lo, col_lo;
     int signed_out;
+    int lin_q0, lin_r0;     // (G G / 2 + G / 2) = lin_q0 G + lin_r0 (set by launch_image)
 };
 void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev,
                   const float* conv, const ImageParams& ip, float* image);

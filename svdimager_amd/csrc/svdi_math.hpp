@@ -98,14 +98,39 @@ constexpr double kBesselSeriesMax = 1e-2;
 // bessel_sequence (SVDIntegratedImager.cu:901-920), which loses digits to cancellation near x = 10.
 // J_0..J_{COUNT-1}(x) with every index known at compile time, so the sequence stays in
 // registers on the device: Miller's backward recurrence runs from m down to COUNT + 1 without
-// stores, then through the COUNT stored terms unrolled.  Same operations, in the same order, as
-// the runtime-count form below.
+// stores, then through the COUNT stored terms unrolled.  As the runtime-count form below, except
+// that for COUNT <= 16 and 1e-2 <= |x| < 2 the recurrence starts at the fixed m = COUNT + 12 (even)
+// and runs fully unrolled: there J_m(x) / J_{COUNT-1}(x) < 1e-13, and the recurrence grows by
+// less than (2m / |x|)^m < 1e110, so it needs no rescaling.
 template <int COUNT>
 SVDI_HD void bessel_sequence_fixed(double x, double* j) {
     const double ax = fabs(x);
     if (ax < kBesselSeriesMax) {
 #pragma unroll
         for (int n = 0; n < COUNT; ++n) j[n] = bessel_series(n, ax);
+    } else if (COUNT <= 16 && ax < 2.0) {
+        constexpr int m = 2 * ((COUNT + 13) / 2);
+        const double two_over_x = 2.0 / ax;
+        double jp = 0.0, jc = 1.0, norm = 0.0;
+#pragma unroll
+        for (int k = m; k > COUNT; --k) {
+            const double jm = (k * two_over_x) * jc - jp;   // J_{k-1}
+            jp = jc;
+            jc = jm;
+            if (((k - 1) & 1) == 0) norm += 2.0 * jc;
+        }
+#pragma unroll
+        for (int k = COUNT; k > 0; --k) {
+            const double jm = (k * two_over_x) * jc - jp;
+            jp = jc;
+            jc = jm;
+            j[k - 1] = jc;
+            if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2.0 * jc;
+        }
+        norm += jc;                                        // + J_0
+        const double inv = 1.0 / norm;
+#pragma unroll
+        for (int n = 0; n < COUNT; ++n) j[n] *= inv;
     } else if (ax >= 1000.0) {
         double j0, j1;
         bessel_j01_asymptotic(ax, &j0, &j1);

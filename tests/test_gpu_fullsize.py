@@ -1,0 +1,41 @@
+"""GPU cross-checks at the full SKA-Low shape (512 stations, 64 channels, 8,372,224 samples,
+1024^2, G = 1536): the MFMA window gridder against the FP32 tile gridder, both already pinned
+to the oracle at small sizes.  These sizes reach code paths the small cases cannot:
+
+* hour angle 0: one slab, 4-row record classes, windows split over several workgroup pieces
+  whose partial windows are summed in piece order;
+* hour angle 4 h: two slabs and more Chebyshev terms (two or more eight-term passes), where
+  only the 8-row class histogram fits in LDS.
+
+The two gridders differ in summation order and in the MFMA's split-f16 operands (~22
+significant bits per product), so the signed images agree to the gridder tolerance of the
+small cases: 1e-5 of peak over the central half, 1e-4 over the image.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce, assert_parity
+import svdimager_amd as S
+from svdimager_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ha", [0.0, 4.0])
+def test_skalow_mfma_vs_tiled(gpu_ctx, ha):
+    cfg = synth.SKALOW
+    announce(f"SKA-Low full size, HA {ha} h: MFMA vs tiled FP32")
+    snap = synth.make_snapshot(cfg, ha_hours=ha, device="cuda")
+    d = S.DeviceSnapshot(snap, "cuda")
+    N, cell = cfg.image_size, cfg.cell
+    img_t, plan_t = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_TILED)
+    img_m, plan_m = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_MFMA)
+    torch.cuda.synchronize()
+    assert plan_m["num_terms"] == plan_t["num_terms"] and plan_m["num_slabs"] == plan_t["num_slabs"]
+    print(f"HA {ha}: slabs {plan_m['num_slabs']} K {plan_m['num_terms']} records {plan_m['records']}")
+    if ha > 0:
+        assert plan_m["num_slabs"] >= 2 and plan_m["num_terms"] > 8
+    a, b = img_m.cpu().numpy(), img_t.cpu().numpy()
+    assert np.isfinite(a).all() and np.abs(b).max() > 0
+    assert_parity(a, b, f"MFMA vs tiled, HA {ha} h")
