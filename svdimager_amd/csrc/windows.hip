@@ -46,16 +46,20 @@ constexpr int kBinPerThread = 16;        // samples per thread, preprocessed inp
 constexpr int kLdsBins = 36864;          // largest block-local histogram (144 KB of LDS)
 constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
 constexpr int kChunk = 32;               // samples per MFMA reduction step
-constexpr int kStride = 36;              // floats per staging row (32 samples + pad)
-constexpr int kPad = 8;                  // staging rows beyond each window edge (-8 .. 23)
+constexpr int kStride = 36;              // floats per c staging row (32 samples + pad)
+constexpr int kStrideH = 40;             // halves per ES staging row (32 samples + pad, 16-byte rows)
+constexpr int kPad = 8;                  // ES staging rows beyond each window edge (-8 .. 23)
 constexpr int kStageRows = kWin + 2 * kPad;
-constexpr int kStageFloats = (2 * kStageRows + kWin) * kStride;   // ku, kv padded; c[16]
+constexpr int kPlaneHalves = kStageRows * kStrideH;               // one ES plane (hi or lo)
+// Per wavefront: ku hi / lo and kv hi / lo planes (f16), then c[16] (f32).
+constexpr int kStageFloats = (4 * kPlaneHalves) / 2 + kWin * kStride;
 constexpr int kMinItem = 2048;
 constexpr int kMaxItem = 16384;
 constexpr float kEsScaleLog2 = 7.0f;     // ES factors scaled by 2^7 per axis
 constexpr int kEsScaleLog2Sq = 14;
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct WinGeom {
@@ -400,16 +404,21 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One window row a: B_a[r][b] = ku_r[a] kv_r[b] split into hi / lo, D_a += C B_a as
-// hi*hi + lo*hi + hi*lo.
-__device__ __forceinline__ void window_row(floatx4& d, const float* __restrict__ kurow, const float4& kv0,
-                                           const float4& kv1, const half8& a_hi, const half8& a_lo) {
-    const float4 ku0 = *reinterpret_cast<const float4*>(kurow);
-    const float4 ku1 = *reinterpret_cast<const float4*>(kurow + 4);
-    const float4 x0 = make_float4(ku0.x * kv0.x, ku0.y * kv0.y, ku0.z * kv0.z, ku0.w * kv0.w);
-    const float4 x1 = make_float4(ku1.x * kv1.x, ku1.y * kv1.y, ku1.z * kv1.z, ku1.w * kv1.w);
-    half8 b_hi, b_lo;
-    split_hilo(x0, x1, b_hi, b_lo);
+// One window row a: B_a[r][b] = ku_r[a] kv_r[b] from the factors split into f16 hi + lo
+// (ku = uh + ul, kv = vh + vl), in packed f16: P = f16(uh vh), E = uh vh - P exactly (an fma of
+// 11-bit significands), lo = f16(uh vl + ul vh + E), hi = P, so |ku kv - hi - lo| <=
+// 2^-11 |lo| + |ul vl| ~ 2^-22 |ku kv| (products below 2^-24 / 2^-22 of the largest lose bits to
+// f16 subnormals: contributions under 1e-12 of the peak).  Then D_a += C B_a as hi*hi + lo*hi +
+// hi*lo.
+__device__ __forceinline__ void window_row(floatx4& d, const _Float16* __restrict__ uh_row,
+                                           const _Float16* __restrict__ ul_row, const half8& vh, const half8& vl,
+                                           const half8& a_hi, const half8& a_lo) {
+    const half8 uh = *reinterpret_cast<const half8*>(uh_row);
+    const half8 ul = *reinterpret_cast<const half8*>(ul_row);
+    const half8 p = uh * vh;
+    const half8 e = __builtin_elementwise_fma(uh, vh, -p);
+    const half8 b_lo = __builtin_elementwise_fma(ul, vh, __builtin_elementwise_fma(uh, vl, e));
+    const half8 b_hi = p;
     d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d, 0, 0, 0);
     d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d, 0, 0, 0);
     d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d, 0, 0, 0);
@@ -417,10 +426,13 @@ __device__ __forceinline__ void window_row(floatx4& d, const float* __restrict__
 
 // Per-wavefront constants of the window kernel.
 struct WinLane {
-    float* s_ku;        // [32 rows a = -8 .. 23][32 samples]
-    float* s_kv;        // [32 columns b = -8 .. 23][32 samples]
+    _Float16* s_kuh;    // ku hi, lo: [32 rows a = -8 .. 23][32 samples], f16
+    _Float16* s_kul;
+    _Float16* s_kvh;    // kv hi, lo: [32 columns b = -8 .. 23][32 samples], f16
+    _Float16* s_kvl;
     float* s_c;         // [16 m][32 samples]
-    float* s_k;         // this lane's staging column (ku or kv)
+    _Float16* s_kh;     // this lane's axis (ku or kv): hi and lo planes
+    _Float16* s_kl;
     int lane, rl, axis, g8, n16;
     float vscale, inv;
 };
@@ -446,11 +458,16 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     const int cls = 1 << cls_log2, dlo = -(8 >> cls_log2);
     const int q_lo = max(0, ((j_first + dlo) * cls) >> 2);
     const int q_hi = min(3, ((j_last + dlo) * cls + cls + 7) >> 2);
-    // ES factors along this lane's axis: zero the window's 16 rows / columns, then write the 9
-    // cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside the window; the ninth
-    // is nonzero only for an integer p).
+    // ES factors along this lane's axis, as f16 hi + lo: the axis' 32 lanes zero the window's
+    // 16 rows / columns of both planes (one 64-byte row each), then every lane writes its
+    // sample's 9 cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside the
+    // window; the ninth is nonzero only for an integer p).
+    {
+        _Float16* zrow = ((L.rl & 16) ? L.s_kl : L.s_kh) - L.rl + (kPad + (L.rl & 15)) * kStrideH;
+        const uint4 z = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-    for (int a = 0; a < kWin; ++a) L.s_k[(a + kPad) * kStride] = 0.0f;
+        for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
+    }
     if (valid) {
         const int i = i0 + L.rl;
         int jc = 0;
@@ -462,13 +479,17 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         const float fl = ceilf(pr - 4.0f);
         const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
         const float x0 = (fl - pr) * 0.25f;
-        float* dst = L.s_k + (first + kPad) * kStride;
+        _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
+        _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
             const float x = x0 + 0.25f * static_cast<float>(j);
             const float t = fmaf(-x, x, 1.0f);
-            const float v = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
-            dst[j * kStride] = t >= 0.0f ? v : 0.0f;
+            const float e = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
+            const float v = t >= 0.0f ? e : 0.0f;
+            const _Float16 h = static_cast<_Float16>(v);
+            dh[j * kStrideH] = h;
+            dl[j * kStrideH] = static_cast<_Float16>(v - static_cast<float>(h));
         }
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
@@ -498,15 +519,16 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         const float* row = L.s_c + L.n16 * kStride + L.g8;
         split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4), a_hi, a_lo);
     }
-    const float* kvrow = L.s_kv + (kPad + L.n16) * kStride + L.g8;
-    const float4 kv0 = *reinterpret_cast<const float4*>(kvrow);
-    const float4 kv1 = *reinterpret_cast<const float4*>(kvrow + 4);
-    const float* kub = L.s_ku + kPad * kStride + L.g8;
+    const half8 vh = *reinterpret_cast<const half8*>(L.s_kvh + (kPad + L.n16) * kStrideH + L.g8);
+    const half8 vl = *reinterpret_cast<const half8*>(L.s_kvl + (kPad + L.n16) * kStrideH + L.g8);
+    const _Float16* uhb = L.s_kuh + kPad * kStrideH + L.g8;
+    const _Float16* ulb = L.s_kul + kPad * kStrideH + L.g8;
 #pragma unroll
     for (int qd = 0; qd < 4; ++qd) {
         if (qd >= q_lo && qd <= q_hi) {
 #pragma unroll
-            for (int a = 4 * qd; a < 4 * qd + 4; ++a) window_row(d[a], kub + a * kStride, kv0, kv1, a_hi, a_lo);
+            for (int a = 4 * qd; a < 4 * qd + 4; ++a)
+                window_row(d[a], uhb + a * kStrideH, ulb + a * kStrideH, vh, vl, a_hi, a_lo);
         }
     }
     wave_lds_sync();
@@ -561,14 +583,17 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     WinLane L;
     const int wave = threadIdx.x >> 6;
     L.lane = threadIdx.x & 63;
-    L.s_ku = stage_all + wave * kStageFloats;
-    L.s_kv = L.s_ku + kStageRows * kStride;
-    L.s_c = L.s_kv + kStageRows * kStride;
+    L.s_kuh = reinterpret_cast<_Float16*>(stage_all + wave * kStageFloats);
+    L.s_kul = L.s_kuh + kPlaneHalves;
+    L.s_kvh = L.s_kul + kPlaneHalves;
+    L.s_kvl = L.s_kvh + kPlaneHalves;
+    L.s_c = reinterpret_cast<float*>(L.s_kvl + kPlaneHalves);
     L.rl = L.lane & 31;
     L.axis = L.lane >> 5;
     L.g8 = 8 * (L.lane >> 4);
     L.n16 = L.lane & 15;
-    L.s_k = (L.axis ? L.s_kv : L.s_ku) + L.rl;
+    L.s_kh = (L.axis ? L.s_kvh : L.s_kuh) + L.rl;
+    L.s_kl = (L.axis ? L.s_kvl : L.s_kul) + L.rl;
     // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
     {
         const float vmax = __uint_as_float(totals[4]);
