@@ -5,13 +5,13 @@
 // For a window W and the samples r whose footprint meets it, every (term k, re/im, cell) of W is
 //   D[m][(a, b)] = sum_r c[m][r] * ku_r[a] * kv_r[b],   m = 2k + ri,
 //   c[2k + ri][r] = T_k(s_r) * (Re | Im) v_r / |det|,   ku_r[a] = ES((a - p1_r) / 4) (0 off support),
-// with (a, b) the window's cells and p_r the sample's position relative to the window.  One
-// wavefront owns one window: for each window row a it runs D_a = C * B_a on
-// v_mfma_f32_16x16x32_f16, with C = c (16 x 32 samples) and B_a[r][b] = ku_r[a] kv_r[b]
-// (32 samples x 16 columns), the 16 row blocks (64 FP32 accumulators per lane) staying in
-// registers across all of the window's samples.  The window is written to the planes once, so
-// no atomics touch the accumulation (windows split over several work items add atomically).
-// Precision: every FP32 operand is split into f16 hi + lo and each product taken as
+// with (a, b) the window's cells and p_r the sample's position relative to the window.  For each
+// window row a, D_a = C * B_a runs on v_mfma_f32_16x16x32_f16, with C = c (16 x 32 samples) and
+// B_a[r][b] = ku_r[a] kv_r[b] (32 samples x 16 columns), the 16 row blocks (64 FP32 accumulators
+// per lane) staying in registers across the window's samples.  A window is written once: by the
+// wavefront or workgroup that contracted it, or, when it was split over several workgroup
+// pieces, as partial windows that k_win_reduce adds in piece order; no atomics touch the planes.
+// Precision: c and the ES factors are split into f16 hi + lo and each product taken as
 // hi*hi + lo*hi + hi*lo with FP32 accumulation (~22 significant bits per product); c is scaled by
 // a power of two fixed from the largest |v| of the call and the ES factors by 2^7 each, so
 // nothing overflows f16; the scales are undone exactly at write-out.
