@@ -53,8 +53,6 @@ constexpr int kStageRows = kWin + 2 * kPad;
 constexpr int kPlaneHalves = kStageRows * kStrideH;               // one ES plane (hi or lo)
 // Per wavefront: ku hi / lo and kv hi / lo planes (f16), then c[16] (f32).
 constexpr int kStageFloats = (4 * kPlaneHalves) / 2 + kWin * kStride;
-constexpr int kMinItem = 2048;
-constexpr int kMaxItem = 16384;
 constexpr float kEsScaleLog2 = 7.0f;     // ES factors scaled by 2^7 per axis
 constexpr int kEsScaleLog2Sq = 14;
 
@@ -250,7 +248,7 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
 // workgroups finish together.
 // icount: [kItemClasses][nwin_total] wavefront items, [kItemClasses][nwin_total] workgroup items,
 // [nwin_total] partial slots; one exclusive scan gives the queue positions and slot numbers.
-// totals[0] = records, [1] = wavefront items, [2] = wavefront queue head, [3] = workgroup items,
+// totals[0] = records, [1] = wavefront items, [2] = (unused), [3] = workgroup items,
 // [4] = max |v| bits, [5] = workgroup queue head, [6] = windows with several pieces.
 constexpr int kItemClasses = 8;
 constexpr uint32_t kCoopMin = 256;
