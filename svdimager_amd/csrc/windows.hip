@@ -80,11 +80,24 @@ struct WItem {
     int bnd[kMaxClasses];
 };
 
-// Records, structure of arrays: q = (p1 - row-class origin, p2 - window column origin, s, Re v)
-// in cells, vy = Im v.
+// Records, 20 bytes each, packed: (p1 - row-class origin, p2 - window column origin, s, Re v,
+// Im v) in cells.  One record is one dword-aligned 16-byte access plus one 4-byte access, so the
+// scatter writes each record as one contiguous run.
+typedef float float4u __attribute__((ext_vector_type(4), aligned(4)));
 struct WRecords {
-    float4* q;
-    float* vy;
+    float* r;           // 5 floats per record
+
+    __device__ __forceinline__ void put(uint32_t slot, float4 q, float vy) const {
+        float* p = r + 5 * static_cast<size_t>(slot);
+        *reinterpret_cast<float4u*>(p) = float4u{q.x, q.y, q.z, q.w};
+        p[4] = vy;
+    }
+    __device__ __forceinline__ void get(int i, float4& q, float& vy) const {
+        const float* p = r + 5 * static_cast<size_t>(i);
+        const float4u v = *reinterpret_cast<const float4u*>(p);
+        q = make_float4(v.x, v.y, v.z, v.w);
+        vy = p[4];
+    }
 };
 
 // Every (sample, bin) pair of this block's samples: one row class h = a0 / 8, and the window
@@ -198,8 +211,7 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
             p1r = f.q1 - static_cast<float>((h << wg.cls_log2) - off);
             p2r = f.q2 - static_cast<float>(wb * kWin - off);
         }
-        rec.q[slot] = make_float4(p1r, p2r, f.s, f.vw.x);
-        rec.vy[slot] = f.vw.y;
+        rec.put(slot, make_float4(p1r, p2r, f.s, f.vw.x), f.vw.y);
     };
     if (!wg.use_lds) {
         for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
@@ -643,8 +655,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             float vy_next = 0.0f;
             if (c0 < it.count) {
                 const int i = it.start + min(c0 + L.rl, it.count - 1);
-                q_next = rec.q[i];
-                vy_next = rec.vy[i];
+                rec.get(i, q_next, vy_next);
             }
             while (c0 < it.count) {
                 const float4 q = q_next;
@@ -652,8 +663,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                 const int c1 = claim_chunk();
                 if (c1 < it.count) {
                     const int i = it.start + min(c1 + L.rl, it.count - 1);
-                    q_next = rec.q[i];
-                    vy_next = rec.vy[i];
+                    rec.get(i, q_next, vy_next);
                 }
                 window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
                 c0 = c1;
@@ -724,8 +734,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     float vy_next;
     {
         const int i = it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0;
-        q_next = rec.q[i];
-        vy_next = rec.vy[i];
+        rec.get(i, q_next, vy_next);
     }
     for (;;) {
         ticket = claim_item();
@@ -741,8 +750,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                 const float vy = vy_next;
                 if (c0 + kChunk < it.count) {
                     const int i = it.start + min(c0 + kChunk + L.rl, it.count - 1);
-                    q_next = rec.q[i];
-                    vy_next = rec.vy[i];
+                    rec.get(i, q_next, vy_next);
                 }
                 window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
             }
@@ -762,8 +770,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     }
                 }
                 const int i = pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0;
-                q_next = rec.q[i];
-                vy_next = rec.vy[i];
+                rec.get(i, q_next, vy_next);
             }
             // Rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16) and
             // kg + 2 (lane / 16) + 1; column n16.
@@ -840,8 +847,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
-    rec.q = static_cast<float4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(float4)));
-    rec.vy = static_cast<float*>(ctx->ws.get(Workspace::kRecVim, max_records * sizeof(float)));
+    rec.r = static_cast<float*>(ctx->ws.get(Workspace::kRecPos, max_records * 5 * sizeof(float)));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
