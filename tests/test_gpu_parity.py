@@ -143,18 +143,20 @@ def test_multislab_and_batching_vs_oracle(gpu_ctx, small_snap, oracle_pre):
                           central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
 
 
-@pytest.mark.parametrize("N", [6, 10, 14, 33])
-def test_odd_grid_sizes_vs_oracle(gpu_ctx, small_snap, oracle_pre, N):
-    """Image sizes whose grid G = ceil(1.5 N) is odd or whose N is odd."""
-    announce(f"odd sizes N={N}")
+@pytest.mark.parametrize("gridder", [S.GRIDDER_ATOMIC, S.GRIDDER_TILED, S.GRIDDER_MFMA])
+@pytest.mark.parametrize("N", [6, 10, 14, 33, 90, 170])
+def test_odd_grid_sizes_vs_oracle(gpu_ctx, small_snap, oracle_pre, N, gridder):
+    """Image sizes whose grid G = ceil(1.5 N) is odd or whose N is odd; G = 9 .. 255, so the
+    MFMA gridder's last 16-cell window and the tiled gridder's last tile are partial."""
+    announce(f"odd sizes N={N} gridder={gridder}")
     cell = small_snap["cell"] * 128 / N
     t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
          for k in ("bin", "vin", "vis_re", "vis_im")}
     ref = O.fipipe_device_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
                                 oracle_pre["vis_im"], N, cell, num_terms=4)
     img, _ = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N, cell,
-                            output=S.OUTPUT_SIGNED, num_terms=4, gridder=S.GRIDDER_ATOMIC)
-    assert_parity(img.cpu().numpy(), ref["signed"], f"N={N} G={(3 * N + 1) // 2}")
+                            output=S.OUTPUT_SIGNED, num_terms=4, gridder=gridder)
+    assert_parity(img.cpu().numpy(), ref["signed"], f"N={N} G={(3 * N + 1) // 2} gridder={gridder}")
 
 
 @pytest.mark.parametrize("gridder", [S.GRIDDER_TILED, S.GRIDDER_MFMA])
