@@ -310,6 +310,96 @@ k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, FftStages st
             dst[static_cast<size_t>(r) * cols + c] = res[r * pn + pidx(col_lo + c)];
 }
 
+// Row pass for G = 1536 = 24 x 64, one wavefront per row, the row held in registers.  With
+// n = 64 n1 + l and k = k1 + 24 k2 (Cooley-Tukey, W = e^{+2 pi i / 1536}):
+//   lane l:  Y[k1] = W^{l k1} sum_{n1} x[64 n1 + l] W_24^{n1 k1}   (24-point DFT as 8 x 3),
+//   X[k1 + 24 k2] = sum_l Y_l[k1] W_64^{l k2},
+// the 64-point DFTs across lanes done as 8 x 8 (l = 8a + b, k2 = c + 8d) through one padded LDS
+// tile per wavefront: pass A takes (k1, b) and transforms over a, times W_64^{b c}; pass B takes
+// (k1, c) and transforms over b, giving X[k1 + 24 c + 192 d].  Loads (stride 64 per lane) and
+// stores (consecutive k across lanes) are coalesced; only the columns col_lo .. col_lo + cols - 1
+// are stored.
+constexpr int kRowWaves = 4;
+constexpr int kTileStride = 65;              // float2 per k1 row of the LDS tile (64 + 1 pad)
+
+__global__ void __launch_bounds__(64 * kRowWaves)
+k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int total_rows, int col_lo, int cols) {
+    constexpr int G = 1536;
+    __shared__ float2 tile_all[kRowWaves][24 * kTileStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * kRowWaves + wave;
+    if (row >= total_rows) return;                 // whole wavefronts; no workgroup barrier below
+    float2* tile = tile_all[wave];
+    const float2* src = in + static_cast<size_t>(row) * G;
+    float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) y[v][u] = src[64 * (3 * u + v) + lane];
+    // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
+    // 3-point DFTs over v.  W_24^m = tw[64 m].
+#pragma unroll
+    for (int v = 0; v < 3; ++v) dft8(y[v]);
+#pragma unroll
+    for (int v = 1; v < 3; ++v)
+#pragma unroll
+        for (int s8 = 1; s8 < 8; ++s8) y[v][s8] = cmul(y[v][s8], tw[64 * v * s8]);
+#pragma unroll
+    for (int s8 = 0; s8 < 8; ++s8) {
+        float2 t3[3] = {y[0][s8], y[1][s8], y[2][s8]};
+        dft3(t3);
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int k1 = s8 + 8 * t;
+            // times W^{l k1} (l k1 < 1536), into the tile at [k1][l]
+            tile[k1 * kTileStride + lane] = k1 == 0 ? t3[t] : cmul(t3[t], tw[lane * k1]);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Pass A: q = lane + 64 j -> (k1, b) = (q % 24, q / 24); 8-point DFT over a, times W_64^{b c}.
+    float2 w[3][8];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int q = lane + 64 * j, k1 = q % 24, b = q / 24;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) w[j][a] = tile[k1 * kTileStride + 8 * a + b];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int q = lane + 64 * j, k1 = q % 24, b = q / 24;
+        dft8(w[j]);
+#pragma unroll
+        for (int c = 1; c < 8; ++c) w[j][c] = cmul(w[j][c], tw[24 * b * c]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) tile[k1 * kTileStride + 8 * b + c] = w[j][c];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Pass B: p = lane + 64 j -> (k1, c) = (p % 24, p / 24); 8-point DFT over b gives
+    // X[p + 192 d].
+    float2* dst = out + static_cast<size_t>(row) * cols - col_lo;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int p = lane + 64 * j, k1 = p % 24, c = p / 24;
+        float2 z[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) z[b] = tile[k1 * kTileStride + 8 * b + c];
+        dft8(z);
+#pragma unroll
+        for (int d = 0; d < 8; ++d) {
+            const int k = p + 192 * d;
+            if (k >= col_lo && k < col_lo + cols) dst[k] = z[d];
+        }
+    }
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin in launch order, each XCD with its own L2.
 // Remapped so that consecutive logical tiles (neighbouring column strips, which share cache
 // lines) run on the same XCD at about the same time.
@@ -472,9 +562,16 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cols_fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    dim3 gr(static_cast<unsigned>((G + batch - 1) / batch), static_cast<unsigned>(P));
-    rows_fn<<<gr, threads, lds, ctx->stream>>>(planes, row_out, st, tw, batch, static_cast<int>(col_lo),
-                                               static_cast<int>(cols));
+    if (G == 1536) {
+        const int total_rows = static_cast<int>(P * G);
+        k_fft_rows_1536<<<static_cast<unsigned>((total_rows + kRowWaves - 1) / kRowWaves), 64 * kRowWaves, 0,
+                          ctx->stream>>>(planes, row_out, tw, total_rows, static_cast<int>(col_lo),
+                                         static_cast<int>(cols));
+    } else {
+        dim3 gr(static_cast<unsigned>((G + batch - 1) / batch), static_cast<unsigned>(P));
+        rows_fn<<<gr, threads, lds, ctx->stream>>>(planes, row_out, st, tw, batch, static_cast<int>(col_lo),
+                                                   static_cast<int>(cols));
+    }
     SVDI_LAUNCH_CHECK();
     const int strips = static_cast<int>((cols + width - 1) / width);
     const unsigned gc = static_cast<unsigned>(strips * P);
