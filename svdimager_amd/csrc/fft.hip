@@ -310,33 +310,31 @@ k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, FftStages st
             dst[static_cast<size_t>(r) * cols + c] = res[r * pn + pidx(col_lo + c)];
 }
 
-// Row pass for G = 1536 = 24 x 64, one wavefront per row, the row held in registers.  With
+// One 1536-point inverse transform per wavefront (G = 24 x 64), held in registers.  With
 // n = 64 n1 + l and k = k1 + 24 k2 (Cooley-Tukey, W = e^{+2 pi i / 1536}):
 //   lane l:  Y[k1] = W^{l k1} sum_{n1} x[64 n1 + l] W_24^{n1 k1}   (24-point DFT as 8 x 3),
 //   X[k1 + 24 k2] = sum_l Y_l[k1] W_64^{l k2},
 // the 64-point DFTs across lanes done as 8 x 8 (l = 8a + b, k2 = c + 8d) through one padded LDS
 // tile per wavefront: pass A takes (k1, b) and transforms over a, times W_64^{b c}; pass B takes
-// (k1, c) and transforms over b, giving X[k1 + 24 c + 192 d].  Loads (stride 64 per lane) and
-// stores (consecutive k across lanes) are coalesced; only the columns col_lo .. col_lo + cols - 1
-// are stored.
-constexpr int kRowWaves = 4;
+// (k1, c) and transforms over b, giving X[k1 + 24 c + 192 d].  load(n) reads x[n] (lane l asks
+// for n = 64 n1 + l); store(k, X[k]) is called for every k, lane l holding k = l + 64 j + 192 d.
+constexpr int kFftWaves = 4;                 // wavefronts (transforms) per workgroup
 constexpr int kTileStride = 65;              // float2 per k1 row of the LDS tile (64 + 1 pad)
 
-__global__ void __launch_bounds__(64 * kRowWaves)
-k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-                int total_rows, int col_lo, int cols) {
-    constexpr int G = 1536;
-    __shared__ float2 tile_all[kRowWaves][24 * kTileStride];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int row = blockIdx.x * kRowWaves + wave;
-    if (row >= total_rows) return;                 // whole wavefronts; no workgroup barrier below
-    float2* tile = tile_all[wave];
-    const float2* src = in + static_cast<size_t>(row) * G;
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class Load, class Store>
+__device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restrict__ tw, int lane,
+                                             Load&& load, Store&& store) {
     float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-        for (int v = 0; v < 3; ++v) y[v][u] = src[64 * (3 * u + v) + lane];
+        for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
     // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
     // 3-point DFTs over v.  W_24^m = tw[64 m].
 #pragma unroll
@@ -356,9 +354,7 @@ k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
             tile[k1 * kTileStride + lane] = k1 == 0 ? t3[t] : cmul(t3[t], tw[lane * k1]);
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_fence();
     // Pass A: q = lane + 64 j -> (k1, b) = (q % 24, q / 24); 8-point DFT over a, times W_64^{b c}.
     float2 w[3][8];
 #pragma unroll
@@ -367,9 +363,7 @@ k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
 #pragma unroll
         for (int a = 0; a < 8; ++a) w[j][a] = tile[k1 * kTileStride + 8 * a + b];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_fence();
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const int q = lane + 64 * j, k1 = q % 24, b = q / 24;
@@ -379,12 +373,8 @@ k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
 #pragma unroll
         for (int c = 0; c < 8; ++c) tile[k1 * kTileStride + 8 * b + c] = w[j][c];
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // Pass B: p = lane + 64 j -> (k1, c) = (p % 24, p / 24); 8-point DFT over b gives
-    // X[p + 192 d].
-    float2* dst = out + static_cast<size_t>(row) * cols - col_lo;
+    wave_lds_fence();
+    // Pass B: p = lane + 64 j -> (k1, c) = (p % 24, p / 24); 8-point DFT over b gives X[p + 192 d].
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const int p = lane + 64 * j, k1 = p % 24, c = p / 24;
@@ -393,11 +383,27 @@ k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
         for (int b = 0; b < 8; ++b) z[b] = tile[k1 * kTileStride + 8 * b + c];
         dft8(z);
 #pragma unroll
-        for (int d = 0; d < 8; ++d) {
-            const int k = p + 192 * d;
-            if (k >= col_lo && k < col_lo + cols) dst[k] = z[d];
-        }
+        for (int d = 0; d < 8; ++d) store(p + 192 * d, z[d]);
     }
+}
+
+// Row pass for G = 1536: one wavefront per row of the P x G rows; loads stride 64 per lane and
+// stores of consecutive k across lanes are coalesced; only the columns col_lo .. col_lo + cols - 1
+// are stored.
+__global__ void __launch_bounds__(64 * kFftWaves)
+k_fft_rows_1536(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int total_rows, int col_lo, int cols) {
+    constexpr int G = 1536;
+    __shared__ float2 tile_all[kFftWaves][24 * kTileStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * kFftWaves + wave;
+    if (row >= total_rows) return;                 // whole wavefronts; no workgroup barrier below
+    const float2* src = in + static_cast<size_t>(row) * G;
+    float2* dst = out + static_cast<size_t>(row) * cols - col_lo;
+    fft1536_wave(tile_all[wave], tw, lane, [&](int n) { return src[n]; },
+                 [&](int k, float2 v) {
+                     if (k >= col_lo && k < col_lo + cols) dst[k] = v;
+                 });
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin in launch order, each XCD with its own L2.
@@ -564,7 +570,7 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     if (G == 1536) {
         const int total_rows = static_cast<int>(P * G);
-        k_fft_rows_1536<<<static_cast<unsigned>((total_rows + kRowWaves - 1) / kRowWaves), 64 * kRowWaves, 0,
+        k_fft_rows_1536<<<static_cast<unsigned>((total_rows + kFftWaves - 1) / kFftWaves), 64 * kFftWaves, 0,
                           ctx->stream>>>(planes, row_out, tw, total_rows, static_cast<int>(col_lo),
                                          static_cast<int>(cols));
     } else {
