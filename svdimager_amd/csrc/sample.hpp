@@ -50,13 +50,20 @@ struct SnapLoad {
     uint8_t f0, f3;
 };
 
+// kVis = false loads only what decides the footprint and whether the sample is gridded
+// (frequency, flags, weights); the visibility fields are then zero.
+template <bool kVis = true>
 __device__ __forceinline__ SnapLoad load_snapshot_sample(const SampleInput& in, size_t idx, size_t chan) {
     SnapLoad s;
     s.nu = in.freq[chan];
-    s.v0r = in.v0r[idx];
-    s.v0i = in.v0i[idx];
-    s.v3r = in.v3r[idx];
-    s.v3i = in.v3i[idx];
+    if constexpr (kVis) {
+        s.v0r = in.v0r[idx];
+        s.v0i = in.v0i[idx];
+        s.v3r = in.v3r[idx];
+        s.v3i = in.v3i[idx];
+    } else {
+        s.v0r = s.v0i = s.v3r = s.v3i = 0.0f;
+    }
     s.f0 = in.f0[idx];
     s.f3 = in.f3[idx];
     s.w0 = in.w0 ? in.w0[idx] : 1.0f;

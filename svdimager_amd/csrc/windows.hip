@@ -104,9 +104,10 @@ struct WRecords {
 // columns b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
 // Snapshot input: 512 consecutive rows in two sets of 256, each row split over 4 threads that
 // take every fourth channel (a wavefront reads 64 consecutive rows of one channel: coalesced),
-// loading four samples before working on the first.  Preprocessed input: 16384 consecutive samples, 16 per
-// thread at a stride of the block size.
-template <bool kSnap, class Fn>
+// loading four samples before working on the first.  Preprocessed input: 16384 consecutive
+// samples, 16 per thread at a stride of the block size.  kVis = false (snapshot input): the visibilities are not
+// loaded and f.vw is zero (the bins do not depend on them).
+template <bool kSnap, bool kVis, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
                                                 const WinGeom& wg, Fn&& fn) {
     auto bins_of = [&](const Footprint& f) {
@@ -126,7 +127,7 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const size_t chan = c0 + u * kBinChanGroups;
-                if (chan < in.num_channels) ld[u] = load_snapshot_sample(in, row + chan * in.num_rows, chan);
+                if (chan < in.num_channels) ld[u] = load_snapshot_sample<kVis>(in, row + chan * in.num_rows, chan);
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
@@ -167,17 +168,15 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
-            uint32_t* __restrict__ blockhist, uint32_t* __restrict__ vmax_bits) {
+            uint32_t* __restrict__ blockhist) {
     extern __shared__ uint32_t hist[];
     if (wg.use_lds) {
         for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
         __syncthreads();
     }
-    float m = 0.0f;
-    for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int, int) {
+    for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
         if (wg.use_lds) atomicAdd(&hist[bin], 1u);
         else atomicAdd(&counts[bin], 1u);
-        m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
     });
     __syncthreads();
     if (wg.use_lds) {
@@ -188,7 +187,6 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
             if (c != 0) atomicAdd(&counts[b], c);
         }
     }
-    block_max_to(m, vmax_bits);
 }
 
 // Records of this block's samples, in one pass: with LDS histograms the block reserves each
@@ -198,10 +196,12 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ blockhist,
-              uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit) {
+              uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit, uint32_t* __restrict__ vmax_bits) {
     extern __shared__ uint32_t hist[];
     const int off = gp.G / 2;
+    float m = 0.0f;
     auto emit = [&](uint32_t slot, const Footprint& f, int h, int wb) {
+        m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
         if (slot >= limit) return;
         float p1r, p2r;
         if constexpr (kSnap) {
@@ -214,9 +214,10 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         rec.put(slot, make_float4(p1r, p2r, f.s, f.vw.x), f.vw.y);
     };
     if (!wg.use_lds) {
-        for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+        for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
             emit(atomicAdd(&cursor[bin], 1u), f, h, wb);
         });
+        block_max_to(m, vmax_bits);
         return;
     }
     const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
@@ -225,9 +226,10 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
     }
     __syncthreads();
-    for_sample_bins<kSnap>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+    for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
         emit(atomicAdd(&hist[bin], 1u), f, h, wb);
     });
+    block_max_to(m, vmax_bits);
 }
 
 // Record range of window w (slab, wa, wb): bnd[j] = first record of the window's class j
@@ -852,10 +854,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
         allow_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
-        k_win_count<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+        k_win_count<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     } else {
         allow_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
-        k_win_count<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, totals + 4);
+        k_win_count<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     }
     SVDI_LAUNCH_CHECK();
     launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
@@ -868,10 +870,12 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot) {
         allow_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
-        k_win_scatter<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
+        k_win_scatter<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
+                                                                         totals + 4);
     } else {
         allow_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
-        k_win_scatter<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit);
+        k_win_scatter<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
+                                                                          totals + 4);
     }
     SVDI_LAUNCH_CHECK();
 
