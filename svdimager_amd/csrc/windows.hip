@@ -30,6 +30,7 @@
 // relative to the class / window-column origin, s, v).
 #include "sample.hpp"
 #include "scan.hpp"
+#include "es_poly.hpp"
 
 namespace svdi {
 namespace {
@@ -453,8 +454,6 @@ struct WinLane {
 // into the accumulators d for the terms kg .. kg + 7.
 __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
                                              int cls_log2, int ncls, const float4& q, float vy, floatx4 (&d)[kWin]) {
-    // ES(x) 2^7 = exp2(c (sqrt(1 - x^2) - 1) + 7), c = beta log2(e).
-    constexpr float kEsC = kBeta * 1.4426950408889634f;
     const bool valid = c0 + L.rl < it.count;
     // Classes of the chunk's first and last records (wave-uniform), and the 4-row quarters of
     // the window that they touch: class j covers rows [c d, c d + c + 7], d = j - 8 / c.
@@ -490,18 +489,30 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         const float pr = L.axis ? q.y : q.x + shift;
         const float fl = ceilf(pr - 4.0f);
         const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
-        const float x0 = (fl - pr) * 0.25f;
+        // Cell j is at x = (f + j) / 4 - 1, f = fl - pr + 4 in [0, 1]: for j < 8 the quarter-support
+        // polynomial pair (j, 7 - j) in t = f - 1/2, for j = 8 ES(1) when f = 0 and zero otherwise.
+        const float dd = fl - pr;
+        const float t = dd + 3.5f, t2 = t * t;
+        float v[9];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float e = kEsPolyEven[j][kEsPolyDegree / 2];
+#pragma unroll
+            for (int i = kEsPolyDegree / 2 - 1; i >= 0; --i) e = fmaf(e, t2, kEsPolyEven[j][i]);
+            float o = kEsPolyOdd[j][kEsPolyDegree / 2 - 1];
+#pragma unroll
+            for (int i = kEsPolyDegree / 2 - 2; i >= 0; --i) o = fmaf(o, t2, kEsPolyOdd[j][i]);
+            v[j] = fmaf(t, o, e);
+            v[7 - j] = fmaf(-t, o, e);
+        }
+        v[8] = dd <= -4.0f ? kEsPolyEdge : 0.0f;
         _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
         _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
 #pragma unroll
         for (int j = 0; j < 9; ++j) {
-            const float x = x0 + 0.25f * static_cast<float>(j);
-            const float t = fmaf(-x, x, 1.0f);
-            const float e = __builtin_amdgcn_exp2f(fmaf(kEsC, __builtin_amdgcn_sqrtf(t), kEsScaleLog2 - kEsC));
-            const float v = t >= 0.0f ? e : 0.0f;
-            const _Float16 h = static_cast<_Float16>(v);
+            const _Float16 h = static_cast<_Float16>(v[j]);
             dh[j * kStrideH] = h;
-            dl[j * kStrideH] = static_cast<_Float16>(v - static_cast<float>(h));
+            dl[j * kStrideH] = static_cast<_Float16>(v[j] - static_cast<float>(h));
         }
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
