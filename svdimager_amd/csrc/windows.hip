@@ -56,9 +56,13 @@ constexpr int kPlaneHalves = kStageRows * kStrideH;               // one ES plan
 constexpr int kStageFloats = (4 * kPlaneHalves) / 2 + kWin * kStride;
 constexpr float kEsScaleLog2 = 7.0f;     // ES factors scaled by 2^7 per axis
 constexpr int kEsScaleLog2Sq = 14;
+// ES(1) 2^7, the ninth footprint cell of an integer position, as f16 hi + lo.
+constexpr _Float16 kEsPolyEdgeHi = static_cast<_Float16>(kEsPolyEdge);
+constexpr _Float16 kEsPolyEdgeLo = static_cast<_Float16>(kEsPolyEdge - static_cast<float>(kEsPolyEdgeHi));
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct WinGeom {
@@ -73,7 +77,8 @@ struct WinGeom {
 };
 
 // A work item: records [start, start + count) of window `win`; bnd[j] (j = 1 .. ncls - 1) is the
-// first record of the window's class j (d = j - 8 / c), records before bnd[1] are of class 0.
+// first record of the window's class j (d = j - 8 / c), records before bnd[1] are of class 0,
+// and bnd[j] for j >= ncls is the window's end.
 // slot >= 0: the item is one piece of a window split over several workgroup items and writes its
 // partial window to partial slot `slot`; slot < 0: it writes the window to the planes.
 struct WItem {
@@ -205,11 +210,14 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
         if (slot >= limit) return;
         float p1r, p2r;
+        // Row position relative to the class's home window row (c h + 8) / 16: the row window
+        // that reads the class at d < 8 / c (the row above reads it at d >= 8 / c).
+        const int home = ((h << wg.cls_log2) + 8) >> 4;
         if constexpr (kSnap) {
-            p1r = f.q1 + static_cast<float>(f.a0 - (h << wg.cls_log2));
+            p1r = f.q1 + static_cast<float>(f.a0 - home * kWin);
             p2r = f.q2 + static_cast<float>(f.b0 - wb * kWin);
         } else {
-            p1r = f.q1 - static_cast<float>((h << wg.cls_log2) - off);
+            p1r = f.q1 - static_cast<float>(home * kWin - off);
             p2r = f.q2 - static_cast<float>(wb * kWin - off);
         }
         rec.put(slot, make_float4(p1r, p2r, f.s, f.vw.x), f.vw.y);
@@ -235,7 +243,7 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
 
 // Record range of window w (slab, wa, wb): bnd[j] = first record of the window's class j
 // (h = (16 / c) wa - 8 / c + j, clamped to the column's classes [0, nh)), j = 0 .. ncls;
-// bnd[0] = start, bnd[ncls] = end.
+// bnd[0] = start, bnd[j] = end for j >= ncls.
 struct WinRange {
     uint32_t bnd[kMaxClasses + 1];
     __device__ uint32_t start() const { return bnd[0]; }
@@ -248,7 +256,7 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
     WinRange x;
 #pragma unroll
     for (int j = 0; j <= kMaxClasses; ++j)
-        x.bnd[j] = j <= wg.ncls ? boff[base + min(max(h0 + j, 0), wg.nh)] : 0u;
+        x.bnd[j] = boff[base + min(max(h0 + min(j, wg.ncls), 0), wg.nh)];
     end = x.bnd[wg.ncls];
     return x;
 }
@@ -395,19 +403,19 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
     }
 }
 
-// x = hi + lo: hi rounds toward zero (one packed conversion per two values), lo = x - hi
-// rounded to nearest (the compiler fuses the product that forms x into a mixed-precision fma),
-// so |x - hi - lo| <= 2^-11 |lo| < 2^-21 |x|.
-typedef __fp16 fp16x2 __attribute__((ext_vector_type(2)));
+// x = hi + lo, both rounded to nearest by packed conversions (two values per instruction):
+// |x - hi - lo| <= 2^-12 |lo| <= 2^-23 |x| (x in f16's normal range).
 __device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
     const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
     for (int i = 0; i < 8; i += 2) {
-        const fp16x2 h = __builtin_amdgcn_cvt_pkrtz(v[i], v[i + 1]);
-        hi[i] = static_cast<_Float16>(h[0]);
-        hi[i + 1] = static_cast<_Float16>(h[1]);
-        lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h[0]));
-        lo[i + 1] = static_cast<_Float16>(v[i + 1] - static_cast<float>(h[1]));
+        const float2v x = (float2v){v[i], v[i + 1]};
+        const half2v h = __builtin_convertvector(x, half2v);
+        const half2v l = __builtin_convertvector(x - __builtin_convertvector(h, float2v), half2v);
+        hi[i] = h[0];
+        hi[i + 1] = h[1];
+        lo[i] = l[0];
+        lo[i + 1] = l[1];
     }
 }
 
@@ -453,18 +461,18 @@ struct WinLane {
 // One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q, vy) contracted
 // into the accumulators d for the terms kg .. kg + 7.
 __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
-                                             int cls_log2, int ncls, const float4& q, float vy, floatx4 (&d)[kWin]) {
+                                             int cls_log2, const float4& q, float vy, floatx4 (&d)[kWin]) {
     const bool valid = c0 + L.rl < it.count;
     // Classes of the chunk's first and last records (wave-uniform), and the 4-row quarters of
     // the window that they touch: class j covers rows [c d, c d + c + 7], d = j - 8 / c.
     const int i0 = it.start + c0, i1 = min(it.start + it.count, i0 + kChunk);
+    // (i >= bnd as the sign of bnd - 1 - i, so that the sums stay in scalar registers; bnd[j] for
+    // j >= ncls is the window's end, which no record reaches.)
     int j_first = 0, j_last = 0;
 #pragma unroll
     for (int j = 1; j < kMaxClasses; ++j) {
-        if (j < ncls) {
-            j_first += i0 >= it.bnd[j] ? 1 : 0;
-            j_last += i1 - 1 >= it.bnd[j] ? 1 : 0;
-        }
+        j_first += static_cast<int>((static_cast<uint32_t>(it.bnd[j]) + ~static_cast<uint32_t>(i0)) >> 31);
+        j_last += static_cast<int>((static_cast<uint32_t>(it.bnd[j]) - static_cast<uint32_t>(i1)) >> 31);
     }
     const int cls = 1 << cls_log2, dlo = -(8 >> cls_log2);
     const int q_lo = max(0, ((j_first + dlo) * cls) >> 2);
@@ -479,21 +487,20 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
     }
+    // A record's row position is relative to its class's home window row (the first window row
+    // that reads the class); this window reads the classes j >= 16 / c as the row above their
+    // home, 16 rows up.
+    const int foreign = cls_log2 == 2 ? it.bnd[4] : it.bnd[2];
     if (valid) {
         const int i = i0 + L.rl;
-        int jc = 0;
-#pragma unroll
-        for (int j = 1; j < kMaxClasses; ++j)
-            if (j < ncls) jc += i >= it.bnd[j] ? 1 : 0;
-        const float shift = static_cast<float>((jc + dlo) * cls);
-        const float pr = L.axis ? q.y : q.x + shift;
+        const float pr = L.axis ? q.y : q.x + (i >= foreign ? static_cast<float>(kWin) : 0.0f);
         const float fl = ceilf(pr - 4.0f);
         const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
         // Cell j is at x = (f + j) / 4 - 1, f = fl - pr + 4 in [0, 1]: for j < 8 the quarter-support
         // polynomial pair (j, 7 - j) in t = f - 1/2, for j = 8 ES(1) when f = 0 and zero otherwise.
         const float dd = fl - pr;
         const float t = dd + 3.5f, t2 = t * t;
-        float v[9];
+        float v[8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float e = kEsPolyEven[j][kEsPolyDegree / 2];
@@ -505,15 +512,22 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
             v[j] = fmaf(t, o, e);
             v[7 - j] = fmaf(-t, o, e);
         }
-        v[8] = dd <= -4.0f ? kEsPolyEdge : 0.0f;
         _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
         _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
+        // hi and lo of the pair (j, 7 - j) by two packed round-to-nearest conversions.
 #pragma unroll
-        for (int j = 0; j < 9; ++j) {
-            const _Float16 h = static_cast<_Float16>(v[j]);
-            dh[j * kStrideH] = h;
-            dl[j * kStrideH] = static_cast<_Float16>(v[j] - static_cast<float>(h));
+        for (int j = 0; j < 4; ++j) {
+            const half2v h = __builtin_convertvector((float2v){v[j], v[7 - j]}, half2v);
+            const float2v r = (float2v){v[j], v[7 - j]} - __builtin_convertvector(h, float2v);
+            const half2v l = __builtin_convertvector(r, half2v);
+            dh[j * kStrideH] = h[0];
+            dh[(7 - j) * kStrideH] = h[1];
+            dl[j * kStrideH] = l[0];
+            dl[(7 - j) * kStrideH] = l[1];
         }
+        const bool edge = dd <= -4.0f;
+        dh[8 * kStrideH] = edge ? kEsPolyEdgeHi : static_cast<_Float16>(0.0f);
+        dl[8 * kStrideH] = edge ? kEsPolyEdgeLo : static_cast<_Float16>(0.0f);
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
     {
@@ -678,7 +692,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     const int i = it.start + min(c1 + L.rl, it.count - 1);
                     rec.get(i, q_next, vy_next);
                 }
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, vy, d);
                 c0 = c1;
             }
             // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
@@ -765,7 +779,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     const int i = it.start + min(c0 + kChunk + L.rl, it.count - 1);
                     rec.get(i, q_next, vy_next);
                 }
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, wg.ncls, q, vy, d);
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, vy, d);
             }
             // What comes next: this item's next term block, or the next item.  Its first
             // records are requested now, ahead of the stores below.
