@@ -76,14 +76,11 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     const double v13 = V[2], v23 = V[5], v33 = V[8];
     const double ra = 1.0 - (t1 + v13) * (t1 + v13) - (t2 + v23) * (t2 + v23);
     if (ra < 0.0) return;
-    const double t3 = -v33 + (v33 >= 0.0 ? 1.0 : -1.0) * sqrt(ra);
-    const double alpha = 2.0 * kPi * t3 * static_cast<double>(ip.slab_half);
-    double jn[KMAX];
-    bessel_sequence_fixed<KMAX>(alpha, jn);
     // Plane element read by this pixel: linear index G*G/2 + G/2 + idyr*G + idxr of a G x G
     // plane (SVDIntegratedImager.cu:1397), located in the transformed window.  With
     // G*G/2 + G/2 = q0 G + r0 (0 <= r0 < G) and |idxr| < G, the row and column are
-    // q0 + idyr and r0 + idxr with at most one carry.
+    // q0 + idyr and r0 + idxr with at most one carry.  The first slab's K elements are
+    // requested before the FP64 work below, so that their latency overlaps it.
     const int G = ip.G;
     int lr = ip.lin_q0 + static_cast<int>(idyr), lc = ip.lin_r0 + static_cast<int>(idxr);
     if (lc < 0) {
@@ -96,16 +93,25 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     const long gr = lr - ip.row_lo, gc = lc - ip.col_lo;
     const size_t plane_stride = static_cast<size_t>(ip.rows) * ip.cols;
     const float2* px = planes + static_cast<size_t>(gr) * ip.cols + gc;
+    float2 m[KMAX];
+    auto load_slab = [&](int ls) {
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k)
+            m[k] = k < ip.K ? px[static_cast<size_t>(ls * ip.K + k) * plane_stride] : make_float2(0.0f, 0.0f);
+    };
+    load_slab(0);
+    const double t3 = -v33 + (v33 >= 0.0 ? 1.0 : -1.0) * sqrt(ra);
+    const double alpha = 2.0 * kPi * t3 * static_cast<double>(ip.slab_half);
+    double jn[KMAX];
+    bessel_sequence_fixed<KMAX>(alpha, jn);
     double sum = 0.0;
     for (int ls = 0; ls < ip.nslabs; ++ls) {
         const float centre = ip.archive ? ip.r3_centre
                                         : ip.r3_min + (static_cast<float>(ip.slab_start + ls) + 0.5f) * ip.slab_width;
-        double sb, cb;
-        sincos(2.0 * kPi * t3 * static_cast<double>(centre), &sb, &cb);
+        double sb, cb;                          // e^{i 2 pi t3 centre}
+        sincospi(2.0 * t3 * static_cast<double>(centre), &sb, &cb);
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
-            if (k >= ip.K) break;
-            const float2 m = px[static_cast<size_t>(ls * ip.K + k) * plane_stride];
             const double a = (k == 0 ? 1.0 : 2.0) * jn[k];
             double cr, ci;
             switch (k & 3) {                    // a i^k e^{i beta}
@@ -114,8 +120,9 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
                 case 2: cr = -a * cb; ci = -a * sb; break;
                 default: cr = a * sb; ci = -a * cb; break;
             }
-            sum += cr * m.x - ci * m.y;
+            sum += cr * m[k].x - ci * m[k].y;   // zero for k >= K
         }
+        if (ls + 1 < ip.nslabs) load_slab(ls + 1);
     }
     float ps = static_cast<float>(sum);
     if ((idxr + idyr) & 1) ps = -ps;

@@ -106,8 +106,19 @@ template <int COUNT>
 SVDI_HD void bessel_sequence_fixed(double x, double* j) {
     const double ax = fabs(x);
     if (ax < kBesselSeriesMax) {
+        // J_n = h^n / n! (1 + q r1 (1 + q r2 (1 + q r3 (1 + q r4)))), h = x / 2, q = -h^2,
+        // r_k = 1 / (k (n + k)): the power series through q^4 (|q| < 2.5e-5, so the q^5 term is
+        // below 1e-25 of the first), with the reciprocals folded at compile time.
+        const double h = 0.5 * ax, q = -h * h;
+        double p = 1.0;                                    // h^n / n!
 #pragma unroll
-        for (int n = 0; n < COUNT; ++n) j[n] = bessel_series(n, ax);
+        for (int n = 0; n < COUNT; ++n) {
+            const double r1 = 1.0 / (1.0 * (n + 1)), r2 = 1.0 / (2.0 * (n + 2));
+            const double r3 = 1.0 / (3.0 * (n + 3)), r4 = 1.0 / (4.0 * (n + 4));
+            const double s = fma(q * r1, fma(q * r2, fma(q * r3, fma(q, r4, 1.0), 1.0), 1.0), 1.0);
+            j[n] = p * s;
+            p *= h * (1.0 / (n + 1));
+        }
     } else if (COUNT <= 16 && ax < 2.0) {
         constexpr int m = 2 * ((COUNT + 13) / 2);
         const double two_over_x = 2.0 / ax;
