@@ -25,9 +25,9 @@
 // ordered by class, each record is written once per window column it meets, and a chunk of 32
 // records drawn from neighbouring classes skips the 4-row quarters of the window that none of
 // them touches (with c = 4, 12 of 16 rows per class).  Counting uses block-local
-// LDS histograms kept per block; after a scan of the bin totals each block reserves its ranges
-// with one atomic per bin and one more pass over its samples writes the records (position
-// relative to the class / window-column origin, s, v).
+// LDS histograms kept per block; a prefix over the blocks and a scan of the bin totals give each
+// block its first slot in every bin, and one more pass over its samples writes the records
+// (position relative to the class's home window row / the window column, s, v).
 #include "sample.hpp"
 #include "scan.hpp"
 #include "es_poly.hpp"
@@ -168,9 +168,8 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
-// Bin counts: per block in an LDS histogram (dynamic LDS, nbins words), added to counts[bin]
-// and also kept per block in blockhist[block * nbins + bin] for the scatter (without LDS
-// histograms the counts go straight to counts[bin]).
+// Bin counts: per block in an LDS histogram (dynamic LDS, nbins words) written whole to
+// blockhist[block * nbins + bin] (without LDS histograms the counts go straight to counts[bin]).
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
@@ -187,17 +186,78 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     __syncthreads();
     if (wg.use_lds) {
         uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
-        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
-            const uint32_t c = hist[b];
-            row[b] = c;
-            if (c != 0) atomicAdd(&counts[b], c);
-        }
+        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) row[b] = hist[b];
     }
 }
 
-// Records of this block's samples, in one pass: with LDS histograms the block reserves each
-// bin's range with one atomic on the bin's cursor (its count read back from blockhist) and
-// hands out slots with LDS cursors; otherwise every record takes a global cursor slot.
+// The per-block histograms turned into per-block record offsets without global atomics per
+// (block, bin): the blocks are split into kBinSegs segments; k_bin_seg_sums forms each segment's
+// count of every bin (and the bin totals), and after the scan of the totals k_bin_block_offsets
+// replaces blockhist[block][bin] by boff[bin] + the records of the bin in earlier blocks.  Each
+// block's records of a bin therefore follow those of the blocks before it.
+constexpr int kBinSegs = 8;
+
+__device__ __forceinline__ void bin_segment(int nblocks, int seg, int& b0, int& b1) {
+    const int per = (nblocks + kBinSegs - 1) / kBinSegs;
+    b0 = min(seg * per, nblocks);
+    b1 = min(b0 + per, nblocks);
+}
+
+__global__ void __launch_bounds__(256)
+k_bin_seg_sums(const uint32_t* __restrict__ blockhist, int nblocks, int nbins, uint32_t* __restrict__ segsum,
+               uint32_t* __restrict__ counts) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= kBinSegs * nbins) return;
+    const int seg = t / nbins, bin = t - seg * nbins;
+    int b0, b1;
+    bin_segment(nblocks, seg, b0, b1);
+    const uint32_t* p = blockhist + bin;
+    uint32_t sum = 0;
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = p[static_cast<size_t>(b + j) * nbins];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) sum += v[j];
+    }
+    for (; b < b1; ++b) sum += p[static_cast<size_t>(b) * nbins];
+    segsum[t] = sum;
+    if (sum != 0) atomicAdd(&counts[bin], sum);
+}
+
+__global__ void __launch_bounds__(256)
+k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, const uint32_t* __restrict__ segsum,
+                    const uint32_t* __restrict__ boff) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= kBinSegs * nbins) return;
+    const int seg = t / nbins, bin = t - seg * nbins;
+    int b0, b1;
+    bin_segment(nblocks, seg, b0, b1);
+    uint32_t run = boff[bin];
+    for (int s = 0; s < seg; ++s) run += segsum[s * nbins + bin];
+    uint32_t* p = blockhist + bin;
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = p[static_cast<size_t>(b + j) * nbins];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            p[static_cast<size_t>(b + j) * nbins] = run;
+            run += v[j];
+        }
+    }
+    for (; b < b1; ++b) {
+        const uint32_t v = p[static_cast<size_t>(b) * nbins];
+        p[static_cast<size_t>(b) * nbins] = run;
+        run += v;
+    }
+}
+
+// Records of this block's samples, in one pass: with LDS histograms the block's first slot of
+// each bin comes from blockhist (the offsets of k_bin_block_offsets) and LDS cursors hand out
+// the slots; otherwise every record takes a global cursor slot.
 // `limit` (the record buffer's capacity) only bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
@@ -230,10 +290,7 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         return;
     }
     const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
-    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) {
-        const uint32_t c = row[b];
-        hist[b] = c != 0 ? atomicAdd(&cursor[b], c) : 0u;
-    }
+    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = row[b];
     __syncthreads();
     for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
         emit(atomicAdd(&hist[bin], 1u), f, h, wb);
@@ -860,8 +917,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 12) * sizeof(uint32_t)));
     uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
-    uint32_t* cursor = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCursor, (nbins + 8 + nwt) * sizeof(uint32_t)));
+    uint32_t* cursor = static_cast<uint32_t*>(
+        ctx->ws.get(Workspace::kBinCursor, (nbins + 8 + nwt + (wg.use_lds ? kBinSegs * nbins : 0)) * sizeof(uint32_t)));
     uint32_t* multi = cursor + nbins + 8;                  // windows with several pieces
+    uint32_t* segsum = multi + nwt;                        // [kBinSegs][nbins]
     const size_t ncounts = (2 * kItemClasses + 1) * nwt;
     uint32_t* sums = static_cast<uint32_t*>(
         ctx->ws.get(Workspace::kScanSums, scan_blocks(std::max(nbins, ncounts)) * sizeof(uint32_t)));
@@ -885,7 +944,16 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
         k_win_count<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     }
     SVDI_LAUNCH_CHECK();
-    launch_exclusive_scan(ctx, counts, boff, cursor, sums, nbins);
+    const unsigned seg_grid = static_cast<unsigned>((kBinSegs * nbins + 255) / 256);
+    if (wg.use_lds) {
+        k_bin_seg_sums<<<seg_grid, 256, 0, ctx->stream>>>(blockhist, static_cast<int>(blocks), wg.nbins, segsum, counts);
+        SVDI_LAUNCH_CHECK();
+    }
+    launch_exclusive_scan(ctx, counts, boff, wg.use_lds ? nullptr : cursor, sums, nbins);
+    if (wg.use_lds) {
+        k_bin_block_offsets<<<seg_grid, 256, 0, ctx->stream>>>(blockhist, static_cast<int>(blocks), wg.nbins, segsum, boff);
+        SVDI_LAUNCH_CHECK();
+    }
     k_win_item_counts<<<static_cast<unsigned>((nwt + 255) / 256), 256, 0, ctx->stream>>>(boff, wg, icount, multi, totals);
     SVDI_LAUNCH_CHECK();
     launch_exclusive_scan(ctx, icount, ioff, nullptr, sums, ncounts);
