@@ -31,6 +31,7 @@
 #include "sample.hpp"
 #include "scan.hpp"
 #include "es_poly.hpp"
+#include "records.hpp"
 
 namespace svdi {
 namespace {
@@ -86,24 +87,14 @@ struct WItem {
     int bnd[kMaxClasses];
 };
 
-// Records, 20 bytes each, packed: (p1 - row-class origin, p2 - window column origin, s, Re v,
-// Im v) in cells.  One record is one dword-aligned 16-byte access plus one 4-byte access, so the
-// scatter writes each record as one contiguous run.
-typedef float float4u __attribute__((ext_vector_type(4), aligned(4)));
+// Records: 16 bytes each (records.hpp), one aligned 16-byte access to write or read.
 struct WRecords {
-    float* r;           // 5 floats per record
+    uint4* r;
 
-    __device__ __forceinline__ void put(uint32_t slot, float4 q, float vy) const {
-        float* p = r + 5 * static_cast<size_t>(slot);
-        *reinterpret_cast<float4u*>(p) = float4u{q.x, q.y, q.z, q.w};
-        p[4] = vy;
+    __device__ __forceinline__ void put(uint32_t slot, const RecordWords& w) const {
+        r[slot] = make_uint4(w.w[0], w.w[1], w.w[2], w.w[3]);
     }
-    __device__ __forceinline__ void get(int i, float4& q, float& vy) const {
-        const float* p = r + 5 * static_cast<size_t>(i);
-        const float4u v = *reinterpret_cast<const float4u*>(p);
-        q = make_float4(v.x, v.y, v.z, v.w);
-        vy = p[4];
-    }
+    __device__ __forceinline__ uint4 get(int i) const { return r[i]; }
 };
 
 // Every (sample, bin) pair of this block's samples: one row class h = a0 / 8, and the window
@@ -269,18 +260,13 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
     auto emit = [&](uint32_t slot, const Footprint& f, int h, int wb) {
         m = fmaxf(m, fmaxf(fabsf(f.vw.x), fabsf(f.vw.y)));
         if (slot >= limit) return;
-        float p1r, p2r;
         // Row position relative to the class's home window row (c h + 8) / 16: the row window
-        // that reads the class at d < 8 / c (the row above reads it at d >= 8 / c).
+        // that reads the class at d < 8 / c (the row above reads it at d >= 8 / c).  Snapshot
+        // input: q = p - (first footprint cell); preprocessed input: q = p (centred).
         const int home = ((h << wg.cls_log2) + 8) >> 4;
-        if constexpr (kSnap) {
-            p1r = f.q1 + static_cast<float>(f.a0 - home * kWin);
-            p2r = f.q2 + static_cast<float>(f.b0 - wb * kWin);
-        } else {
-            p1r = f.q1 - static_cast<float>(home * kWin - off);
-            p2r = f.q2 - static_cast<float>(wb * kWin - off);
-        }
-        rec.put(slot, make_float4(p1r, p2r, f.s, f.vw.x), f.vw.y);
+        const int base1 = kSnap ? f.a0 - home * kWin : off - home * kWin;
+        const int base2 = kSnap ? f.b0 - wb * kWin : off - wb * kWin;
+        rec.put(slot, rec_encode(base1, f.q1, base2, f.q2, f.s, f.vw.x, f.vw.y));
     };
     if (!wg.use_lds) {
         for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
@@ -512,13 +498,14 @@ struct WinLane {
     _Float16* s_kh;     // this lane's axis (ku or kv): hi and lo planes
     _Float16* s_kl;
     int lane, rl, axis, g8, n16;
-    float vscale, inv;
+    int vexp;           // log2 of the c scale
+    float inv;
 };
 
-// One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q, vy) contracted
+// One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q) contracted
 // into the accumulators d for the terms kg .. kg + 7.
 __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
-                                             int cls_log2, const float4& q, float vy, floatx4 (&d)[kWin]) {
+                                             int cls_log2, const uint4& q, floatx4 (&d)[kWin]) {
     const bool valid = c0 + L.rl < it.count;
     // Classes of the chunk's first and last records (wave-uniform), and the 4-row quarters of
     // the window that they touch: class j covers rows [c d, c d + c + 7], d = j - 8 / c.
@@ -550,13 +537,14 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     const int foreign = cls_log2 == 2 ? it.bnd[4] : it.bnd[2];
     if (valid) {
         const int i = i0 + L.rl;
-        const float pr = L.axis ? q.y : q.x + (i >= foreign ? static_cast<float>(kWin) : 0.0f);
-        const float fl = ceilf(pr - 4.0f);
-        const int first = min(max(static_cast<int>(fl), -kPad), kWin - 1);
-        // Cell j is at x = (f + j) / 4 - 1, f = fl - pr + 4 in [0, 1]: for j < 8 the quarter-support
-        // polynomial pair (j, 7 - j) in t = f - 1/2, for j = 8 ES(1) when f = 0 and zero otherwise.
-        const float dd = fl - pr;
-        const float t = dd + 3.5f, t2 = t * t;
+        const uint32_t pw = L.axis ? q.w : q.z;
+        const int fl = rec_cell(pw) + (L.axis == 0 && i >= foreign ? kWin : 0);
+        const uint32_t fb = rec_frac_bits(pw);
+        const int first = min(max(fl, -kPad), kWin - 1);
+        // Cell j is at x = (F + j) / 4 - 1 (F the record's fraction in [0, 1)): for j < 8 the
+        // quarter-support polynomial pair (j, 7 - j) in t = F - 1/2, for j = 8 ES(1) when F = 0
+        // and zero otherwise.
+        const float t = fmaf(static_cast<float>(fb), kRecFracInv, -0.5f), t2 = t * t;
         float v[8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -582,14 +570,14 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
             dl[j * kStrideH] = l[0];
             dl[(7 - j) * kStrideH] = l[1];
         }
-        const bool edge = dd <= -4.0f;
+        const bool edge = fb == 0u;
         dh[8 * kStrideH] = edge ? kEsPolyEdgeHi : static_cast<_Float16>(0.0f);
         dl[8 * kStrideH] = edge ? kEsPolyEdgeLo : static_cast<_Float16>(0.0f);
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
     {
-        const float vs = valid ? (L.axis ? vy : q.w) * L.vscale : 0.0f;
-        const float s = q.z, s2 = 2.0f * s;
+        const float vs = valid ? rec_v(L.axis ? q.y : q.x, q.w, L.vexp) : 0.0f;
+        const float s = rec_s(q.x, q.y, q.z), s2 = 2.0f * s;
         float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
         for (int k = 0; k < kg; ++k) {
             const float t2 = s2 * t1 - t0;
@@ -694,7 +682,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
         int e = 0;
         if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
         e = min(max(e, -100), 100);
-        L.vscale = ldexpf(1.0f, 14 - e);
+        L.vexp = 14 - e;
         L.inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
     }
     const int K = gp.K, G = gp.G;
@@ -735,21 +723,13 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
             int c0 = claim_chunk();
-            float4 q_next = make_float4(0.f, 0.f, 0.f, 0.f);
-            float vy_next = 0.0f;
-            if (c0 < it.count) {
-                const int i = it.start + min(c0 + L.rl, it.count - 1);
-                rec.get(i, q_next, vy_next);
-            }
+            uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
+            if (c0 < it.count) q_next = rec.get(it.start + min(c0 + L.rl, it.count - 1));
             while (c0 < it.count) {
-                const float4 q = q_next;
-                const float vy = vy_next;
+                const uint4 q = q_next;
                 const int c1 = claim_chunk();
-                if (c1 < it.count) {
-                    const int i = it.start + min(c1 + L.rl, it.count - 1);
-                    rec.get(i, q_next, vy_next);
-                }
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, vy, d);
+                if (c1 < it.count) q_next = rec.get(it.start + min(c1 + L.rl, it.count - 1));
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, d);
                 c0 = c1;
             }
             // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
@@ -814,12 +794,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     WItem it = items[item];
     // Records of the next chunk, loaded while the current one is contracted.  Every lane loads
     // (lanes past the end re-read the last record and ignore it), so the loads land in place.
-    float4 q_next;
-    float vy_next;
-    {
-        const int i = it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0;
-        rec.get(i, q_next, vy_next);
-    }
+    uint4 q_next = rec.get(it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0);
     for (;;) {
         ticket = claim_item();
         int next = nwave_items;
@@ -830,13 +805,9 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
             for (int c0 = 0; c0 < it.count; c0 += kChunk) {
-                const float4 q = q_next;
-                const float vy = vy_next;
-                if (c0 + kChunk < it.count) {
-                    const int i = it.start + min(c0 + kChunk + L.rl, it.count - 1);
-                    rec.get(i, q_next, vy_next);
-                }
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, vy, d);
+                const uint4 q = q_next;
+                if (c0 + kChunk < it.count) q_next = rec.get(it.start + min(c0 + kChunk + L.rl, it.count - 1));
+                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, d);
             }
             // What comes next: this item's next term block, or the next item.  Its first
             // records are requested now, ahead of the stores below.
@@ -853,8 +824,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                         pcount = 0;
                     }
                 }
-                const int i = pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0;
-                rec.get(i, q_next, vy_next);
+                q_next = rec.get(pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0);
             }
             // Rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16) and
             // kg + 2 (lane / 16) + 1; column n16.
@@ -933,7 +903,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     uint32_t* totals = counts + nbins + 2;
     WItem* items = static_cast<WItem*>(ctx->ws.get(Workspace::kItems, max_items * sizeof(WItem)));
     WRecords rec;
-    rec.r = static_cast<float*>(ctx->ws.get(Workspace::kRecPos, max_records * 5 * sizeof(float)));
+    rec.r = static_cast<uint4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(uint4)));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
