@@ -425,8 +425,20 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
         const int cell = threadIdx.x;
         const int a = wa * kWin + cell / kWin, b = wb * kWin + cell % kWin;
         const float2* src = partial + first * piece_size + static_cast<size_t>(k) * kWin * kWin + cell;
+        // Sixteen pieces' values requested at a time (the densest windows have many pieces),
+        // summed into four accumulators in a fixed order.
         float2 acc[4] = {};
         uint32_t p = 0;
+        for (; p + 16 <= pieces; p += 16) {
+            float2 v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = src[(p + j) * piece_size];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                acc[j & 3].x += v[j].x;
+                acc[j & 3].y += v[j].y;
+            }
+        }
         for (; p + 4 <= pieces; p += 4) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
