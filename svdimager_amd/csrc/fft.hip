@@ -335,6 +335,7 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
     for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
+    wave_lds_fence();                              // the tile may be where the loads came from
     // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
     // 3-point DFTs over v.  W_24^m = tw[64 m].
 #pragma unroll
@@ -375,15 +376,21 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
     }
     wave_lds_fence();
     // Pass B: p = lane + 64 j -> (k1, c) = (p % 24, p / 24); 8-point DFT over b gives X[p + 192 d].
+    // Every value is read before the first store (the stores may go to the tile's memory).
+    float2 z[3][8];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const int p = lane + 64 * j, k1 = p % 24, c = p / 24;
-        float2 z[8];
 #pragma unroll
-        for (int b = 0; b < 8; ++b) z[b] = tile[k1 * kTileStride + 8 * b + c];
-        dft8(z);
+        for (int b = 0; b < 8; ++b) z[j][b] = tile[k1 * kTileStride + 8 * b + c];
+    }
+    wave_lds_fence();
 #pragma unroll
-        for (int d = 0; d < 8; ++d) store(p + 192 * d, z[d]);
+    for (int j = 0; j < 3; ++j) {
+        const int p = lane + 64 * j;
+        dft8(z[j]);
+#pragma unroll
+        for (int d = 0; d < 8; ++d) store(p + 192 * d, z[j][d]);
     }
 }
 
@@ -416,6 +423,41 @@ __device__ __forceinline__ int xcd_local_tile(int linear, int total) {
     const int q = total / kXcds, r = total % kXcds;
     const int x = linear % kXcds, local = linear / kXcds;
     return x * q + min(x, r) + local;
+}
+
+// Column pass for G = 1536: one workgroup per (plane, strip of kColStrip columns).  The strip is
+// read into LDS with the columns side by side (row segments of kColStrip columns), each
+// wavefront transforms one column in place (the column's LDS region is also its exchange
+// tile), and the rows row_lo .. row_lo + rows - 1 are written back as row segments.
+constexpr int kColStrip = 4;
+constexpr int kColLen = 24 * kTileStride;          // float2 per column region (>= 1536)
+
+__global__ void __launch_bounds__(64 * kColStrip)
+k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int cols, int row_lo, int rows, int strips) {
+    constexpr int G = 1536;
+    __shared__ float2 strip[kColStrip * kColLen];
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    const int plane = tile / strips;
+    const int c0 = (tile - plane * strips) * kColStrip;
+    const int nc = min(kColStrip, cols - c0);
+    const float2* src = in + static_cast<size_t>(plane) * G * cols + c0;
+    for (int t = threadIdx.x; t < G * kColStrip; t += 64 * kColStrip) {
+        const int r = t / kColStrip, c = t % kColStrip;
+        if (c < nc) strip[c * kColLen + r] = src[static_cast<size_t>(r) * cols + c];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (wave < nc) {
+        float2* col = strip + wave * kColLen;
+        fft1536_wave(col, tw, lane, [&](int n) { return col[n]; }, [&](int k, float2 v) { col[k] = v; });
+    }
+    __syncthreads();
+    float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
+    for (int t = threadIdx.x; t < rows * kColStrip; t += 64 * kColStrip) {
+        const int r = t / kColStrip, c = t % kColStrip;
+        if (c < nc) dst[static_cast<size_t>(r) * cols + c] = strip[c * kColLen + row_lo + r];
+    }
 }
 
 // Column pass: row_out [P][G][cols] -> out [P][rows][cols] (output rows row_lo .. row_lo+rows-1).
@@ -579,10 +621,16 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                                    static_cast<int>(cols));
     }
     SVDI_LAUNCH_CHECK();
-    const int strips = static_cast<int>((cols + width - 1) / width);
-    const unsigned gc = static_cast<unsigned>(strips * P);
-    cols_fn<<<gc, threads, lds, ctx->stream>>>(row_out, out, st, tw, width, static_cast<int>(cols),
-                                               static_cast<int>(row_lo), static_cast<int>(rows), strips);
+    if (G == 1536) {
+        const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
+        k_fft_cols_1536<<<static_cast<unsigned>(strips * P), 64 * kColStrip, 0, ctx->stream>>>(
+            row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
+    } else {
+        const int strips = static_cast<int>((cols + width - 1) / width);
+        const unsigned gc = static_cast<unsigned>(strips * P);
+        cols_fn<<<gc, threads, lds, ctx->stream>>>(row_out, out, st, tw, width, static_cast<int>(cols),
+                                                   static_cast<int>(row_lo), static_cast<int>(rows), strips);
+    }
     SVDI_LAUNCH_CHECK();
 }
 
