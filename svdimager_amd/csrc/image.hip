@@ -8,6 +8,8 @@
 //   sum *= (-1)^(idxr+idyr) / (c(|idxr|) c(|idyr|) 2.4937^2),
 // then |sum| / S is splatted bilinearly at the SIN re-projection of the pixel.  t3, the
 // w-phase and the Bessel sequence are FP64 (the reference evaluates them in FP32).
+#include <climits>
+
 #include "svdi_internal.hpp"
 
 namespace svdi {
@@ -60,22 +62,19 @@ __global__ void k_conv_corr(int N, int G, GL28Table gl, float* __restrict__ conv
     conv[t] = static_cast<float>(acc * kSupport / 2.4937047051153827);
 }
 
+// The corrected image value of pixel (x, y) (FP64 t3, w-phase and Bessel terms), or false for a
+// pixel outside the unit sphere (ra < 0), which contributes nothing.
 template <int KMAX>
-__global__ void __launch_bounds__(256)
-k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restrict__ vin,
-        const float* __restrict__ conv, float* __restrict__ image) {
-    const int x = blockIdx.x * 16 + threadIdx.x;
-    const int y = blockIdx.y * 16 + threadIdx.y;
-    if (x >= ip.N || y >= ip.N) return;
+__device__ __forceinline__ bool image_pixel(const float2* __restrict__ planes, const ImageParams& ip,
+                                            const float* V, const float* __restrict__ conv, int x, int y,
+                                            float& ps_out) {
     const long half = ip.N / 2;
     const long idxr = x - half, idyr = y - half;
-    float V[9];
-    for (int i = 0; i < 9; ++i) V[i] = vin[i];
     const double t1 = static_cast<double>(ip.cell) * static_cast<double>(idyr);
     const double t2 = static_cast<double>(ip.cell) * static_cast<double>(idxr);
     const double v13 = V[2], v23 = V[5], v33 = V[8];
     const double ra = 1.0 - (t1 + v13) * (t1 + v13) - (t2 + v23) * (t2 + v23);
-    if (ra < 0.0) return;
+    if (ra < 0.0) return false;
     // Plane element read by this pixel: linear index G*G/2 + G/2 + idyr*G + idxr of a G x G
     // plane (SVDIntegratedImager.cu:1397), located in the transformed window.  With
     // G*G/2 + G/2 = q0 G + r0 (0 <= r0 < G) and |idxr| < G, the row and column are
@@ -127,25 +126,102 @@ k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restri
     float ps = static_cast<float>(sum);
     if ((idxr + idyr) & 1) ps = -ps;
     ps *= 1 / (conv[labs(idxr)] * conv[labs(idyr)] * kConvCorrNorm * kConvCorrNorm);
+    ps_out = ps;
+    return true;
+}
+
+// Bilinear splat tile of a 16 x 16 block: the block's targets are accumulated in LDS and each
+// touched cell is added to the image with one device atomic, instead of four per pixel.
+constexpr int kSplatTile = 32;
+
+template <int KMAX>
+__global__ void __launch_bounds__(256)
+k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restrict__ vin,
+        const float* __restrict__ conv, float* __restrict__ image) {
+    const int x = blockIdx.x * 16 + threadIdx.x;
+    const int y = blockIdx.y * 16 + threadIdx.y;
+    float V[9];
+    for (int i = 0; i < 9; ++i) V[i] = vin[i];
+    float ps = 0.0f;
+    const bool ok = x < ip.N && y < ip.N && image_pixel<KMAX>(planes, ip, V, conv, x, y, ps);
     if (ip.signed_out) {
-        image[static_cast<size_t>(y) * ip.N + x] += ps * ip.inv_num_samples;
+        if (ok) image[static_cast<size_t>(y) * ip.N + x] += ps * ip.inv_num_samples;
         return;
     }
+    // Every thread of the block reaches the barriers below.
+    const long half = ip.N / 2;
     const float value = fabsf(ps) * ip.inv_num_samples;
-    float oi0, oi1;
-    if (ip.archive) sin_reproject_archive(ip.N, ip.cell, V, x, y, &oi0, &oi1);
-    else sin_reproject(ip.N, ip.cell, V, x, y, &oi0, &oi1);
-    const float LL = oi0 - half;
-    const float MM = oi1 - half;
-    if (fabsf(LL) < half - 1 && fabsf(MM) < half - 1) {
-        const float LLf = floorf(LL), MMf = floorf(MM);
-        const float LLc = ceilf(LL), MMc = ceilf(MM);
-        float* d = image + half * ip.N + half;
-        const long n = ip.N;
-        unsafeAtomicAdd(&d[static_cast<long>(MMf) * n + static_cast<long>(LLf)], (1 - LL + LLf) * (1 - MM + MMf) * value);
-        unsafeAtomicAdd(&d[static_cast<long>(MMc) * n + static_cast<long>(LLf)], (1 - LL + LLf) * (0 + MM - MMf) * value);
-        unsafeAtomicAdd(&d[static_cast<long>(MMf) * n + static_cast<long>(LLc)], (0 + LL - LLf) * (1 - MM + MMf) * value);
-        unsafeAtomicAdd(&d[static_cast<long>(MMc) * n + static_cast<long>(LLc)], (0 + LL - LLf) * (0 + MM - MMf) * value);
+    float LL = 0.0f, MM = 0.0f;
+    if (ok) {
+        float oi0, oi1;
+        if (ip.archive) sin_reproject_archive(ip.N, ip.cell, V, x, y, &oi0, &oi1);
+        else sin_reproject(ip.N, ip.cell, V, x, y, &oi0, &oi1);
+        LL = oi0 - half;
+        MM = oi1 - half;
+    }
+    const bool splat = ok && fabsf(LL) < half - 1 && fabsf(MM) < half - 1;
+    const float LLf = floorf(LL), MMf = floorf(MM);
+    const float LLc = ceilf(LL), MMc = ceilf(MM);
+    const int lf = static_cast<int>(LLf), mf = static_cast<int>(MMf);
+    const int lc = static_cast<int>(LLc), mc = static_cast<int>(MMc);
+    const float w00 = (1 - LL + LLf) * (1 - MM + MMf) * value, w01 = (1 - LL + LLf) * (0 + MM - MMf) * value;
+    const float w10 = (0 + LL - LLf) * (1 - MM + MMf) * value, w11 = (0 + LL - LLf) * (0 + MM - MMf) * value;
+    float* d = image + half * ip.N + half;
+    const long n = ip.N;
+    __shared__ int s_box[4];                       // min l, max l, min m, max m of the targets
+    __shared__ float s_tile[kSplatTile * kSplatTile];
+    const int tid = threadIdx.y * 16 + threadIdx.x;
+    if (tid == 0) {
+        s_box[0] = INT_MAX;
+        s_box[1] = INT_MIN;
+        s_box[2] = INT_MAX;
+        s_box[3] = INT_MIN;
+    }
+    for (int c = tid; c < kSplatTile * kSplatTile; c += 256) s_tile[c] = 0.0f;
+    __syncthreads();
+    {
+        // Bounds per wavefront by shuffles, then one LDS atomic per wavefront and bound (the
+        // same-address atomics of every thread would serialise).
+        int l0 = splat ? lf : INT_MAX, l1 = splat ? lc : INT_MIN;
+        int m0 = splat ? mf : INT_MAX, m1 = splat ? mc : INT_MIN;
+        for (int off = 32; off > 0; off >>= 1) {
+            l0 = min(l0, __shfl_xor(l0, off, 64));
+            l1 = max(l1, __shfl_xor(l1, off, 64));
+            m0 = min(m0, __shfl_xor(m0, off, 64));
+            m1 = max(m1, __shfl_xor(m1, off, 64));
+        }
+        if ((tid & 63) == 0 && l1 >= l0) {
+            atomicMin(&s_box[0], l0);
+            atomicMax(&s_box[1], l1);
+            atomicMin(&s_box[2], m0);
+            atomicMax(&s_box[3], m1);
+        }
+    }
+    __syncthreads();
+    const int bx0 = s_box[0], bx1 = s_box[1], by0 = s_box[2], by1 = s_box[3];
+    if (bx1 < bx0) return;                          // no target in this block
+    const bool fits = bx1 - bx0 < kSplatTile && by1 - by0 < kSplatTile;
+    if (!fits) {
+        if (splat) {
+            unsafeAtomicAdd(&d[static_cast<long>(mf) * n + lf], w00);
+            unsafeAtomicAdd(&d[static_cast<long>(mc) * n + lf], w01);
+            unsafeAtomicAdd(&d[static_cast<long>(mf) * n + lc], w10);
+            unsafeAtomicAdd(&d[static_cast<long>(mc) * n + lc], w11);
+        }
+        return;
+    }
+    if (splat) {
+        atomicAdd(&s_tile[(mf - by0) * kSplatTile + (lf - bx0)], w00);
+        atomicAdd(&s_tile[(mc - by0) * kSplatTile + (lf - bx0)], w01);
+        atomicAdd(&s_tile[(mf - by0) * kSplatTile + (lc - bx0)], w10);
+        atomicAdd(&s_tile[(mc - by0) * kSplatTile + (lc - bx0)], w11);
+    }
+    __syncthreads();
+    const int bw = bx1 - bx0 + 1, cells = bw * (by1 - by0 + 1);
+    for (int c = tid; c < cells; c += 256) {
+        const int i = c / bw, j = c - i * bw;
+        const float v = s_tile[i * kSplatTile + j];
+        if (v != 0.0f) unsafeAtomicAdd(&d[static_cast<long>(by0 + i) * n + (bx0 + j)], v);
     }
 }
 
