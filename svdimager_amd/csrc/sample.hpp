@@ -21,21 +21,24 @@ struct Footprint {
 // 1261-1275).  Non-finite positions convert to arbitrary integers: they are rejected, and so is
 // any footprint wider than the 9 cells a finite position can cover.
 __device__ __forceinline__ bool footprint_at(const GridParams& gp, double p1, double p2, Footprint& f) {
-    const long gmin = -static_cast<long>(gp.G) / 2;
-    const long gmax = (static_cast<long>(gp.G) - 1) / 2;
-    const long off = static_cast<long>(gp.G) / 2;
-    const long lo1 = max(static_cast<long>(ceil(p1 - 4.0)), gmin);
-    const long hi1 = min(static_cast<long>(floor(p1 + 4.0)), gmax);
-    const long lo2 = max(static_cast<long>(ceil(p2 - 4.0)), gmin);
-    const long hi2 = min(static_cast<long>(floor(p2 + 4.0)), gmax);
+    const int gmin = -gp.G / 2;
+    const int gmax = (gp.G - 1) / 2;
+    const int off = gp.G / 2;
+    // Positions clamped to +-1e9 cells (a NaN to -1e9) so that the conversions to int are defined;
+    // such a position has no footprint cell inside the grid.
+    const double c1 = fmin(fmax(p1, -1.0e9), 1.0e9), c2 = fmin(fmax(p2, -1.0e9), 1.0e9);
+    const int lo1 = max(static_cast<int>(ceil(c1 - 4.0)), gmin);
+    const int hi1 = min(static_cast<int>(floor(c1 + 4.0)), gmax);
+    const int lo2 = max(static_cast<int>(ceil(c2 - 4.0)), gmin);
+    const int hi2 = min(static_cast<int>(floor(c2 + 4.0)), gmax);
     const float da = static_cast<float>(static_cast<double>(lo1) - p1);
     const float db = static_cast<float>(static_cast<double>(lo2) - p2);
     if (!(fabsf(da) <= 5.0f) || !(fabsf(db) <= 5.0f)) return false;
     if (lo1 > hi1 || lo2 > hi2 || hi1 - lo1 > 8 || hi2 - lo2 > 8) return false;
-    f.a0 = static_cast<int>(lo1 + off);
-    f.na = static_cast<int>(hi1 - lo1 + 1);
-    f.b0 = static_cast<int>(lo2 + off);
-    f.nb = static_cast<int>(hi2 - lo2 + 1);
+    f.a0 = lo1 + off;
+    f.na = hi1 - lo1 + 1;
+    f.b0 = lo2 + off;
+    f.nb = hi2 - lo2 + 1;
     f.da = da;
     f.db = db;
     return true;
