@@ -246,7 +246,12 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     if (o.verbose) print_report(hp, std::cout);
 
     launch_fill(ctx, image, N * N, 0.0f);
-    launch_conv_corr(ctx, N, G, hp.semantics == Semantics::Archive, conv);
+    if (ctx->conv_at != conv || ctx->conv_n != N || ctx->conv_g != G) {   // depends on N and G only
+        launch_conv_corr(ctx, N, G, hp.semantics == Semantics::Archive, conv);
+        ctx->conv_at = conv;
+        ctx->conv_n = N;
+        ctx->conv_g = G;
+    }
 
     GridParams gp = {};
     gp.G = static_cast<int>(G);
@@ -547,6 +552,7 @@ int svdi_trim(svdi_context* ctx) {
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
         ctx->ws.release();
         ctx->twiddle_n = 0;
+        ctx->conv_at = nullptr;
     });
 }
 

@@ -151,6 +151,9 @@ struct svdi_context {
     std::vector<std::unique_ptr<svdi_context>> lanes;
     std::mutex mu;
     size_t twiddle_n = 0;
+    // Grid-correction table held in the workspace: computed for (N, G) at this address.
+    size_t conv_n = 0, conv_g = 0;
+    const float* conv_at = nullptr;
     int cu_count = 0;
     // stage timing: pairs of events recorded around stages of the current call
     bool profiling = false;
