@@ -474,6 +474,16 @@ __device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, h
     }
 }
 
+// (f16(a - h.x), f16(b - h.y)) with each difference formed exactly in FP32: one mixed-precision
+// fma per value (v_fma_mixlo / mixhi: source 0 the f16 half of h, negated, times 1.0, plus the
+// FP32 value).
+__device__ __forceinline__ half2v residual_f16x2(half2v h, float a, float b) {
+    half2v r;
+    asm volatile("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
+    asm volatile("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(r) : "v"(h), "v"(b));
+    return r;
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -571,12 +581,12 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         }
         _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
         _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
-        // hi and lo of the pair (j, 7 - j) by two packed round-to-nearest conversions.
+        // hi of the pair (j, 7 - j) by one packed round-to-nearest conversion, each lo = f16(v - hi)
+        // with the difference formed in FP32 (one mixed-precision fma per value).
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const half2v h = __builtin_convertvector((float2v){v[j], v[7 - j]}, half2v);
-            const float2v r = (float2v){v[j], v[7 - j]} - __builtin_convertvector(h, float2v);
-            const half2v l = __builtin_convertvector(r, half2v);
+            const half2v l = residual_f16x2(h, v[j], v[7 - j]);
             dh[j * kStrideH] = h[0];
             dh[(7 - j) * kStrideH] = h[1];
             dl[j * kStrideH] = l[0];
