@@ -458,22 +458,6 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
     }
 }
 
-// x = hi + lo, both rounded to nearest by packed conversions (two values per instruction):
-// |x - hi - lo| <= 2^-12 |lo| <= 2^-23 |x| (x in f16's normal range).
-__device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
-    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) {
-        const float2v x = (float2v){v[i], v[i + 1]};
-        const half2v h = __builtin_convertvector(x, half2v);
-        const half2v l = __builtin_convertvector(x - __builtin_convertvector(h, float2v), half2v);
-        hi[i] = h[0];
-        hi[i + 1] = h[1];
-        lo[i] = l[0];
-        lo[i + 1] = l[1];
-    }
-}
-
 // (f16(a - h.x), f16(b - h.y)) with each difference formed exactly in FP32: one mixed-precision
 // fma per value (v_fma_mixlo / mixhi: source 0 the f16 half of h, negated, times 1.0, plus the
 // FP32 value).
@@ -482,6 +466,22 @@ __device__ __forceinline__ half2v residual_f16x2(half2v h, float a, float b) {
     asm volatile("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(a));
     asm volatile("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(r) : "v"(h), "v"(b));
     return r;
+}
+
+// x = hi + lo: hi rounded to nearest by a packed conversion (two values per instruction), lo =
+// f16(x - hi) by mixed-precision fmas: |x - hi - lo| <= 2^-12 |lo| <= 2^-23 |x| (x in f16's
+// normal range).
+__device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, half8& hi, half8& lo) {
+    const float v[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        const half2v h = __builtin_convertvector((float2v){v[i], v[i + 1]}, half2v);
+        const half2v l = residual_f16x2(h, v[i], v[i + 1]);
+        hi[i] = h[0];
+        hi[i + 1] = h[1];
+        lo[i] = l[0];
+        lo[i + 1] = l[1];
+    }
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
