@@ -162,6 +162,15 @@ int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snapshots,
                               float* const* images, size_t image_size, float cell_size,
                               const svdi_options* opts, svdi_plan_info* infos);
 
+/* svdi_image_snapshot_batch without the final wait (no counterpart in the reference; for callers
+ * that pipeline batches): the call returns once the last snapshot's work is enqueued, which is
+ * after every snapshot's plan is known.  The context stream (the caller's stream when one was
+ * set with svdi_set_stream) waits for all of the call's work, so the images are complete once
+ * that stream reaches this point; snapshots and images must stay allocated until then. */
+int svdi_image_snapshot_batch_async(svdi_context* ctx, const svdi_snapshot* snapshots, size_t count,
+                                    float* const* images, size_t image_size, float cell_size,
+                                    const svdi_options* opts, svdi_plan_info* infos);
+
 /* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
  * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

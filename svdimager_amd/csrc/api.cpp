@@ -489,6 +489,58 @@ std::unique_ptr<svdi_context> create_context(int device) {
     return c;
 }
 
+// The batched call: enqueue every snapshot on the lanes, then wait for the lanes or not.
+int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
+                size_t image_size, float cell_size, const svdi_options* opts, svdi_plan_info* infos,
+                bool wait) {
+    return guarded([&] {
+        if (!ctx || !snaps || !images) throw ArgError("ctx / snapshots / images is null");
+        for (size_t i = 0; i < count; ++i) check_device_pointer(images[i], "image");
+        if (count == 0) return;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        const size_t L = std::min(count, kBatchLanes);
+        while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
+        // The lanes start after the caller's earlier work on the context stream, and the context
+        // stream continues after the last lane's work.
+        hipEvent_t ready = nullptr;
+        SVDI_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+        std::unique_ptr<void, void (*)(void*)> ready_guard(ready, [](void* e) {
+            (void)hipEventDestroy(static_cast<hipEvent_t>(e));
+        });
+        SVDI_HIP(hipEventRecord(ready, ctx->stream));
+        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ready, 0));
+        std::vector<CallState> cs(count);
+        std::vector<SampleInput> in(count);
+        // Rounds of L snapshots, one per lane: every lane's planning is enqueued first, then each
+        // lane's gridding as soon as its plan is back, while the other lanes keep the GPU busy.
+        for (size_t r0 = 0; r0 < count; r0 += L) {
+            const size_t n = std::min(L, count - r0);
+            for (size_t j = 0; j < n; ++j) {
+                svdi_context* lane = ctx->lanes[j].get();
+                begin_call(lane);
+                in[r0 + j] = snapshot_input(lane, snaps[r0 + j]);
+                image_begin(lane, in[r0 + j], image_size, cell_size, o, cs[r0 + j]);
+            }
+            for (size_t j = 0; j < n; ++j) {
+                svdi_context* lane = ctx->lanes[j].get();
+                SVDI_HIP(hipEventSynchronize(lane->ev_plan));
+                image_enqueue(lane, in[r0 + j], o, cs[r0 + j], images[r0 + j]);
+            }
+        }
+        for (size_t j = 0; j < L; ++j) {
+            svdi_context* lane = ctx->lanes[j].get();
+            SVDI_HIP(hipEventRecord(lane->ev_stop, lane->stream));
+            SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_stop, 0));
+        }
+        if (wait)
+            for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamSynchronize(ctx->lanes[j]->stream));
+        if (infos)
+            for (size_t i = 0; i < count; ++i) fill_info(cs[i], 0.0f, &infos[i]);
+    });
+}
+
 }  // namespace svdi
 
 svdi_context::~svdi_context() {
@@ -628,54 +680,17 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snap, float* ima
     });
 }
 
+
 int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
                               float* const* images, size_t image_size, float cell_size,
                               const svdi_options* opts, svdi_plan_info* infos) {
-    return guarded([&] {
-        if (!ctx || !snaps || !images) throw ArgError("ctx / snapshots / images is null");
-        for (size_t i = 0; i < count; ++i) check_device_pointer(images[i], "image");
-        if (count == 0) return;
-        std::lock_guard<std::mutex> lk(ctx->mu);
-        DeviceGuard g(ctx->device);
-        const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
-        const size_t L = std::min(count, kBatchLanes);
-        while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
-        // The lanes start after the caller's earlier work on the context stream, and the context
-        // stream continues after the last lane's work.
-        hipEvent_t ready = nullptr;
-        SVDI_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-        std::unique_ptr<void, void (*)(void*)> ready_guard(ready, [](void* e) {
-            (void)hipEventDestroy(static_cast<hipEvent_t>(e));
-        });
-        SVDI_HIP(hipEventRecord(ready, ctx->stream));
-        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ready, 0));
-        std::vector<CallState> cs(count);
-        std::vector<SampleInput> in(count);
-        // Rounds of L snapshots, one per lane: every lane's planning is enqueued first, then each
-        // lane's gridding as soon as its plan is back, while the other lanes keep the GPU busy.
-        for (size_t r0 = 0; r0 < count; r0 += L) {
-            const size_t n = std::min(L, count - r0);
-            for (size_t j = 0; j < n; ++j) {
-                svdi_context* lane = ctx->lanes[j].get();
-                begin_call(lane);
-                in[r0 + j] = snapshot_input(lane, snaps[r0 + j]);
-                image_begin(lane, in[r0 + j], image_size, cell_size, o, cs[r0 + j]);
-            }
-            for (size_t j = 0; j < n; ++j) {
-                svdi_context* lane = ctx->lanes[j].get();
-                SVDI_HIP(hipEventSynchronize(lane->ev_plan));
-                image_enqueue(lane, in[r0 + j], o, cs[r0 + j], images[r0 + j]);
-            }
-        }
-        for (size_t j = 0; j < L; ++j) {
-            svdi_context* lane = ctx->lanes[j].get();
-            SVDI_HIP(hipEventRecord(lane->ev_stop, lane->stream));
-            SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_stop, 0));
-        }
-        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamSynchronize(ctx->lanes[j]->stream));
-        if (infos)
-            for (size_t i = 0; i < count; ++i) fill_info(cs[i], 0.0f, &infos[i]);
-    });
+    return svdi::image_batch(ctx, snaps, count, images, image_size, cell_size, opts, infos, true);
+}
+
+int svdi_image_snapshot_batch_async(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
+                                    float* const* images, size_t image_size, float cell_size,
+                                    const svdi_options* opts, svdi_plan_info* infos) {
+    return svdi::image_batch(ctx, snaps, count, images, image_size, cell_size, opts, infos, false);
 }
 
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

@@ -134,9 +134,12 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
 
 
 def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size: int,
-                         cell_size: float, images=None, **options) -> Tuple[List, List[Dict]]:
+                         cell_size: float, images=None, wait: bool = True,
+                         **options) -> Tuple[List, List[Dict]]:
     """Image several device-resident snapshots in one call (concurrent lanes); returns (images,
-    plan dicts).  `images`: optional list of (N, N) float32 device tensors."""
+    plan dicts).  `images`: optional list of (N, N) float32 device tensors.  wait=False returns
+    once the work is enqueued (svdi_image_snapshot_batch_async): the images are complete when the
+    current stream reaches this point, and the snapshots must stay alive until then."""
     torch = _torch()
     n = len(dsnaps)
     if n == 0:
@@ -152,8 +155,8 @@ def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size:
     structs = (Snapshot * n)(*[d.struct() for d in dsnaps])
     ptrs = (ctypes.c_void_p * n)(*[_ptr(im) for im in images])
     _bind_stream(ctx, dev)
-    check(lib().svdi_image_snapshot_batch(ctx.handle, structs, n, ptrs, N, float(cell_size),
-                                          ctypes.byref(opts), infos))
+    fn = lib().svdi_image_snapshot_batch if wait else lib().svdi_image_snapshot_batch_async
+    check(fn(ctx.handle, structs, n, ptrs, N, float(cell_size), ctypes.byref(opts), infos))
     return images, [infos[i].as_dict() for i in range(n)]
 
 

@@ -54,3 +54,25 @@ def test_batch_of_one_and_reuse(gpu_ctx, snaps):
     three, _ = S.image_snapshot_batch(gpu_ctx, snaps[:3], N, cell, gridder=S.GRIDDER_MFMA,
                                       output=S.OUTPUT_SIGNED)
     assert_parity(one[0].cpu().numpy(), three[0].cpu().numpy(), "first image, two batches")
+
+
+def test_pipelined_batches_match_single(gpu_ctx, snaps):
+    """Two batched calls enqueued back to back without waiting (svdi_image_snapshot_batch_async,
+    the second over the snapshots in reverse order into other images), then one synchronize:
+    every image equals the snapshot imaged alone."""
+    cfg = synth.SMALL
+    N, cell = cfg.image_size, cfg.cell
+    announce(f"two pipelined batches of {len(snaps)}")
+    first, _ = S.image_snapshot_batch(gpu_ctx, snaps, N, cell, gridder=S.GRIDDER_MFMA,
+                                      output=S.OUTPUT_SIGNED, num_terms=cfg.num_terms, wait=False)
+    rev = list(reversed(snaps))
+    second, plans = S.image_snapshot_batch(gpu_ctx, rev, N, cell, gridder=S.GRIDDER_MFMA,
+                                           output=S.OUTPUT_SIGNED, num_terms=cfg.num_terms, wait=False)
+    torch.cuda.synchronize()
+    assert len(plans) == len(snaps) and all(p["num_terms"] == cfg.num_terms for p in plans)
+    for i, d in enumerate(snaps):
+        ref, _ = S.image_snapshot(gpu_ctx, d, N, cell, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_SIGNED,
+                                  num_terms=cfg.num_terms)
+        r = ref.cpu().numpy()
+        assert_parity(first[i].cpu().numpy(), r, f"pipelined batch 1, snapshot {i}")
+        assert_parity(second[len(snaps) - 1 - i].cpu().numpy(), r, f"pipelined batch 2, snapshot {i}")
