@@ -39,3 +39,23 @@ def test_skalow_mfma_vs_tiled(gpu_ctx, ha):
     a, b = img_m.cpu().numpy(), img_t.cpu().numpy()
     assert np.isfinite(a).all() and np.abs(b).max() > 0
     assert_parity(a, b, f"MFMA vs tiled, HA {ha} h")
+
+
+def test_skalow_4k_mfma_vs_tiled(gpu_ctx):
+    """The 4096^2 form (512 stations, 128 channels, 16,744,448 samples, G = 6144): no row-class
+    histogram fits in LDS there, so the window binning counts and places records with global
+    atomics.  MFMA against the FP32 tile gridder, signed images."""
+    cfg = synth.SKALOW_4K
+    announce("SKA-Low 4096^2 (128 channels): MFMA vs tiled FP32")
+    snap = synth.make_snapshot(cfg, device="cuda")
+    d = S.DeviceSnapshot(snap, "cuda")
+    del snap
+    N, cell = cfg.image_size, cfg.cell
+    img_t, plan_t = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_TILED)
+    img_m, plan_m = S.image_snapshot(gpu_ctx, d, N, cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_MFMA)
+    torch.cuda.synchronize()
+    assert plan_m["grid_size"] == 6144 and plan_m["num_terms"] == plan_t["num_terms"]
+    print(f"4k: slabs {plan_m['num_slabs']} K {plan_m['num_terms']} records {plan_m['records']}")
+    a, b = img_m.cpu().numpy(), img_t.cpu().numpy()
+    assert np.isfinite(a).all() and np.abs(b).max() > 0
+    assert_parity(a, b, "MFMA vs tiled, 4096^2")
