@@ -55,8 +55,7 @@ constexpr int kStageRows = kWin + 2 * kPad;
 constexpr int kPlaneHalves = kStageRows * kStrideH;               // one ES plane (hi or lo)
 // Per wavefront: ku hi / lo and kv hi / lo planes (f16), then c[16] (f32).
 constexpr int kStageFloats = (4 * kPlaneHalves) / 2 + kWin * kStride;
-constexpr float kEsScaleLog2 = 7.0f;     // ES factors scaled by 2^7 per axis
-constexpr int kEsScaleLog2Sq = 14;
+constexpr int kEsScaleLog2Sq = 14;      // ES factors scaled by 2^7 per axis (in the polynomial table)
 // ES(1) 2^7, the ninth footprint cell of an integer position, as f16 hi + lo.
 constexpr _Float16 kEsPolyEdgeHi = static_cast<_Float16>(kEsPolyEdge);
 constexpr _Float16 kEsPolyEdgeLo = static_cast<_Float16>(kEsPolyEdge - static_cast<float>(kEsPolyEdgeHi));
@@ -97,13 +96,13 @@ struct WRecords {
     __device__ __forceinline__ uint4 get(int i) const { return r[i]; }
 };
 
-// Every (sample, bin) pair of this block's samples: one row class h = a0 / 8, and the window
-// columns b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets (at most two).
-// Snapshot input: 512 consecutive rows in two sets of 256, each row split over 4 threads that
-// take every fourth channel (a wavefront reads 64 consecutive rows of one channel: coalesced),
-// loading four samples before working on the first.  Preprocessed input: 16384 consecutive
-// samples, 16 per thread at a stride of the block size.  kVis = false (snapshot input): the visibilities are not
-// loaded and f.vw is zero (the bins do not depend on them).
+// Every (sample, bin) pair of this block's samples: one row class h = a0 / c (c the class
+// height), and the window columns b0 / 16 .. (b0 + nb - 1) / 16 that the clipped footprint meets
+// (at most two).  Snapshot input: 512 consecutive rows in two sets of 256, each row split over 4
+// threads that take every fourth channel (a wavefront reads 64 consecutive rows of one channel:
+// coalesced), loading four samples before working on the first.  Preprocessed input: 16384
+// consecutive samples, 16 per thread at a stride of the block size.  kVis = false (snapshot
+// input): the visibilities are not loaded and f.vw is zero (the bins do not depend on them).
 template <bool kSnap, bool kVis, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
                                                 const WinGeom& wg, Fn&& fn) {
