@@ -27,7 +27,9 @@
 // them touches (with c = 4, 12 of 16 rows per class).  Counting uses block-local
 // LDS histograms kept per block; a prefix over the blocks and a scan of the bin totals give each
 // block its first slot in every bin, and one more pass over its samples writes the records
-// (position relative to the class's home window row / the window column, s, v).
+// (position relative to the class's home window row / the window column, s, v).  When the bins
+// do not fit in one LDS histogram they are split into bands of consecutive bins, and each band
+// is a separate set of workgroups that reads the block's samples and handles only its own bins.
 #include "sample.hpp"
 #include "scan.hpp"
 #include "es_poly.hpp"
@@ -46,6 +48,7 @@ constexpr int kBinRows = kBinRowSets * kBinRowsPerSet;
 constexpr int kBinChanGroups = kBinThreads / kBinRowsPerSet;
 constexpr int kBinPerThread = 16;        // samples per thread, preprocessed input
 constexpr int kLdsBins = 36864;          // largest block-local histogram (144 KB of LDS)
+constexpr size_t kMaxBlockHistBytes = size_t(1) << 31;   // per-block histograms of all bands
 constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
 constexpr int kChunk = 32;               // samples per MFMA reduction step
 constexpr int kStride = 36;              // floats per c staging row (32 samples + pad)
@@ -73,7 +76,8 @@ struct WinGeom {
     int nwin;        // nw * nw
     int nwin_total;  // nslabs * nwin
     int nbins;       // nslabs * nw * nh, ordered (slab, window column, row class)
-    int use_lds;     // histogram in LDS (nbins <= kLdsBins)
+    int use_lds;     // block histograms in LDS (else one global atomic per record)
+    int band_bins;   // bins per LDS band (<= kLdsBins); band y covers [y band_bins, ...)
 };
 
 // A work item: records [start, start + count) of window `win`; bnd[j] (j = 1 .. ncls - 1) is the
@@ -158,26 +162,37 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
-// Bin counts: per block in an LDS histogram (dynamic LDS, nbins words) written whole to
-// blockhist[block * nbins + bin] (without LDS histograms the counts go straight to counts[bin]).
+// Bins [lo, lo + n) of LDS band blockIdx.y.
+__device__ __forceinline__ void lds_band(const WinGeom& wg, int& lo, int& n) {
+    lo = static_cast<int>(blockIdx.y) * wg.band_bins;
+    n = min(wg.band_bins, wg.nbins - lo);
+}
+
+// Bin counts: per (block, band) in an LDS histogram (dynamic LDS, band_bins words) written to
+// blockhist[block * nbins + bin] for the band's bins (without LDS histograms the counts go
+// straight to counts[bin]).
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
             uint32_t* __restrict__ blockhist) {
     extern __shared__ uint32_t hist[];
-    if (wg.use_lds) {
-        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = 0;
-        __syncthreads();
+    if (!wg.use_lds) {
+        for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
+            atomicAdd(&counts[bin], 1u);
+        });
+        return;
     }
+    int lo, n;
+    lds_band(wg, lo, n);
+    for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = 0;
+    __syncthreads();
     for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
-        if (wg.use_lds) atomicAdd(&hist[bin], 1u);
-        else atomicAdd(&counts[bin], 1u);
+        const uint32_t b = static_cast<uint32_t>(bin - lo);
+        if (b < static_cast<uint32_t>(n)) atomicAdd(&hist[b], 1u);
     });
     __syncthreads();
-    if (wg.use_lds) {
-        uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
-        for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) row[b] = hist[b];
-    }
+    uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
+    for (int b = threadIdx.x; b < n; b += kBinThreads) row[b] = hist[b];
 }
 
 // The per-block histograms turned into per-block record offsets without global atomics per
@@ -245,9 +260,9 @@ k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, co
     }
 }
 
-// Records of this block's samples, in one pass: with LDS histograms the block's first slot of
-// each bin comes from blockhist (the offsets of k_bin_block_offsets) and LDS cursors hand out
-// the slots; otherwise every record takes a global cursor slot.
+// Records of this block's samples, in one pass per band: with LDS histograms the block's first
+// slot of each of the band's bins comes from blockhist (the offsets of k_bin_block_offsets) and
+// LDS cursors hand out the slots; otherwise every record takes a global cursor slot.
 // `limit` (the record buffer's capacity) only bounds the writes.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
@@ -274,11 +289,14 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         block_max_to(m, vmax_bits);
         return;
     }
-    const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins;
-    for (int b = threadIdx.x; b < wg.nbins; b += kBinThreads) hist[b] = row[b];
+    int lo, n;
+    lds_band(wg, lo, n);
+    const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
+    for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = row[b];
     __syncthreads();
     for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
-        emit(atomicAdd(&hist[bin], 1u), f, h, wb);
+        const uint32_t b = static_cast<uint32_t>(bin - lo);
+        if (b < static_cast<uint32_t>(n)) emit(atomicAdd(&hist[b], 1u), f, h, wb);
     });
     block_max_to(m, vmax_bits);
 }
@@ -874,11 +892,11 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
     WinGeom wg;
     wg.nw = (gp.G + kWin - 1) / kWin;
-    // Row classes of 4 cells, or of 8 when only the coarser histogram fits in LDS.
+    // Row classes of 4 cells when that histogram fits in LDS, else of 8 (half the bins, so half
+    // the LDS bands when even those do not fit).
     const size_t cols = static_cast<size_t>(gp.nslabs) * wg.nw;
     const bool fits4 = cols * ((gp.G + 3) / 4) <= static_cast<size_t>(kLdsBins);
-    const bool fits8 = cols * ((gp.G + 7) / 8) <= static_cast<size_t>(kLdsBins);
-    wg.cls_log2 = (!fits4 && fits8) ? 3 : 2;
+    wg.cls_log2 = fits4 ? 2 : 3;
     wg.ncls = 24 >> wg.cls_log2;
     wg.nh = (gp.G + (1 << wg.cls_log2) - 1) >> wg.cls_log2;
     wg.nwin = wg.nw * wg.nw;
@@ -886,7 +904,6 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const size_t nbins_sz = static_cast<size_t>(gp.nslabs) * wg.nw * wg.nh;
     if (nbins_sz >= (1ull << 30) || gp.G > 8192) throw UnsupportedError("grid too large for the window gridder");
     wg.nbins = static_cast<int>(nbins_sz);
-    wg.use_lds = wg.nbins <= kLdsBins ? 1 : 0;
     const size_t S = in.num_samples;
     const size_t max_records = 2 * S;                      // one row class, at most two window columns
     if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
@@ -901,8 +918,15 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     unsigned blocks;
     if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinRows - 1) / kBinRows);
     else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));
+    // LDS bands of at most kLdsBins bins, unless the per-block histograms would pass
+    // kMaxBlockHistBytes (then one global atomic per record).
+    const size_t nbands = (nbins + kLdsBins - 1) / kLdsBins;
+    wg.use_lds = static_cast<size_t>(blocks) * nbins * sizeof(uint32_t) <= kMaxBlockHistBytes ? 1 : 0;
+    wg.band_bins = static_cast<int>((nbins + nbands - 1) / nbands);
+    const unsigned bands = wg.use_lds ? static_cast<unsigned>(nbands) : 1u;
     const size_t nhist = wg.use_lds ? static_cast<size_t>(blocks) * nbins : 1;
-    const size_t bin_lds = wg.use_lds ? nbins * sizeof(uint32_t) : 0;
+    const size_t bin_lds = wg.use_lds ? static_cast<size_t>(wg.band_bins) * sizeof(uint32_t) : 0;
+    const dim3 bin_grid(blocks, bands);
 
     // counts[0, nbins) | counts[nbins + 2, +8) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 12) * sizeof(uint32_t)));
@@ -929,10 +953,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
         allow_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
-        k_win_count<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
+        k_win_count<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     } else {
         allow_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
-        k_win_count<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
+        k_win_count<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     }
     SVDI_LAUNCH_CHECK();
     const unsigned seg_grid = static_cast<unsigned>((kBinSegs * nbins + 255) / 256);
@@ -954,11 +978,11 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot) {
         allow_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
-        k_win_scatter<true><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
+        k_win_scatter<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
                                                                          totals + 4);
     } else {
         allow_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
-        k_win_scatter<false><<<blocks, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
+        k_win_scatter<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
                                                                           totals + 4);
     }
     SVDI_LAUNCH_CHECK();
