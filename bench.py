@@ -18,6 +18,8 @@ pixels plus the peak pixel of the signed, pre-reprojection image.
                   [--batch B] [--no-compare] [--no-accuracy] [--no-cpu-baseline]
                   [--cpu-samples N]
   python bench.py --copy-calibration MIB     (plain copy, the PMC byte counters' reference)
+  python bench.py --serve 50,100,200,400 [--serve-seconds T] [--serve-pool P]
+                                             (streaming form: latency under Poisson arrivals)
 """
 from __future__ import annotations
 
@@ -188,6 +190,69 @@ def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64):
     return out
 
 
+def serve_main(args):
+    """Streaming form: requests for snapshots of the batching form (hour angles over +-4 h, a
+    fraction U(0, 0.6) of rows removed, images of 512^2, 1024^2 or 2048^2) arrive as a Poisson
+    process; the service admits them against a device memory budget, batches them by image
+    size and reports latency from arrival to image-ready."""
+    import numpy as np
+    import torch
+    import svdimager_amd as S
+    from svdimager_amd import serve, synth
+
+    rates = [float(r) for r in args.serve.split(",") if r.strip()]
+    cfg = synth.SKALOW_BATCHING
+    n = args.serve_pool
+    pool = []
+    t0 = time.perf_counter()
+    for i in range(n):
+        s = synth.make_batching_snapshot(i, n, cfg, device="cuda")
+        pool.append((S.DeviceSnapshot(s, "cuda"), int(s["image_size"]), cfg.cell))
+        log(f"serve: snapshot {i + 1}/{n}: {s['num_rows'] * s['num_channels']} samples, "
+            f"{s['image_size']}^2")
+        del s
+    log(f"serve: pool generated in {time.perf_counter() - t0:.1f} s")
+    budget = int(args.serve_budget_gib * (1 << 30))
+    results = {}
+    workspace = {}
+    for name, gridder in (("mfma", S.GRIDDER_MFMA), ("plain-fp32", S.GRIDDER_TILED)):
+        ctx = S.Context(0)
+        opts = dict(gridder=gridder, output=S.OUTPUT_DIRTY)
+        svc = serve.SnapshotService(ctx, budget, max_batch=8, max_inflight=2, **opts)
+        workspace[name] = svc.reserve([serve.Request(i, d, N, c, 0.0) for i, (d, N, c) in enumerate(pool)])
+        log(f"serve[{name}]: workspace {workspace[name] / 2**30:.2f} GiB after the reserve pass")
+        serve.run_poisson(svc, pool, rate=min(rates), duration=0.3, seed=12345)     # warm-up
+        rows = []
+        for rate in rates:
+            svc = serve.SnapshotService(ctx, budget, max_batch=8, max_inflight=2, **opts)
+            out = serve.run_poisson(svc, pool, rate=rate, duration=args.serve_seconds,
+                                    seed=int(rate), drain_limit=120.0)
+            row = {k: out[k] for k in ("offered_rate", "count", "snapshots_per_s", "mvis_per_s",
+                                       "latency_ms", "queue_ms", "mean_batch")}
+            row["peak_admitted_bytes"] = svc.peak_held
+            row["workspace_bytes"] = svc.workspace
+            rows.append(row)
+            lat = out["latency_ms"]
+            log(f"serve[{name}] {rate:g} req/s: {out['count']} requests, p50 {lat['p50']:.2f} ms, "
+                f"p99 {lat['p99']:.2f} ms, {out['snapshots_per_s']:.0f} snapshots/s, "
+                f"mean batch {out['mean_batch']:.2f}")
+        results[name] = rows
+        torch.cuda.synchronize()
+        ctx.close()
+    sizes = [N for _, N, _ in pool]
+    samples = [d.num_samples for d, _, _ in pool]
+    print(json.dumps({
+        "metric": "p50/p99 latency per snapshot, Poisson arrivals (streaming form)",
+        "unit": "ms", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+        "config": {"workload": "skalow-512st-64ch batching form (hour angles over +-4 h, "
+                               "U(0, 0.6) of rows removed, images 512/1024/2048)",
+                   "pool": n, "image_sizes": {str(k): sizes.count(k) for k in sorted(set(sizes))},
+                   "samples_min": min(samples), "samples_max": max(samples),
+                   "arrival_window_s": args.serve_seconds, "budget_bytes": budget,
+                   "max_batch": 8, "max_inflight_batches": 2, "output": "dirty"},
+        "workspace_bytes": workspace, "results": results}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +270,24 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (three times) and exit: the "
                          "reference reading for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--serve", default="", metavar="RATES",
+                    help="only run the streaming form and exit: Poisson arrivals at each of the "
+                         "comma-separated RATES (requests/s, e.g. 50,100,200,400) of snapshots "
+                         "drawn from the batching form, through the MFMA and the plain FP32 "
+                         "imagers; prints p50/p99 latency and throughput per rate")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per rate")
+    ap.add_argument("--serve-pool", type=int, default=16,
+                    help="distinct device-resident snapshots the requests are drawn from")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0,
+                    help="device memory budget of the service (workspace + admitted requests)")
     args = ap.parse_args()
+    if args.serve:
+        if args.gpus != 1:
+            print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported",
+                  file=sys.stderr)
+            sys.exit(2)
+        serve_main(args)
+        return
     if args.copy_calibration > 0:
         import torch
         n = args.copy_calibration * (1 << 20) // 4

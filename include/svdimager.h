@@ -113,8 +113,13 @@ int svdi_create(svdi_context** ctx, int device);
 int svdi_destroy(svdi_context* ctx);
 /* Run subsequent work on a caller stream (hipStream_t); NULL restores the context's stream. */
 int svdi_set_stream(svdi_context* ctx, void* hip_stream);
-/* Release the workspace (it regrows on the next call). */
+/* Release the workspace, the batch lanes' included (it regrows on the next call). */
 int svdi_trim(svdi_context* ctx);
+/* Device bytes the context's workspace holds, the batch lanes' included (no counterpart in the
+ * reference, which allocates and frees inside every call, SVDIntegratedImager.cu:1498-1761).
+ * The workspace only grows, so after calls at the largest shapes a caller knows what the
+ * context keeps and can budget the rest of device memory. */
+int svdi_workspace_bytes(svdi_context* ctx, size_t* bytes);
 
 /* Stage timing with HIP events on the context stream (no counterpart in the reference, which
  * times only the whole slab loop, SVDIntegratedImager.cu:1657-1741).  When enabled, each call

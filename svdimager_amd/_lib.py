@@ -27,7 +27,7 @@ EXPORTED_SYMBOLS = [
     "svdi_set_stream", "svdi_trim", "svdi_preprocess", "svdi_image_preprocessed",
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
-    "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async",
+    "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -99,6 +99,7 @@ def lib():
             "svdi_destroy": (ctypes.c_int, [vp]),
             "svdi_set_stream": (ctypes.c_int, [vp, vp]),
             "svdi_trim": (ctypes.c_int, [vp]),
+            "svdi_workspace_bytes": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_size_t)]),
             "svdi_preprocess": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, vp, vp, vp]),
             "svdi_snapshot_geometry": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, vp]),
             "svdi_image_preprocessed": (ctypes.c_int, [vp, vp, vp, vp, vp, sz, vp, sz, f,
@@ -167,6 +168,12 @@ class Context:
 
     def trim(self) -> None:
         check(lib().svdi_trim(self._h))
+
+    def workspace_bytes(self) -> int:
+        """Device bytes the context's workspace holds (batch lanes included)."""
+        n = ctypes.c_size_t()
+        check(lib().svdi_workspace_bytes(self._h, ctypes.byref(n)))
+        return int(n.value)
 
     def set_profiling(self, enable: bool) -> None:
         check(lib().svdi_set_profiling(self._h, 1 if enable else 0))

@@ -602,9 +602,26 @@ int svdi_trim(svdi_context* ctx) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& lane : ctx->lanes) {
+            SVDI_HIP(hipStreamSynchronize(lane->stream));
+            lane->ws.release();
+            lane->twiddle_n = 0;
+            lane->conv_at = nullptr;
+        }
         ctx->ws.release();
         ctx->twiddle_n = 0;
         ctx->conv_at = nullptr;
+    });
+}
+
+int svdi_workspace_bytes(svdi_context* ctx, size_t* bytes) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        if (!bytes) throw ArgError("bytes is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        size_t t = ctx->ws.bytes_held();
+        for (const auto& lane : ctx->lanes) t += lane->ws.bytes_held();
+        *bytes = t;
     });
 }
 

@@ -58,6 +58,11 @@ class DeviceSnapshot:
     def num_samples(self) -> int:
         return self.num_rows * self.num_channels
 
+    @property
+    def nbytes(self) -> int:
+        """Device bytes of the snapshot's arrays."""
+        return sum(t.numel() * t.element_size() for t in self.tensors.values() if t is not None)
+
     def struct(self) -> Snapshot:
         t = self.tensors
         p = lambda k: (None if t[k] is None else _ptr(t[k]))

@@ -202,3 +202,51 @@ def make_snapshot(cfg: SnapshotConfig, ha_hours: Optional[float] = None,
     out["array_center_x"] = ARRAY_CENTER_X
     out["sources"] = np.stack([l, m, flux], axis=1)
     return out
+
+
+# Batching form: the SKA-Low array at hour angles spread over +-4 h.  Snapshot i of n keeps a
+# fraction 1 - f of its rows, f ~ U(0, 0.6) (flagged rows removed, so the row counts are uneven:
+# 3.3 M to 8.4 M samples), and is imaged at a size drawn from {512, 1024, 2048}.
+BATCHING_SIZES = (512, 1024, 2048)
+SKALOW_BATCHING = dataclasses.replace(
+    SKALOW, name="skalow-512st-64ch-mixed", flag_fraction=0.0, seed=BASE_SEED + 3)
+PER_SAMPLE_FIELDS = ("vis0_re", "vis0_im", "vis3_re", "vis3_im", "flag0", "flag3", "weight0",
+                     "weight3")
+
+
+def batching_draw(index: int, count: int, cfg: SnapshotConfig = SKALOW_BATCHING,
+                  sizes=BATCHING_SIZES, max_removed: float = 0.6):
+    """(hour angle in hours, fraction of rows removed, image size) of snapshot `index` of
+    `count`: hour angles evenly over [-4 h, +4 h], the rest drawn from a stream seeded by
+    (cfg.seed, index)."""
+    rng = np.random.default_rng([cfg.seed, index])
+    ha = -4.0 + 8.0 * index / (count - 1) if count > 1 else 0.0
+    removed = float(rng.uniform(0.0, max_removed))
+    size = int(sizes[int(rng.integers(0, len(sizes)))])
+    return ha, removed, size
+
+
+def drop_rows(snap: Dict, keep: np.ndarray) -> Dict:
+    """The snapshot restricted to rows `keep` (ascending), in the same row + chan * R layout."""
+    R, C = int(snap["num_rows"]), int(snap["num_channels"])
+    out = dict(snap)
+    out["uvw"] = np.ascontiguousarray(snap["uvw"][keep])
+    for k in PER_SAMPLE_FIELDS:
+        if snap.get(k) is not None:
+            out[k] = np.ascontiguousarray(snap[k].reshape(C, R)[:, keep].reshape(-1))
+    out["num_rows"] = int(len(keep))
+    return out
+
+
+def make_batching_snapshot(index: int, count: int, cfg: SnapshotConfig = SKALOW_BATCHING,
+                           sizes=BATCHING_SIZES, device: Optional[str] = None) -> Dict:
+    """Snapshot `index` of `count` of the batching form, with its image size under
+    ``"image_size"``."""
+    ha, removed, size = batching_draw(index, count, cfg, sizes)
+    snap = make_snapshot(cfg, ha_hours=ha, device=device, seed_offset=index)
+    R = int(snap["num_rows"])
+    rng = np.random.default_rng([cfg.seed, index, 1])
+    keep = np.sort(rng.choice(R, size=int(round(R * (1.0 - removed))), replace=False))
+    out = drop_rows(snap, keep)
+    out["image_size"] = size
+    return out

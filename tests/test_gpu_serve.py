@@ -1,0 +1,77 @@
+"""GPU tests of the streaming service (svdimager_amd/serve.py) on the batching form at small
+size: snapshots of the 64-station array at hour angles over +-4 h with a random fraction of rows
+removed and image sizes 64 and 128, arriving as a Poisson process.  Every served image equals
+the same snapshot imaged alone (to the FP32 summation-order tolerance), under a loose budget and
+under one that admits about two requests at a time."""
+import dataclasses
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import DIRTY_CENTRAL, announce, assert_parity
+import svdimager_amd as S
+from svdimager_amd import serve, synth
+
+pytestmark = pytest.mark.gpu
+
+SIZES = (64, 128)
+
+
+@pytest.fixture(scope="module")
+def pool():
+    cfg = dataclasses.replace(synth.SMALL, flag_fraction=0.0)
+    out = []
+    for i in range(6):
+        s = synth.make_batching_snapshot(i, 6, cfg, sizes=SIZES)
+        out.append((S.DeviceSnapshot(s, "cuda"), s["image_size"], cfg.cell))
+    return out
+
+
+@pytest.fixture(scope="module")
+def singles(gpu_ctx, pool):
+    refs = []
+    for d, N, cell in pool:
+        img, _ = S.image_snapshot(gpu_ctx, d, N, cell, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
+        refs.append(img.cpu().numpy())
+    torch.cuda.synchronize()
+    return refs
+
+
+def check_served(out, pool, singles):
+    index = {id(d): i for i, (d, _, _) in enumerate(pool)}
+    for r in out["requests"]:
+        i = index[id(r.snapshot)]
+        assert r.image.shape == (r.image_size, r.image_size)
+        assert_parity(r.image.cpu().numpy(), singles[i], f"request {r.rid} (pool entry {i})",
+                      central=DIRTY_CENTRAL)
+
+
+def test_served_images_match_single_calls(gpu_ctx, pool, singles):
+    announce("service: loose budget, 400 req/s for 0.25 s")
+    svc = serve.SnapshotService(gpu_ctx, budget_bytes=8 << 30, max_batch=4, max_inflight=2,
+                                keep_images=True, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
+    svc.reserve([serve.Request(i, d, N, cell, 0.0) for i, (d, N, cell) in enumerate(pool)])
+    out = serve.run_poisson(svc, pool, rate=400.0, duration=0.25, seed=1, drain_limit=30.0)
+    torch.cuda.synchronize()
+    assert out["count"] > 20
+    assert {r.image_size for r in out["requests"]} == set(SIZES)
+    assert svc.workspace + svc.peak_held <= svc.budget
+    check_served(out, pool, singles)
+
+
+def test_tight_budget_admits_two_at_a_time(gpu_ctx, pool, singles):
+    announce("service: budget for two requests")
+    svc = serve.SnapshotService(gpu_ctx, budget_bytes=1 << 40, max_batch=4, max_inflight=2,
+                                keep_images=True, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
+    ws = svc.reserve([serve.Request(i, d, N, cell, 0.0) for i, (d, N, cell) in enumerate(pool)])
+    costs = [serve.request_bytes(serve.Request(0, d, N, cell, 0.0)) for d, N, cell in pool]
+    biggest, smallest = max(costs), min(costs)
+    svc.budget = ws + 2 * biggest
+    out = serve.run_poisson(svc, pool, rate=1000.0, duration=0.05, seed=2, drain_limit=30.0)
+    torch.cuda.synchronize()
+    assert out["count"] > 10
+    assert svc.workspace == ws                   # no growth after the reserve pass
+    assert svc.peak_held <= 2 * biggest
+    assert max(svc.batch_sizes) <= (2 * biggest) // smallest
+    check_served(out, pool, singles)
