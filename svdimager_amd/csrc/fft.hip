@@ -327,7 +327,11 @@ __device__ __forceinline__ void wave_lds_fence() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <class Load, class Store>
+// kTwStride: the table holds e^{2 pi i t / (1536 kTwStride)} (W_1536^m = tw[kTwStride m]).
+// kBlockSync: a workgroup barrier (instead of a wavefront fence) between the loads and the first
+// tile store, for callers whose wavefronts load from each other's tiles; every wavefront of the
+// workgroup must then call this function.
+template <int kTwStride = 1, bool kBlockSync = false, class Load, class Store>
 __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restrict__ tw, int lane,
                                              Load&& load, Store&& store) {
     float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
@@ -335,7 +339,8 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
     for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
-    wave_lds_fence();                              // the tile may be where the loads came from
+    if constexpr (kBlockSync) __syncthreads();
+    else wave_lds_fence();                         // the tile may be where the loads came from
     // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
     // 3-point DFTs over v.  W_24^m = tw[64 m].
 #pragma unroll
@@ -343,7 +348,7 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
 #pragma unroll
     for (int v = 1; v < 3; ++v)
 #pragma unroll
-        for (int s8 = 1; s8 < 8; ++s8) y[v][s8] = cmul(y[v][s8], tw[64 * v * s8]);
+        for (int s8 = 1; s8 < 8; ++s8) y[v][s8] = cmul(y[v][s8], tw[kTwStride * 64 * v * s8]);
 #pragma unroll
     for (int s8 = 0; s8 < 8; ++s8) {
         float2 t3[3] = {y[0][s8], y[1][s8], y[2][s8]};
@@ -352,7 +357,7 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
         for (int t = 0; t < 3; ++t) {
             const int k1 = s8 + 8 * t;
             // times W^{l k1} (l k1 < 1536), into the tile at [k1][l]
-            tile[k1 * kTileStride + lane] = k1 == 0 ? t3[t] : cmul(t3[t], tw[lane * k1]);
+            tile[k1 * kTileStride + lane] = k1 == 0 ? t3[t] : cmul(t3[t], tw[kTwStride * lane * k1]);
         }
     }
     wave_lds_fence();
@@ -370,7 +375,7 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
         const int q = lane + 64 * j, k1 = q % 24, b = q / 24;
         dft8(w[j]);
 #pragma unroll
-        for (int c = 1; c < 8; ++c) w[j][c] = cmul(w[j][c], tw[24 * b * c]);
+        for (int c = 1; c < 8; ++c) w[j][c] = cmul(w[j][c], tw[kTwStride * 24 * b * c]);
 #pragma unroll
         for (int c = 0; c < 8; ++c) tile[k1 * kTileStride + 8 * b + c] = w[j][c];
     }
@@ -457,6 +462,82 @@ k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
     for (int t = threadIdx.x; t < rows * kColStrip; t += 64 * kColStrip) {
         const int r = t / kColStrip, c = t % kColStrip;
         if (c < nc) dst[static_cast<size_t>(r) * cols + c] = strip[c * kColLen + row_lo + r];
+    }
+}
+
+// G = 3072 = 2 x 1536: one radix-2 step over two 1536-point transforms.  E and O are the
+// transforms of the even and odd samples (twiddles W_1536^m = tw[2 m] of the 3072-point table);
+// X[k] = E[k] + W^k O[k] and X[k + 1536] = E[k] - W^k O[k], W = e^{2 pi i / 3072} = tw[1].
+
+// Row pass for G = 3072: a pair of wavefronts per row (even samples, odd samples), each into its
+// own LDS tile, then the pair combines them and stores the columns col_lo .. col_lo + cols - 1.
+// Two rows per workgroup; total_rows (P x 3072) is even, so every wavefront of every workgroup
+// has a row and reaches the barrier.
+constexpr int kRows3072 = 2;
+
+__global__ void __launch_bounds__(128 * kRows3072)
+k_fft_rows_3072(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int col_lo, int cols) {
+    constexpr int G = 3072, H = 1536;
+    __shared__ float2 tile_all[2 * kRows3072][24 * kTileStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pair = wave >> 1, half = wave & 1;
+    const int row = blockIdx.x * kRows3072 + pair;
+    const float2* src = in + static_cast<size_t>(row) * G;
+    float2* tile = tile_all[wave];
+    fft1536_wave<2>(tile, tw, lane, [&](int n) { return src[2 * n + half]; },
+                    [&](int k, float2 v) { tile[k] = v; });
+    __syncthreads();
+    const float2* E = tile_all[2 * pair];
+    const float2* O = tile_all[2 * pair + 1];
+    float2* dst = out + static_cast<size_t>(row) * cols - col_lo;
+    for (int k = half * 64 + lane; k < H; k += 128) {
+        const float2 e = E[k], wo = cmul(O[k], tw[k]);
+        if (k >= col_lo && k < col_lo + cols) dst[k] = cadd(e, wo);
+        if (k + H >= col_lo && k + H < col_lo + cols) dst[k + H] = csub(e, wo);
+    }
+}
+
+// Column pass for G = 3072: one workgroup per (plane, strip of kColStrip columns), two
+// wavefronts per column.  The strip is read into LDS (column c at c kCol3072Len, each column
+// region two exchange tiles long); the even-sample wavefront's tile is the first half of the
+// region and the odd one's the second, so both load their samples before either stores (the
+// barrier inside fft1536_wave).  Wavefronts of columns past the last one transform stale LDS
+// and store nothing.  Rows row_lo .. row_lo + rows - 1 are written back as row segments.
+constexpr int kCol3072Len = 2 * kColLen;           // float2 per column region (>= 3072)
+constexpr size_t kCol3072Lds = sizeof(float2) * kColStrip * kCol3072Len;
+
+__global__ void __launch_bounds__(128 * kColStrip)
+k_fft_cols_3072(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int cols, int row_lo, int rows, int strips) {
+    constexpr int G = 3072, H = 1536;
+    extern __shared__ float2 strip3[];
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    const int plane = tile / strips;
+    const int c0 = (tile - plane * strips) * kColStrip;
+    const int nc = min(kColStrip, cols - c0);
+    const float2* src = in + static_cast<size_t>(plane) * G * cols + c0;
+    for (int t = threadIdx.x; t < G * kColStrip; t += 128 * kColStrip) {
+        const int r = t / kColStrip, c = t % kColStrip;
+        if (c < nc) strip3[c * kCol3072Len + r] = src[static_cast<size_t>(r) * cols + c];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = wave >> 1, half = wave & 1;
+    float2* col = strip3 + c * kCol3072Len;
+    float2* tl = col + half * kColLen;
+    fft1536_wave<2, true>(tl, tw, lane, [&](int n) { return col[2 * n + half]; },
+                          [&](int k, float2 v) { tl[k] = v; });
+    __syncthreads();
+    float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
+    for (int t = threadIdx.x; t < H * kColStrip; t += 128 * kColStrip) {
+        const int k = t / kColStrip, cc = t % kColStrip;
+        if (cc >= nc) continue;
+        const float2* cl = strip3 + cc * kCol3072Len;
+        const float2 e = cl[k], wo = cmul(cl[kColLen + k], tw[k]);
+        const int r0 = k - row_lo, r1 = k + H - row_lo;
+        if (r0 >= 0 && r0 < rows) dst[static_cast<size_t>(r0) * cols + cc] = cadd(e, wo);
+        if (r1 >= 0 && r1 < rows) dst[static_cast<size_t>(r1) * cols + cc] = csub(e, wo);
     }
 }
 
@@ -610,7 +691,11 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cols_fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    if (G == 1536) {
+    if (G == 3072) {
+        const int total_rows = static_cast<int>(P * G);          // even: two rows per workgroup
+        k_fft_rows_3072<<<static_cast<unsigned>(total_rows / kRows3072), 128 * kRows3072, 0, ctx->stream>>>(
+            planes, row_out, tw, static_cast<int>(col_lo), static_cast<int>(cols));
+    } else if (G == 1536) {
         const int total_rows = static_cast<int>(P * G);
         k_fft_rows_1536<<<static_cast<unsigned>((total_rows + kFftWaves - 1) / kFftWaves), 64 * kFftWaves, 0,
                           ctx->stream>>>(planes, row_out, tw, total_rows, static_cast<int>(col_lo),
@@ -621,7 +706,13 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                                    static_cast<int>(cols));
     }
     SVDI_LAUNCH_CHECK();
-    if (G == 1536) {
+    if (G == 3072) {
+        const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
+        SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols_3072),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCol3072Lds)));
+        k_fft_cols_3072<<<static_cast<unsigned>(strips * P), 128 * kColStrip, kCol3072Lds, ctx->stream>>>(
+            row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
+    } else if (G == 1536) {
         const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
         k_fft_cols_1536<<<static_cast<unsigned>(strips * P), 64 * kColStrip, 0, ctx->stream>>>(
             row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
