@@ -328,10 +328,7 @@ __device__ __forceinline__ void wave_lds_fence() {
 }
 
 // kTwStride: the table holds e^{2 pi i t / (1536 kTwStride)} (W_1536^m = tw[kTwStride m]).
-// kBlockSync: a workgroup barrier (instead of a wavefront fence) between the loads and the first
-// tile store, for callers whose wavefronts load from each other's tiles; every wavefront of the
-// workgroup must then call this function.
-template <int kTwStride = 1, bool kBlockSync = false, class Load, class Store>
+template <int kTwStride = 1, class Load, class Store>
 __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restrict__ tw, int lane,
                                              Load&& load, Store&& store) {
     float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
@@ -339,8 +336,7 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
     for (int u = 0; u < 8; ++u)
 #pragma unroll
         for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
-    if constexpr (kBlockSync) __syncthreads();
-    else wave_lds_fence();                         // the tile may be where the loads came from
+    wave_lds_fence();                              // the tile may be where the loads came from
     // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
     // 3-point DFTs over v.  W_24^m = tw[64 m].
 #pragma unroll
@@ -430,24 +426,30 @@ __device__ __forceinline__ int xcd_local_tile(int linear, int total) {
     return x * q + min(x, r) + local;
 }
 
-// Column pass for G = 1536: one workgroup per (plane, strip of kColStrip columns).  The strip is
-// read into LDS with the columns side by side (row segments of kColStrip columns), each
-// wavefront transforms one column in place (the column's LDS region is also its exchange
-// tile), and the rows row_lo .. row_lo + rows - 1 are written back as row segments.
+// Column pass of 1536-point transforms: one workgroup per (half-plane, strip of kColStrip
+// columns).  The strip is read into LDS with the columns side by side (row segments of
+// kColStrip columns), each wavefront transforms one column in place (the column's LDS region is
+// also its exchange tile), and the output rows in the window row_lo .. row_lo + rows - 1 are
+// written back as row segments.  kSplit = 1: G = 1536, the half-planes are the planes and
+// transform index k is output row k.  kSplit = 2: G = 3072 after the row pass's radix-2 step;
+// half-plane q holds a (q even) or b (q odd) of plane q / 2, and k is output row 2 k + q % 2
+// (twiddles W_1536^m = tw[2 m]).
 constexpr int kColStrip = 4;
 constexpr int kColLen = 24 * kTileStride;          // float2 per column region (>= 1536)
 
+template <int kSplit>
 __global__ void __launch_bounds__(64 * kColStrip)
 k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
                 int cols, int row_lo, int rows, int strips) {
-    constexpr int G = 1536;
+    constexpr int H = 1536;
     __shared__ float2 strip[kColStrip * kColLen];
     const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
-    const int plane = tile / strips;
-    const int c0 = (tile - plane * strips) * kColStrip;
+    const int q = tile / strips;
+    const int plane = q / kSplit, parity = q - plane * kSplit;
+    const int c0 = (tile - q * strips) * kColStrip;
     const int nc = min(kColStrip, cols - c0);
-    const float2* src = in + static_cast<size_t>(plane) * G * cols + c0;
-    for (int t = threadIdx.x; t < G * kColStrip; t += 64 * kColStrip) {
+    const float2* src = in + static_cast<size_t>(q) * H * cols + c0;
+    for (int t = threadIdx.x; t < H * kColStrip; t += 64 * kColStrip) {
         const int r = t / kColStrip, c = t % kColStrip;
         if (c < nc) strip[c * kColLen + r] = src[static_cast<size_t>(r) * cols + c];
     }
@@ -455,89 +457,63 @@ k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave < nc) {
         float2* col = strip + wave * kColLen;
-        fft1536_wave(col, tw, lane, [&](int n) { return col[n]; }, [&](int k, float2 v) { col[k] = v; });
+        fft1536_wave<kSplit>(col, tw, lane, [&](int n) { return col[n]; }, [&](int k, float2 v) { col[k] = v; });
     }
     __syncthreads();
+    // Transform outputs k_lo .. k_lo + nk - 1 land in the window (output row kSplit k + parity).
+    const int k_lo = max(0, (row_lo - parity + kSplit - 1) / kSplit);
+    const int nk = min(H, (row_lo + rows - parity + kSplit - 1) / kSplit) - k_lo;
     float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
-    for (int t = threadIdx.x; t < rows * kColStrip; t += 64 * kColStrip) {
-        const int r = t / kColStrip, c = t % kColStrip;
-        if (c < nc) dst[static_cast<size_t>(r) * cols + c] = strip[c * kColLen + row_lo + r];
+    for (int t = threadIdx.x; t < nk * kColStrip; t += 64 * kColStrip) {
+        const int k = k_lo + t / kColStrip, c = t % kColStrip;
+        const int r = kSplit * k + parity - row_lo;
+        if (c < nc && r >= 0 && r < rows) dst[static_cast<size_t>(r) * cols + c] = strip[c * kColLen + k];
     }
 }
 
-// G = 3072 = 2 x 1536: one radix-2 step over two 1536-point transforms.  E and O are the
-// transforms of the even and odd samples (twiddles W_1536^m = tw[2 m] of the 3072-point table);
-// X[k] = E[k] + W^k O[k] and X[k + 1536] = E[k] - W^k O[k], W = e^{2 pi i / 3072} = tw[1].
+// G = 3072 = 2 x 1536, a radix-2 step in each direction around 1536-point transforms.
+// Along a row (decimation in time): E and O are the transforms of the even and odd samples
+// (twiddles W_1536^m = tw[2 m] of the 3072-point table), X[k] = E[k] + W^k O[k] and
+// X[k + 1536] = E[k] - W^k O[k], W = e^{2 pi i / 3072}.  Along a column (decimation in
+// frequency, done in the row pass on the row-transformed rows x_n): a_n = x_n + x_{n + 1536} and
+// b_n = (x_n - x_{n + 1536}) W^n for n < 1536; the 1536-point column transforms of a and b give
+// the even and odd output rows, X[2 k] = A[k] and X[2 k + 1] = B[k].
 
-// Row pass for G = 3072: a pair of wavefronts per row (even samples, odd samples), each into its
-// own LDS tile, then the pair combines them and stores the columns col_lo .. col_lo + cols - 1.
-// Two rows per workgroup; total_rows (P x 3072) is even, so every wavefront of every workgroup
-// has a row and reaches the barrier.
-constexpr int kRows3072 = 2;
-
-__global__ void __launch_bounds__(128 * kRows3072)
+// Row pass for G = 3072: one workgroup per (plane, n < 1536) transforms rows n and n + 1536, a
+// pair of wavefronts per row (even samples, odd samples, each into its own LDS tile), combines
+// them and stores a_n as row n and b_n as row 1536 + n of the plane's output (the columns
+// col_lo .. col_lo + cols - 1).
+__global__ void __launch_bounds__(256)
 k_fft_rows_3072(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
                 int col_lo, int cols) {
     constexpr int G = 3072, H = 1536;
-    __shared__ float2 tile_all[2 * kRows3072][24 * kTileStride];
+    __shared__ float2 tile_all[4][24 * kTileStride];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int pair = wave >> 1, half = wave & 1;
-    const int row = blockIdx.x * kRows3072 + pair;
-    const float2* src = in + static_cast<size_t>(row) * G;
+    const int plane = blockIdx.x / H, n = blockIdx.x - plane * H;
+    const int which = wave >> 1, half = wave & 1;               // row n or n + 1536; even or odd
+    const float2* src = in + (static_cast<size_t>(plane) * G + n + which * H) * G;
     float2* tile = tile_all[wave];
-    fft1536_wave<2>(tile, tw, lane, [&](int n) { return src[2 * n + half]; },
+    fft1536_wave<2>(tile, tw, lane, [&](int m) { return src[2 * m + half]; },
                     [&](int k, float2 v) { tile[k] = v; });
     __syncthreads();
-    const float2* E = tile_all[2 * pair];
-    const float2* O = tile_all[2 * pair + 1];
-    float2* dst = out + static_cast<size_t>(row) * cols - col_lo;
-    for (int k = half * 64 + lane; k < H; k += 128) {
-        const float2 e = E[k], wo = cmul(O[k], tw[k]);
-        if (k >= col_lo && k < col_lo + cols) dst[k] = cadd(e, wo);
-        if (k + H >= col_lo && k + H < col_lo + cols) dst[k + H] = csub(e, wo);
-    }
-}
-
-// Column pass for G = 3072: one workgroup per (plane, strip of kColStrip columns), two
-// wavefronts per column.  The strip is read into LDS (column c at c kCol3072Len, each column
-// region two exchange tiles long); the even-sample wavefront's tile is the first half of the
-// region and the odd one's the second, so both load their samples before either stores (the
-// barrier inside fft1536_wave).  Wavefronts of columns past the last one transform stale LDS
-// and store nothing.  Rows row_lo .. row_lo + rows - 1 are written back as row segments.
-constexpr int kCol3072Len = 2 * kColLen;           // float2 per column region (>= 3072)
-constexpr size_t kCol3072Lds = sizeof(float2) * kColStrip * kCol3072Len;
-
-__global__ void __launch_bounds__(128 * kColStrip)
-k_fft_cols_3072(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-                int cols, int row_lo, int rows, int strips) {
-    constexpr int G = 3072, H = 1536;
-    extern __shared__ float2 strip3[];
-    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
-    const int plane = tile / strips;
-    const int c0 = (tile - plane * strips) * kColStrip;
-    const int nc = min(kColStrip, cols - c0);
-    const float2* src = in + static_cast<size_t>(plane) * G * cols + c0;
-    for (int t = threadIdx.x; t < G * kColStrip; t += 128 * kColStrip) {
-        const int r = t / kColStrip, c = t % kColStrip;
-        if (c < nc) strip3[c * kCol3072Len + r] = src[static_cast<size_t>(r) * cols + c];
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c = wave >> 1, half = wave & 1;
-    float2* col = strip3 + c * kCol3072Len;
-    float2* tl = col + half * kColLen;
-    fft1536_wave<2, true>(tl, tw, lane, [&](int n) { return col[2 * n + half]; },
-                          [&](int k, float2 v) { tl[k] = v; });
-    __syncthreads();
-    float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
-    for (int t = threadIdx.x; t < H * kColStrip; t += 128 * kColStrip) {
-        const int k = t / kColStrip, cc = t % kColStrip;
-        if (cc >= nc) continue;
-        const float2* cl = strip3 + cc * kCol3072Len;
-        const float2 e = cl[k], wo = cmul(cl[kColLen + k], tw[k]);
-        const int r0 = k - row_lo, r1 = k + H - row_lo;
-        if (r0 >= 0 && r0 < rows) dst[static_cast<size_t>(r0) * cols + cc] = cadd(e, wo);
-        if (r1 >= 0 && r1 < rows) dst[static_cast<size_t>(r1) * cols + cc] = csub(e, wo);
+    const float2 wn = tw[n];
+    float2* dst_a = out + (static_cast<size_t>(plane) * G + n) * cols - col_lo;
+    float2* dst_b = dst_a + static_cast<size_t>(H) * cols;
+    for (int k = threadIdx.x; k < H; k += 256) {
+        const float2 wk = tw[k];
+        const float2 en = tile_all[0][k], on = cmul(tile_all[1][k], wk);
+        const float2 em = tile_all[2][k], om = cmul(tile_all[3][k], wk);
+        const float2 xn0 = cadd(en, on), xn1 = csub(en, on);      // row n at columns k, k + 1536
+        const float2 xm0 = cadd(em, om), xm1 = csub(em, om);      // row n + 1536
+        if (k >= col_lo && k < col_lo + cols) {
+            dst_a[k] = cadd(xn0, xm0);
+            dst_b[k] = cmul(csub(xn0, xm0), wn);
+        }
+        const int k1 = k + H;
+        if (k1 >= col_lo && k1 < col_lo + cols) {
+            dst_a[k1] = cadd(xn1, xm1);
+            dst_b[k1] = cmul(csub(xn1, xm1), wn);
+        }
     }
 }
 
@@ -692,8 +668,7 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
     SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cols_fn),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     if (G == 3072) {
-        const int total_rows = static_cast<int>(P * G);          // even: two rows per workgroup
-        k_fft_rows_3072<<<static_cast<unsigned>(total_rows / kRows3072), 128 * kRows3072, 0, ctx->stream>>>(
+        k_fft_rows_3072<<<static_cast<unsigned>(P * 1536), 256, 0, ctx->stream>>>(
             planes, row_out, tw, static_cast<int>(col_lo), static_cast<int>(cols));
     } else if (G == 1536) {
         const int total_rows = static_cast<int>(P * G);
@@ -708,13 +683,11 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
     SVDI_LAUNCH_CHECK();
     if (G == 3072) {
         const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
-        SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_cols_3072),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kCol3072Lds)));
-        k_fft_cols_3072<<<static_cast<unsigned>(strips * P), 128 * kColStrip, kCol3072Lds, ctx->stream>>>(
+        k_fft_cols_1536<2><<<static_cast<unsigned>(strips * 2 * P), 64 * kColStrip, 0, ctx->stream>>>(
             row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
     } else if (G == 1536) {
         const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
-        k_fft_cols_1536<<<static_cast<unsigned>(strips * P), 64 * kColStrip, 0, ctx->stream>>>(
+        k_fft_cols_1536<1><<<static_cast<unsigned>(strips * P), 64 * kColStrip, 0, ctx->stream>>>(
             row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
     } else {
         const int strips = static_cast<int>((cols + width - 1) / width);
