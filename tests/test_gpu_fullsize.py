@@ -5,7 +5,9 @@ to the oracle at small sizes.  These sizes reach code paths the small cases cann
 * hour angle 0: one slab, 4-row record classes, windows split over several workgroup pieces
   whose partial windows are summed in piece order;
 * hour angle 4 h: two slabs and more Chebyshev terms (two or more eight-term passes), where
-  only the 8-row class histogram fits in LDS.
+  only the 8-row class histogram fits in LDS;
+* 2048^2 at hour angle -4 h and 4096^2: bins split into LDS bands, and the G = 3072 and general
+  long FFTs.
 
 The two gridders differ in summation order and in the MFMA's split-f16 operands (~22
 significant bits per product), so the signed images agree to the gridder tolerance of the
@@ -43,8 +45,8 @@ def test_skalow_mfma_vs_tiled(gpu_ctx, ha):
 
 def test_skalow_4k_mfma_vs_tiled(gpu_ctx):
     """The 4096^2 form (512 stations, 128 channels, 16,744,448 samples, G = 6144): no row-class
-    histogram fits in LDS there, so the window binning counts and places records with global
-    atomics.  MFMA against the FP32 tile gridder, signed images."""
+    histogram fits in LDS there, so the window binning splits its 294,912 bins into eight LDS
+    bands.  MFMA against the FP32 tile gridder, signed images."""
     cfg = synth.SKALOW_4K
     announce("SKA-Low 4096^2 (128 channels): MFMA vs tiled FP32")
     snap = synth.make_snapshot(cfg, device="cuda")
@@ -59,3 +61,24 @@ def test_skalow_4k_mfma_vs_tiled(gpu_ctx):
     a, b = img_m.cpu().numpy(), img_t.cpu().numpy()
     assert np.isfinite(a).all() and np.abs(b).max() > 0
     assert_parity(a, b, "MFMA vs tiled, 4096^2")
+
+
+def test_skalow_2048_far_from_transit_mfma_vs_tiled(gpu_ctx):
+    """A 2048^2 snapshot of the batching form at hour angle -4 h (G = 3072, four slabs, K = 16:
+    64 planes; window bins split into eight LDS bands; the G = 3072 transforms on 64 planes).
+    MFMA against the FP32 tile gridder, signed images."""
+    cfg = synth.SKALOW_BATCHING
+    announce("SKA-Low batching form, snapshot 0 of 16 (2048^2, HA -4 h): MFMA vs tiled FP32")
+    snap = synth.make_batching_snapshot(0, 16, cfg, device="cuda")
+    N = snap["image_size"]
+    d = S.DeviceSnapshot(snap, "cuda")
+    del snap
+    img_t, plan_t = S.image_snapshot(gpu_ctx, d, N, cfg.cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_TILED)
+    img_m, plan_m = S.image_snapshot(gpu_ctx, d, N, cfg.cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_MFMA)
+    torch.cuda.synchronize()
+    print(f"2048: slabs {plan_m['num_slabs']} K {plan_m['num_terms']} records {plan_m['records']}")
+    assert N == 2048 and plan_m["grid_size"] == 3072
+    assert plan_m["num_slabs"] == plan_t["num_slabs"] >= 3 and plan_m["num_terms"] == plan_t["num_terms"] == 16
+    a, b = img_m.cpu().numpy(), img_t.cpu().numpy()
+    assert np.isfinite(a).all() and np.abs(b).max() > 0
+    assert_parity(a, b, "MFMA vs tiled, 2048^2 at HA -4 h")
