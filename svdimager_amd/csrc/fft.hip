@@ -271,10 +271,27 @@ __device__ float2* stockham(float2* src, float2* dst, int ntrans, int pn, const 
     return src;
 }
 
+// put(t, get(t)) for t = threadIdx.x, + blockDim.x, ... < n, with the gets of kCopyBatch
+// consecutive steps issued before their puts (several global loads in flight per thread).
+constexpr int kCopyBatch = 8;
+template <class Get, class Put>
+__device__ __forceinline__ void copy_batched(int n, Get&& get, Put&& put) {
+    const int step = static_cast<int>(blockDim.x);
+    int t = static_cast<int>(threadIdx.x);
+    for (; t + (kCopyBatch - 1) * step < n; t += kCopyBatch * step) {
+        float2 v[kCopyBatch];
+#pragma unroll
+        for (int j = 0; j < kCopyBatch; ++j) v[j] = get(t + j * step);
+#pragma unroll
+        for (int j = 0; j < kCopyBatch; ++j) put(t + j * step, v[j]);
+    }
+    for (; t < n; t += step) put(t, get(t));
+}
+
 // LDS of a pass: the twiddle table (n float2), then two ping-pong buffers of ntrans padded
 // transforms.
 __device__ __forceinline__ const float2* load_twiddles(float2* lds, const float2* __restrict__ tw, int n) {
-    for (int t = threadIdx.x; t < n; t += blockDim.x) lds[t] = tw[t];
+    copy_batched(n, [&](int t) { return tw[t]; }, [&](int t, float2 v) { lds[t] = v; });
     return lds;
 }
 
@@ -295,7 +312,8 @@ k_fft_rows(const float2* __restrict__ in, float2* __restrict__ out, FftStages st
     const int nrows = min(batch, G - row0);
     const float2* src = in + (plane * G + row0) * static_cast<size_t>(G);
     for (int r = 0; r < nrows; ++r)
-        for (int c = threadIdx.x; c < G; c += blockDim.x) buf0[r * pn + pidx(c)] = src[static_cast<size_t>(r) * G + c];
+        copy_batched(G, [&](int c) { return src[static_cast<size_t>(r) * G + c]; },
+                     [&](int c, float2 v) { buf0[r * pn + pidx(c)] = v; });
     __syncthreads();
     const float2* res;
     if constexpr (kInplace) {
@@ -449,9 +467,18 @@ k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
     const int c0 = (tile - q * strips) * kColStrip;
     const int nc = min(kColStrip, cols - c0);
     const float2* src = in + static_cast<size_t>(q) * H * cols + c0;
-    for (int t = threadIdx.x; t < H * kColStrip; t += 64 * kColStrip) {
-        const int r = t / kColStrip, c = t % kColStrip;
-        if (c < nc) strip[c * kColLen + r] = src[static_cast<size_t>(r) * cols + c];
+    {
+        // Element (r, c) of the strip: thread t = (r mod 64) kColStrip + c loads rows r0 + 64 j,
+        // all 24 loads issued before the first LDS store.
+        constexpr int kLoads = H / 64;
+        const int c = threadIdx.x % kColStrip, r0 = threadIdx.x / kColStrip;
+        if (c < nc) {
+            float2 v[kLoads];
+#pragma unroll
+            for (int j = 0; j < kLoads; ++j) v[j] = src[static_cast<size_t>(r0 + 64 * j) * cols + c];
+#pragma unroll
+            for (int j = 0; j < kLoads; ++j) strip[c * kColLen + r0 + 64 * j] = v[j];
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -464,6 +491,7 @@ k_fft_cols_1536(const float2* __restrict__ in, float2* __restrict__ out, const f
     const int k_lo = max(0, (row_lo - parity + kSplit - 1) / kSplit);
     const int nk = min(H, (row_lo + rows - parity + kSplit - 1) / kSplit) - k_lo;
     float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
+#pragma unroll 8
     for (int t = threadIdx.x; t < nk * kColStrip; t += 64 * kColStrip) {
         const int k = k_lo + t / kColStrip, c = t % kColStrip;
         const int r = kSplit * k + parity - row_lo;
@@ -534,10 +562,15 @@ k_fft_cols(const float2* __restrict__ in, float2* __restrict__ out, FftStages st
     const int c0 = (tile - static_cast<int>(plane) * strips) * width;
     const int nc = min(width, cols - c0);
     const float2* src = in + plane * G * static_cast<size_t>(cols) + c0;
-    for (int t = threadIdx.x; t < G * nc; t += blockDim.x) {
-        const int r = t / nc, c = t - r * nc;
-        buf0[c * pn + pidx(r)] = src[static_cast<size_t>(r) * cols + c];
-    }
+    copy_batched(G * nc,
+                 [&](int t) {
+                     const int r = t / nc, c = t - r * nc;
+                     return src[static_cast<size_t>(r) * cols + c];
+                 },
+                 [&](int t, float2 v) {
+                     const int r = t / nc, c = t - r * nc;
+                     buf0[c * pn + pidx(r)] = v;
+                 });
     __syncthreads();
     const float2* res;
     if constexpr (kInplace) {

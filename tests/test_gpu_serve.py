@@ -29,10 +29,19 @@ def pool():
 
 
 @pytest.fixture(scope="module")
-def singles(gpu_ctx, pool):
+def ctx():
+    # A context of its own: the workspace of the session context depends on the tests before.
+    c = S.Context(0)
+    yield c
+    torch.cuda.synchronize()
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def singles(ctx, pool):
     refs = []
     for d, N, cell in pool:
-        img, _ = S.image_snapshot(gpu_ctx, d, N, cell, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
+        img, _ = S.image_snapshot(ctx, d, N, cell, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
         refs.append(img.cpu().numpy())
     torch.cuda.synchronize()
     return refs
@@ -47,11 +56,12 @@ def check_served(out, pool, singles):
                       central=DIRTY_CENTRAL)
 
 
-def test_served_images_match_single_calls(gpu_ctx, pool, singles):
+def test_served_images_match_single_calls(ctx, pool, singles):
     announce("service: loose budget, 400 req/s for 0.25 s")
-    svc = serve.SnapshotService(gpu_ctx, budget_bytes=8 << 30, max_batch=4, max_inflight=2,
+    svc = serve.SnapshotService(ctx, budget_bytes=1 << 40, max_batch=4, max_inflight=2,
                                 keep_images=True, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
-    svc.reserve([serve.Request(i, d, N, cell, 0.0) for i, (d, N, cell) in enumerate(pool)])
+    ws = svc.reserve([serve.Request(i, d, N, cell, 0.0) for i, (d, N, cell) in enumerate(pool)])
+    svc.budget = ws + (1 << 30)
     out = serve.run_poisson(svc, pool, rate=400.0, duration=0.25, seed=1, drain_limit=30.0)
     torch.cuda.synchronize()
     assert out["count"] > 20
@@ -60,9 +70,9 @@ def test_served_images_match_single_calls(gpu_ctx, pool, singles):
     check_served(out, pool, singles)
 
 
-def test_tight_budget_admits_two_at_a_time(gpu_ctx, pool, singles):
+def test_tight_budget_admits_two_at_a_time(ctx, pool, singles):
     announce("service: budget for two requests")
-    svc = serve.SnapshotService(gpu_ctx, budget_bytes=1 << 40, max_batch=4, max_inflight=2,
+    svc = serve.SnapshotService(ctx, budget_bytes=1 << 40, max_batch=4, max_inflight=2,
                                 keep_images=True, gridder=S.GRIDDER_MFMA, output=S.OUTPUT_DIRTY)
     ws = svc.reserve([serve.Request(i, d, N, cell, 0.0) for i, (d, N, cell) in enumerate(pool)])
     costs = [serve.request_bytes(serve.Request(0, d, N, cell, 0.0)) for d, N, cell in pool]
