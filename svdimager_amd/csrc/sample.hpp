@@ -53,9 +53,9 @@ struct SnapLoad {
     uint8_t f0, f3;
 };
 
-// kVis = false loads only what decides the footprint and whether the sample is gridded
-// (frequency, flags, weights); the visibility fields are then zero.
-template <bool kVis = true>
+// kVis = false leaves out the visibilities (the fields are then zero); kWeights = false leaves
+// out the weights (then 1), for callers whose decisions depend on flags and geometry only.
+template <bool kVis = true, bool kWeights = true>
 __device__ __forceinline__ SnapLoad load_snapshot_sample(const SampleInput& in, size_t idx, size_t chan) {
     SnapLoad s;
     s.nu = in.freq[chan];
@@ -69,20 +69,28 @@ __device__ __forceinline__ SnapLoad load_snapshot_sample(const SampleInput& in, 
     }
     s.f0 = in.f0[idx];
     s.f3 = in.f3[idx];
-    s.w0 = in.w0 ? in.w0[idx] : 1.0f;
-    s.w3 = in.w3 ? in.w3[idx] : 1.0f;
+    s.w0 = kWeights && in.w0 ? in.w0[idx] : 1.0f;
+    s.w3 = kWeights && in.w3 ? in.w3[idx] : 1.0f;
     return s;
 }
 
 // Pol collapse (SVDIntegratedImager.cu:588-617): wp = flag_p ? 0 : weight_p, V = (w0 V0 +
-// w3 V3) / (w0 + w3); a sample with no weight is exactly zero and is skipped.  Position
-// b = uvw nu / c in FP64, in cells; slab and Chebyshev argument from r3 (:1249-1254).
+// w3 V3) / (w0 + w3) if w0 + w3 > 0, else 0.  A sample with V = 0 adds nothing: it is skipped,
+// or with kFlagsOnly (whether a sample is kept then depends on its flags alone, so a pass that
+// does not load the weights keeps the same samples) it is only skipped when both polarisations
+// are flagged and is otherwise kept with V = 0.  Position b = uvw nu / c in FP64, in cells; slab
+// and Chebyshev argument from r3 (:1249-1254).
+template <bool kFlagsOnly = false>
 __device__ __forceinline__ bool finish_snapshot_sample(const GridParams& gp, double ru, double rv, double rw,
                                                        const SnapLoad& s, Footprint& f) {
     const float w0 = s.f0 ? 0.0f : s.w0;
     const float w3 = s.f3 ? 0.0f : s.w3;
     const float ws = w0 + w3;
-    if (!(ws > 0.0f)) return false;
+    if constexpr (kFlagsOnly) {
+        if (s.f0 && s.f3) return false;
+    } else {
+        if (!(ws > 0.0f)) return false;
+    }
     const double sc = s.nu * (1.0 / kSpeedOfLight);
     const double p1 = ru * sc * gp.scale_f64;
     const double p2 = rv * sc * gp.scale_f64;
@@ -104,9 +112,13 @@ __device__ __forceinline__ bool finish_snapshot_sample(const GridParams& gp, dou
     f.q2 = -f.db;
     f.local_slab = slab - (gp.archive ? 0 : gp.slab_start);
     f.s = static_cast<float>(fmin(1.0, fmax(-1.0, (r3 - centre) * static_cast<double>(gp.inv_half))));
-    const float scale = gp.inv_det / ws;
-    f.vw.x = (s.v0r * w0 + s.v3r * w3) * scale;
-    f.vw.y = (s.v0i * w0 + s.v3i * w3) * scale;
+    if (ws > 0.0f) {
+        const float scale = gp.inv_det / ws;
+        f.vw.x = (s.v0r * w0 + s.v3r * w3) * scale;
+        f.vw.y = (s.v0i * w0 + s.v3i * w3) * scale;
+    } else {
+        f.vw = make_float2(0.0f, 0.0f);
+    }
     return true;
 }
 

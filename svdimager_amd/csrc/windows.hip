@@ -106,7 +106,9 @@ struct WRecords {
 // threads that take every fourth channel (a wavefront reads 64 consecutive rows of one channel:
 // coalesced), loading four samples before working on the first.  Preprocessed input: 16384
 // consecutive samples, 16 per thread at a stride of the block size.  kVis = false (snapshot
-// input): the visibilities are not loaded and f.vw is zero (the bins do not depend on them).
+// input): neither visibilities nor weights are loaded and f.vw is zero; the bins depend on flags
+// and geometry only (samples are kept unless both polarisations are flagged, V = 0 when the
+// weights sum to zero), so both passes keep the same samples.
 template <bool kSnap, bool kVis, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
                                                 const WinGeom& wg, Fn&& fn) {
@@ -127,13 +129,13 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const size_t chan = c0 + u * kBinChanGroups;
-                if (chan < in.num_channels) ld[u] = load_snapshot_sample<kVis>(in, row + chan * in.num_rows, chan);
+                if (chan < in.num_channels) ld[u] = load_snapshot_sample<kVis, kVis>(in, row + chan * in.num_rows, chan);
             }
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) {
                 const size_t chan = c0 + u * kBinChanGroups;
                 Footprint f;
-                if (chan < in.num_channels && finish_snapshot_sample(gp, ru, rv, rw, ld[u], f)) bins_of(f);
+                if (chan < in.num_channels && finish_snapshot_sample<true>(gp, ru, rv, rw, ld[u], f)) bins_of(f);
             }
         }
         }

@@ -176,6 +176,38 @@ def test_large_grid_vs_oracle(gpu_ctx, small_snap, oracle_pre, gridder, N):
     assert_parity(img.cpu().numpy(), ref["signed"], f"N={N} G={(3 * N + 1) // 2}")
 
 
+def test_zero_weight_samples(gpu_ctx, small_snap):
+    """A sample whose unflagged polarisation has zero weight collapses to V = 0
+    (SVDIntegratedImager.cu:588-617, the else branch), NaN visibilities included.  The MFMA
+    window binning keeps such samples as records with V = 0 (its count pass reads flags, not
+    weights); the tile gridder skips them.  Both must give the image of the same snapshot with
+    those samples flagged in both polarisations."""
+    announce("zero-weight samples with NaN visibilities")
+    n = small_snap["num_rows"] * small_snap["num_channels"]
+    pick = np.random.default_rng(11).choice(n, n // 5, replace=False)
+    zw, ff = dict(small_snap), dict(small_snap)
+    for k in ("flag0", "flag3", "weight0", "vis0_re", "vis0_im", "vis3_re", "vis3_im"):
+        zw[k] = np.array(small_snap[k], copy=True)
+        ff[k] = np.array(small_snap[k], copy=True)
+    zw["flag0"][pick] = 0
+    zw["flag3"][pick] = 1
+    zw["weight0"][pick] = 0.0
+    for k in ("vis0_re", "vis0_im", "vis3_re", "vis3_im"):
+        zw[k][pick] = np.nan
+    ff["flag0"][pick] = 1
+    ff["flag3"][pick] = 1
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    d_zw, d_ff = S.DeviceSnapshot(zw, "cuda"), S.DeviceSnapshot(ff, "cuda")
+    ref, _ = S.image_snapshot(gpu_ctx, d_ff, N, cell, output=S.OUTPUT_SIGNED, gridder=S.GRIDDER_TILED)
+    ref = ref.cpu().numpy()
+    assert np.isfinite(ref).all() and np.abs(ref).max() > 0
+    for g in (S.GRIDDER_MFMA, S.GRIDDER_TILED):
+        img, _ = S.image_snapshot(gpu_ctx, d_zw, N, cell, output=S.OUTPUT_SIGNED, gridder=g)
+        a = img.cpu().numpy()
+        assert np.isfinite(a).all(), g
+        assert_parity(a, ref, f"zero weights, gridder {g}")
+
+
 def test_invalid_arguments(gpu_ctx, small_snap, dsnap):
     announce("invalid arguments")
     with pytest.raises(S.SvdiError):
