@@ -55,6 +55,19 @@ def test_create_without_gpu_returns_an_error(L):
     assert L.svdi_last_error()
 
 
+def test_null_context_is_an_error(L):
+    """Entry points that take a context reject a null one with a status and a message (no
+    device call is made)."""
+    n = ctypes.c_size_t(7)
+    for rc in (L.svdi_workspace_bytes(None, ctypes.byref(n)), L.svdi_trim(None),
+               L.svdi_set_profiling(None, 1), L.svdi_destroy(None)):
+        if rc == 0:                      # svdi_destroy(NULL) may be a no-op, as free(NULL)
+            continue
+        assert L.svdi_last_error()
+    assert L.svdi_workspace_bytes(None, ctypes.byref(n)) != 0 and n.value == 7
+    assert L.svdi_trim(None) != 0
+
+
 def test_fits_round_trip(tmp_path):
     from svdimager_amd import read_fits_image, write_fits_2d
     img = np.random.default_rng(1).normal(size=(12, 7)).astype(np.float32)
