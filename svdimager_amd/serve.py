@@ -178,6 +178,12 @@ class SnapshotService:
             r = self.waiting[0]
             c = request_bytes(r)
             if self.workspace + self.held + c > self.budget:
+                if self.held == 0:
+                    # Nothing admitted will free memory: the workspace has grown past what the
+                    # budget leaves for this request (a shape that reserve() did not see).
+                    raise RuntimeError(
+                        f"request {r.rid} ({c} bytes) cannot be admitted: the workspace is now "
+                        f"{self.workspace} bytes of the {self.budget}-byte budget")
                 break
             self.waiting.popleft()
             r.admitted = self.clock()

@@ -113,6 +113,23 @@ def test_oldest_key_first_and_inflight_cap():
     assert all(q.latency == q.done - q.arrival for q in svc.completed)
 
 
+def test_workspace_growth_past_the_budget_is_an_error():
+    ws = [1000]
+    clock = FakeClock()
+    svc = serve.SnapshotService(None, 1000 + int(1.5 * serve.request_bytes(req(0))), clock=clock,
+                                launcher=FakeDevice(clock), workspace_bytes=lambda: ws[0],
+                                max_inflight=1)
+    svc.submit(req(0))
+    svc.submit(req(1))
+    svc.poll()                                   # request 0 admitted and launched, 1 waits
+    assert [r.rid for r in svc.waiting] == [1]
+    ws[0] = svc.budget
+    svc.workspace = ws[0]
+    clock.t = 5.0
+    with pytest.raises(RuntimeError):
+        svc.poll()                               # request 0 retired; request 1 can never fit
+
+
 def test_request_over_budget_is_refused():
     svc, clock, dev = make_service(budget=1000 + 100)
     with pytest.raises(ValueError):
