@@ -12,7 +12,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -436,6 +438,18 @@ size_t stage_begin(svdi_context* ctx, int stage) {
 void stage_end(svdi_context* ctx, size_t token) {
     if (token == SIZE_MAX) return;
     SVDI_HIP(hipEventRecord(ctx->event_pool[token + 1], ctx->stream));
+}
+
+void allow_dynamic_lds(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, size_t> granted;
+    int device = 0;
+    SVDI_HIP(hipGetDevice(&device));
+    std::lock_guard<std::mutex> lk(mu);
+    size_t& have = granted[{device, kernel}];
+    if (bytes <= have) return;
+    SVDI_HIP(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    have = bytes;
 }
 
 // ----------------------------------------------------------------------------- Workspace

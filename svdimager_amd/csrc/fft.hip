@@ -696,10 +696,8 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
     const int threads = inplace ? kInplaceThreads : kThreads;
     auto rows_fn = inplace ? &k_fft_rows<true> : &k_fft_rows<false>;
     auto cols_fn = inplace ? &k_fft_cols<true> : &k_fft_cols<false>;
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(rows_fn),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(cols_fn),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    allow_dynamic_lds(reinterpret_cast<const void*>(rows_fn), lds);
+    allow_dynamic_lds(reinterpret_cast<const void*>(cols_fn), lds);
     if (G == 3072) {
         k_fft_rows_3072<<<static_cast<unsigned>(P * 1536), 256, 0, ctx->stream>>>(
             planes, row_out, tw, static_cast<int>(col_lo), static_cast<int>(cols));

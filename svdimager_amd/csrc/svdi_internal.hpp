@@ -171,6 +171,9 @@ namespace svdi {
 // Records a start event for `stage` now and returns a token; stage_end records its end.
 size_t stage_begin(svdi_context* ctx, int stage);
 void stage_end(svdi_context* ctx, size_t token);
+// Raise a kernel's dynamic-LDS limit on the current device to at least `bytes`; the runtime is
+// only called when the kernel needs more than it was last given there.
+void allow_dynamic_lds(const void* kernel, size_t bytes);
 }  // namespace svdi
 
 namespace svdi {

@@ -506,8 +506,7 @@ void launch_tile_kernel(svdi_context* ctx, const Item* items, uint32_t* totals, 
     static_assert(bytes <= 160 * 1024 - 64, "tile LDS exceeds a CU");
     if (tg.T != T || tg.KS != L::KS || gp.K > KMAX)
         throw std::logic_error("tile kernel instantiated for another tile geometry");
-    SVDI_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX, WAVES>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_tile_grid<kSnap, T, KMAX, WAVES>), bytes);
     const int per_cu = std::max<int>(1, static_cast<int>((160 * 1024) / (bytes + 64)));
     const unsigned grid = static_cast<unsigned>(std::min(per_cu, 2) * std::max(1, ctx->cu_count));
     k_tile_grid<kSnap, T, KMAX, WAVES><<<grid, WAVES * 64, bytes, ctx->stream>>>(items, totals, boff, rec,

@@ -882,11 +882,6 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     }
 }
 
-// Dynamic LDS above the default limit for a kernel.
-void allow_lds(const void* fn, size_t bytes) {
-    SVDI_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
-}
-
 }  // namespace
 
 const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
@@ -954,10 +949,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
-        allow_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
         k_win_count<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     } else {
-        allow_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
         k_win_count<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
     }
     SVDI_LAUNCH_CHECK();
@@ -979,11 +974,11 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     SVDI_LAUNCH_CHECK();
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot) {
-        allow_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
         k_win_scatter<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
                                                                          totals + 4);
     } else {
-        allow_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
         k_win_scatter<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
                                                                           totals + 4);
     }
@@ -991,7 +986,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
 
     const size_t tok = stage_begin(ctx, 5);
     constexpr size_t lds = sizeof(float) * kStageFloats * kWaves;
-    allow_lds(reinterpret_cast<const void*>(&k_win_grid), lds);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_grid), lds);
     const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
     k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
     SVDI_LAUNCH_CHECK();
