@@ -20,6 +20,7 @@ pixels plus the peak pixel of the signed, pre-reprojection image.
   python bench.py --copy-calibration MIB     (plain copy, the PMC byte counters' reference)
   python bench.py --serve 50,100,200,400 [--serve-seconds T] [--serve-pool P]
                                              (streaming form: latency under Poisson arrivals)
+  python bench.py --precision                (4096^2 form: error against the FP64 DFT)
 """
 from __future__ import annotations
 
@@ -253,6 +254,45 @@ def serve_main(args):
         "workspace_bytes": workspace, "results": results}), flush=True)
 
 
+def precision_main(args):
+    """The 4096^2 precision form: the SKA-Low array with 128 channels (16,744,448 samples),
+    cell 2e-5, G = 6144.  Error against the FP64 DFT as in the default run, and the single-call
+    time of each imager."""
+    import torch
+    import svdimager_amd as S
+    from svdimager_amd import synth
+
+    cfg = synth.SKALOW_4K
+    log(f"precision: generating {cfg.name} ({cfg.num_samples} samples) ...")
+    snap = synth.make_snapshot(cfg, device="cuda")
+    dsnap = S.DeviceSnapshot(snap, "cuda")
+    ctx = S.Context(0)
+    gridders = {"mfma": S.GRIDDER_MFMA, "plain-fp32": S.GRIDDER_TILED}
+    times = {}
+    plan = None
+    for name, g in gridders.items():
+        image = torch.empty((cfg.image_size, cfg.image_size), dtype=torch.float32, device="cuda")
+        for _ in range(2):
+            _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=g)
+        torch.cuda.synchronize()
+        times[name] = (time.perf_counter() - t0) / 3 * 1e3
+        log(f"precision[{name}]: {times[name]:.2f} ms per snapshot")
+    log("precision: accuracy against the FP64 DFT ...")
+    acc = accuracy(S, ctx, snap, cfg, gridders)
+    print(json.dumps({
+        "metric": "max / RMS error vs FP64 DFT (64 seeded pixels + peak), 4096^2 precision form",
+        "n_gpus": 1, "data": "synthetic",
+        "config": {"workload": cfg.name, "samples": cfg.num_samples, "image_size": cfg.image_size,
+                   "grid_size": cfg.grid_size, "cell": cfg.cell, "num_terms": plan["num_terms"],
+                   "num_slabs": plan["num_slabs"]},
+        "ms_per_snapshot": times, "accuracy_vs_fp64_dft": acc}), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -280,7 +320,14 @@ def main():
                     help="distinct device-resident snapshots the requests are drawn from")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0,
                     help="device memory budget of the service (workspace + admitted requests)")
+    ap.add_argument("--precision", action="store_true",
+                    help="only measure the 4096^2 precision form (512 stations, 128 channels, "
+                         "16.7 M samples): error of the MFMA and plain FP32 imagers against the "
+                         "FP64 DFT at 64 seeded pixels plus the peak, and their single-call times")
     args = ap.parse_args()
+    if args.precision:
+        precision_main(args)
+        return
     if args.serve:
         if args.gpus != 1:
             print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported",
