@@ -160,11 +160,12 @@ def test_odd_grid_sizes_vs_oracle(gpu_ctx, small_snap, oracle_pre, N, gridder):
 
 
 @pytest.mark.parametrize("gridder", [S.GRIDDER_TILED, S.GRIDDER_MFMA])
-@pytest.mark.parametrize("N", [1024, 2048])
+@pytest.mark.parametrize("N", [1024, 2048, 4096])
 def test_large_grid_vs_oracle(gpu_ctx, small_snap, oracle_pre, gridder, N):
-    """N = 1024 (G = 1536: the wavefront transform of 24 x 64 points) and N = 2048 (G = 3072:
+    """N = 1024 (G = 1536: the wavefront transform of 24 x 64 points), N = 2048 (G = 3072:
     one radix-2 step over two such transforms, by two wavefronts per row or column; bins split
-    into two LDS bands; a window layout of 192 x 192 windows)."""
+    into two LDS bands; a window layout of 192 x 192 windows) and N = 4096 (G = 6144: the general
+    Stockham kernels' longest transform; eight LDS bands)."""
     announce(f"large grid N={N} gridder={gridder}")
     cell = small_snap["cell"] * 128 / N
     t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
