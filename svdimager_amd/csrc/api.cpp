@@ -491,7 +491,9 @@ std::unique_ptr<svdi_context> create_context(int device) {
     SVDI_HIP(hipSetDevice(device));
     std::unique_ptr<svdi_context> c(new svdi_context());
     c->device = device;
-    SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    // A blocking stream: ordered with the null stream both ways, so a caller that works on the
+    // null stream (torch's default stream) needs no synchronisation of its own around a call.
+    SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault));
     c->stream = c->own_stream;
     SVDI_HIP(hipEventCreate(&c->ev_start));
     SVDI_HIP(hipEventCreate(&c->ev_stop));
