@@ -253,6 +253,21 @@ def grid_svdi(bin_, vre, vim, vin, N, cell, r3_min, slab_width, inv_half, K, sla
     return grid.view(np.complex64).reshape(nslabs * K, G, G)
 
 
+def grid_archive(bin_, vre, vim, vin, N, cell, r3_centre, inv_half, K) -> np.ndarray:
+    """Archive gridding (FIPk.cu:383-452): one slab centred at r3_centre, vis / |det| first.
+    Returns (K, G, G) complex64 planes."""
+    G = grid_size(N)
+    b = _c(bin_, np.float32).reshape(-1)
+    S = b.size // 3
+    vf = _c(vin, np.float32).reshape(-1)
+    inten = np.float32(abs(vf[0] * vf[4] - vf[1] * vf[3]))
+    grid = np.zeros(K * G * G * 2, np.float32)
+    lib().orc_grid_archive(b, _c(vre, np.float32), _c(vim, np.float32), S, float(inten),
+                           float(np.float32(cell) * np.float32(G)), G, float(r3_centre),
+                           float(inv_half), int(K), grid)
+    return grid.view(np.complex64).reshape(K, G, G)
+
+
 def dft64(bin64, vre, vim, vin64, N: int, cell: float, pixels=None, keep=None) -> np.ndarray:
     """FP64 direct DFT on the rotated frame (signed image, / (|det| S)).
 

@@ -320,3 +320,125 @@ def test_synthetic_generator_reproducible(small_snap):
     lengths = np.linalg.norm(snap["uvw"], axis=1)
     assert lengths.max() < 2000.0 + 50.0
     assert abs(snap["flag0"].reshape(16, -1)[0].mean() - 0.05) < 0.01
+
+
+# ------------------------------------------------------------------ archive FIpipe pipeline
+# The archive FIpipe (archive/FIPkernels.cu:679-911) differs from FIpipeDevice in the steps
+# pinned below: visibilities divided by |V00 V11 - V01 V10| before gridding (:781-786), one
+# slab centred on the r3 range (:336-338), its own K rule (pinned above) and its degree-based
+# SIN re-projection (pinned above against Calabretta & Greisen).
+
+def _rot_x(angle):
+    c, s = math.cos(angle), math.sin(angle)
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, -s], [0.0, s, c]])
+
+
+def _cheb(k, s):
+    return math.cos(k * math.acos(max(-1.0, min(1.0, s))))
+
+
+def test_archive_grid_single_sample_closed_form():
+    """One sample: plane k holds (v / |det|) T_k(s) ES((g1 - p1)/4) ES((g2 - p2)/4)
+    (-1)^(g1 + g2) on the footprint cells and zero elsewhere.  V00 V11 - V01 V10 = cos 2.5 < 0
+    here, so a missing |.| or a missing division changes every value."""
+    V = _rot_x(2.5)
+    det = abs(V[0, 0] * V[1, 1] - V[0, 1] * V[1, 0])
+    N, cell, K = 64, 1e-3, 5
+    G = O.grid_size(N)
+    b = np.array([[12.3, -40.7, 5.0]])
+    centre, half = 2.0, 6.0
+    vr, vi = 0.7, -0.2
+    planes = O.grid_archive(b, [vr], [vi], V, N, cell, centre, 1.0 / half, K)
+    p1, p2 = b[0, 0] * cell * G, b[0, 1] * cell * G
+    s = (b[0, 2] - centre) / half
+    ref = np.zeros((K, G, G), np.complex128)
+    for g1 in range(math.ceil(p1 - 4), math.floor(p1 + 4) + 1):
+        for g2 in range(math.ceil(p2 - 4), math.floor(p2 + 4) + 1):
+            w = float(es64((g1 - p1) / 4) * es64((g2 - p2) / 4)) * (-1) ** (g1 + g2)
+            for k in range(K):
+                ref[k, g1 + G // 2, g2 + G // 2] = complex(vr, vi) / det * _cheb(k, s) * w
+    assert np.abs(planes - ref).max() <= 2e-6 * np.abs(ref).max()
+    assert np.count_nonzero(np.abs(planes[0]) > 0) == 64
+
+
+@pytest.mark.parametrize("K", [4, 8])
+def test_archive_pipeline_matches_dft(small_snap, K):
+    """The archive restatement end to end (gridding, FFT, w-phase about the r3 centre, grid
+    correction, 1/S) against the FP64 direct DFT of the same Bin / Vin / vis, on the signed
+    image before |.| and re-projection.  The ES kernel's own aliasing error at the image edge
+    is 9.6e-5 of the peak (support 8, 1.5x oversampling)."""
+    p = O.preprocess_ref32(small_snap)
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    res = O.fipipe_archive_ref32(p["bin"], p["vin"], p["vis_re"], p["vis_im"], N, cell, num_terms=K)
+    dft = O.dft64(p["bin"].astype(np.float64), p["vis_re"], p["vis_im"],
+                  p["vin"].astype(np.float64), N, cell)
+    peak = np.abs(dft).max()
+    err = np.abs(res["signed"] - dft)
+    q = N // 4
+    assert res["plan"]["num_terms"] == K
+    assert err.max() / peak <= 1e-4
+    assert err[q:N - q, q:N - q].max() / peak <= 2e-5
+
+
+def _two_sample_archive(centre_shift=0.0):
+    """Two samples far out in r3 (w-phases of ~20 rad at the field edge) through the archive
+    pipeline; centre_shift moves the slab centre of both the gridding and the image pass (a
+    mistaken centre)."""
+    V = np.eye(3)
+    N, cell, K = 64, 4e-3, 12
+    G = O.grid_size(N)
+    b = np.array([[30.0, -20.0, 150.0], [-45.0, 10.0, 190.0]], np.float32)
+    vre = np.array([1.0, 0.5], np.float32)
+    vim = np.array([0.25, -0.75], np.float32)
+    dft = O.dft64(b.astype(np.float64), vre, vim, V, N, cell)
+    if centre_shift == 0.0:
+        return O.fipipe_archive_ref32(b, V.astype(np.float32), vre, vim, N, cell, num_terms=K), dft
+    centre = np.float32(170.0 + centre_shift)
+    half = np.float32(20.0)
+    planes = O.grid_archive(b, vre, vim, V, N, cell, centre, 1.0 / half, K)
+    planes = O.ifft2(planes)
+    conv = O.conv_corr(N, G, 1)
+    dirty = np.zeros(N * N, np.float32)
+    signed = np.zeros(N * N, np.float32)
+    O.lib().orc_image_archive(np.ascontiguousarray(planes.view(np.float32).reshape(-1)), G, N,
+                              float(cell), V.astype(np.float32).reshape(9), conv,
+                              float(np.float32(O.CONV_CORR_NORM)), 2, float(centre), float(half),
+                              K, dirty, signed.ctypes.data)
+    return {"signed": signed.reshape(N, N)}, dft
+
+
+def test_archive_single_slab_centred_on_the_r3_range():
+    """The slab is centred at (r3_min + r3_max) / 2 with half range (r3_max - r3_min) / 2: the
+    two-sample image matches its FP64 DFT (a sum of two cosines with w-phases), and a slab
+    centred 15 wavelengths off (s leaves [-1, 1] and is clamped) does not.  The bound is set by
+    the reference's FP32 t3 = sqrt(1 - t1^2 - t2^2) - 1, which cancels to ~6e-8 absolute: at
+    r3 = 190 wavelengths that is ~7e-5 rad of w-phase at every pixel."""
+    res, dft = _two_sample_archive()
+    N = dft.shape[0]
+    q = N // 4
+    peak = np.abs(dft).max()
+    err = np.abs(res["signed"] - dft) / peak
+    assert res["plan"]["r3_centre"] == pytest.approx(170.0) and res["plan"]["r3_half_range"] == pytest.approx(20.0)
+    assert err.max() <= 2e-4 and err[q:N - q, q:N - q].max() <= 1e-4
+    bad, _ = _two_sample_archive(centre_shift=15.0)
+    assert np.abs(bad["signed"] - dft).max() / peak > 1e-2
+
+
+def test_archive_dirty_conserves_flux_through_the_fp64_map(small_snap):
+    """The archive dirty image is |signed| splatted through its re-projection: its total equals
+    the sum of |DFT| over the pixels whose FP64 Calabretta-Greisen target lies inside the splat
+    window (|offset| < N/2 - 1 on both axes)."""
+    p = O.preprocess_ref32(small_snap)
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    res = O.fipipe_archive_ref32(p["bin"], p["vin"], p["vis_re"], p["vis_im"], N, cell, num_terms=8)
+    V = p["vin"].astype(np.float64)
+    dft = O.dft64(p["bin"].astype(np.float64), p["vis_re"], p["vis_im"], V, N, cell)
+    inside = np.zeros((N, N), bool)
+    for y in range(N):
+        for x in range(N):
+            a, b = _sin_reproject_f64(N, cell, V, x, y)
+            inside[y, x] = abs(a - N // 2) < N // 2 - 1 and abs(b - N // 2) < N // 2 - 1
+    valid = np.isfinite(dft)
+    ref = np.abs(dft[inside & valid]).sum()
+    assert float(res["dirty"].sum()) == pytest.approx(ref, rel=2e-4)
+    assert res["dirty"].min() >= 0.0
