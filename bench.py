@@ -99,18 +99,18 @@ def measured_traffic(kernel, workload):
             "source": f"profiles/traffic.json ({rec.get('summary', '')})"}
 
 
-def time_gridder(S, ctx, dsnap, cfg, image, gridder, steps, warmup, label):
+def time_gridder(S, ctx, dsnap, cfg, image, gridder, steps, warmup, label, **opts):
     """Warm up, then time `steps` snapshots; returns (seconds, mean stage ms, last plan)."""
     import torch
     for i in range(warmup):
-        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
+        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder, **opts)
         log(f"bench[{label}]: warmup {i + 1}/{warmup} done ({plan['elapsed_ms']:.3f} ms device)")
     ctx.set_profiling(True)
     stage_sum = {k: 0.0 for k in S.STAGES}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder)
+        _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=gridder, **opts)
         for k, v in ctx.stage_times().items():
             stage_sum[k] += v
     torch.cuda.synchronize()
@@ -120,9 +120,10 @@ def time_gridder(S, ctx, dsnap, cfg, image, gridder, steps, warmup, label):
     return elapsed, {k: v / steps for k, v in stage_sum.items()}, plan
 
 
-def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64):
+def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64, all_pixels=True):
     """Max / RMS error of the signed image against an FP64 direct DFT (torch, on the GPU).
 
+    `gridders` maps a name to the imaging options (gridder, mfma_precision) of one imager.
     Samples whose ES footprint is not wholly inside the grid are flagged for this comparison
     (the gridder clips them, the DFT cannot), on both sides; S in the 1/S scaling still counts
     every sample.  Pixels: 64 seeded ones plus the peak of the first gridder's image."""
@@ -159,8 +160,9 @@ def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64):
     det = abs(rot[0, 0] * rot[1, 1] - rot[0, 1] * rot[1, 0])
 
     images = {}
+    plan = None
     for name, g in gridders.items():
-        img, _ = S.image_snapshot(ctx, asnap, N, cell, output=S.OUTPUT_SIGNED, gridder=g)
+        img, plan = S.image_snapshot(ctx, asnap, N, cell, output=S.OUTPUT_SIGNED, **g)
         images[name] = img.double()
     first = next(iter(images.values()))
     peak_idx = int(torch.argmax(first.abs()))
@@ -188,7 +190,177 @@ def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64):
         vals = img.cpu().numpy()[pix[:, 1], pix[:, 0]]
         d = np.abs(vals - dft) / dft_peak
         out[name] = {"max_rel_err": float(d.max()), "rms_rel_err": float(np.sqrt(np.mean(d * d)))}
+    if all_pixels:
+        # Every pixel against the FP64 reference of the same algorithm on the same plan: the
+        # imagers' arithmetic error alone (the method's own error cancels).
+        t0 = time.perf_counter()
+        planes = ref64_grid(b, vre, vim, det, plan, np.float32(cell), G)
+        ref = ref64_image(planes, rot, plan, N, np.float32(cell), R * C)
+        del planes
+        torch.cuda.synchronize()
+        ref_peak = float(ref.abs().max())
+        rv = ref.cpu().numpy()[pix[:, 1], pix[:, 0]]
+        dr = np.abs(rv - dft) / dft_peak
+        all_px = {"reference": "FP64 gridding (bench.ref64_grid / ref64_image) on the imagers' plan",
+                  "seconds": time.perf_counter() - t0, "peak": ref_peak,
+                  "reference_vs_dft": {"max_rel_err": float(dr.max()),
+                                       "rms_rel_err": float(np.sqrt(np.mean(dr * dr)))}}
+        for name, img in images.items():
+            e = (img - ref).abs() / ref_peak
+            all_px[name] = {"max_rel_err": float(e.max()), "rms_rel_err": float(torch.sqrt((e * e).mean()))}
+        out["all_pixels_vs_fp64_gridder"] = all_px
     return out
+
+
+PRECISIONS = {"default": 0, "split3": 1, "splitc": 2, "single": 3}
+
+
+def imager_variants(S, headline=None):
+    """Imaging options of every imager whose accuracy is measured: the headline one first, then
+    each MFMA operand precision and the plain FP32 tile imager."""
+    out = {}
+    if headline is not None:
+        out[headline[0]] = headline[1]
+    for name in ("split3", "splitc", "single"):
+        key = f"mfma-{name}"
+        if key not in out:
+            out[key] = {"gridder": S.GRIDDER_MFMA, "mfma_precision": PRECISIONS[name]}
+    out.setdefault("plain-fp32", {"gridder": S.GRIDDER_TILED})
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# FP64 reference of the gridding algorithm (not the oracle): the same plan (slabs, K), the same
+# ES kernel, Chebyshev moments, inverse FFT and Jacobi-Anger recombination with the GL-28 grid
+# correction, every step in FP64 with torch, so that an imager's arithmetic error can be measured
+# on every pixel (the DFT measures the method's error too, and only at a few pixels).
+ES_BETA = 15.3704324328                                   # SVDIntegratedImager.cu:1235
+CONV_NORM = 2.4937047051153827                            # SVDIntegratedImager.cu:1512
+
+
+def es64(x):
+    """exp(beta (sqrt(1 - x^2) - 1)) for |x| <= 1, else 0 (SVDIntegratedImager.cu:884-888)."""
+    import torch
+    inside = x.abs() <= 1.0
+    r = torch.sqrt(torch.clamp(1.0 - x * x, min=0.0))
+    return torch.where(inside, torch.exp(ES_BETA * (r - 1.0)), torch.zeros_like(x))
+
+
+def ref64_grid(b, vre, vim, det, plan, cell, G, chunk=1 << 18):
+    """Planes (num_slabs K, G, G) complex128: every sample's ES footprint (cells ceil(p - 4) ..
+    floor(p + 4) on each axis, clipped to the grid) times T_k(s) (Re, Im) v / det, with the
+    checkerboard sign (-1)^(g1 + g2), at (g1 + G/2, g2 + G/2) of plane slab K + k
+    (SVDIntegratedImager.cu:1218-1293).  b: (S, 3) rotated baselines in wavelengths, FP64;
+    vre, vim: collapsed visibilities (S,), FP64, zero for dropped samples."""
+    import torch
+    dev = b.device
+    K, nsl = int(plan["num_terms"]), int(plan["num_slabs"])
+    r3_min, width, half = float(plan["r3_min"]), float(plan["slab_width"]), float(plan["slab_half_range"])
+    cell = float(cell)
+    gmin, gmax, off = -(G // 2), (G - 1) // 2, G // 2
+    # (slab, cell) rows of 2 K values (Re and Im of every term): one index per footprint cell.
+    grid = torch.zeros(nsl * G * G, 2 * K, dtype=torch.float64, device=dev)
+    j = torch.arange(9, device=dev, dtype=torch.float64)
+    keep = ((vre != 0) | (vim != 0)).nonzero().squeeze(1)
+    for c0 in range(0, keep.numel(), chunk):
+        sel = keep[c0:c0 + chunk]
+        bb = b[sel]
+        vr, vi = vre[sel] / det, vim[sel] / det
+        p1, p2, r3 = bb[:, 0] * (cell * G), bb[:, 1] * (cell * G), bb[:, 2]
+        slab = torch.clamp(torch.floor((r3 - r3_min) / width), 0, nsl - 1)
+        centre = r3_min + (slab + 0.5) * width
+        s = torch.clamp((r3 - centre) / half, -1.0, 1.0)
+        g1 = torch.ceil(p1 - 4.0)[:, None] + j                 # (n, 9)
+        g2 = torch.ceil(p2 - 4.0)[:, None] + j
+        w1 = es64((g1 - p1[:, None]) / 4.0) * ((g1 >= gmin) & (g1 <= gmax))
+        w2 = es64((g2 - p2[:, None]) / 4.0) * ((g2 >= gmin) & (g2 <= gmax))
+        gi1 = torch.clamp(g1, gmin, gmax).long()
+        gi2 = torch.clamp(g2, gmin, gmax).long()
+        sign = 1.0 - 2.0 * ((gi1[:, :, None] + gi2[:, None, :]) & 1).double()
+        w = (w1[:, :, None] * w2[:, None, :] * sign).reshape(-1, 81)
+        rows = (slab.long()[:, None, None] * (G * G) + (gi1 + off)[:, :, None] * G
+                + (gi2 + off)[:, None, :]).reshape(-1)
+        t_prev = t_cur = None
+        coef = torch.empty(sel.numel(), 2 * K, dtype=torch.float64, device=dev)
+        for k in range(K):
+            tk = torch.ones_like(s) if k == 0 else (s if k == 1 else 2.0 * s * t_cur - t_prev)
+            t_prev, t_cur = t_cur, tk
+            coef[:, 2 * k] = tk * vr
+            coef[:, 2 * k + 1] = tk * vi
+        vals = (w[:, :, None] * coef[:, None, :]).reshape(-1, 2 * K)
+        nz = (w != 0).reshape(-1)
+        grid.index_add_(0, rows[nz], vals[nz])
+    planes = torch.view_as_complex(grid.reshape(nsl, G * G, K, 2))
+    return planes.permute(0, 2, 1).reshape(nsl * K, G, G).contiguous()
+
+
+def bessel_j64(alpha, K, terms=48):
+    """J_0 .. J_{K-1} of each alpha (|alpha| <= 12) by the power series, FP64: (K,) + alpha.shape."""
+    import torch
+    if float(alpha.abs().max()) > 12.0:
+        raise ValueError("bessel_j64: |alpha| above 12 (the series would lose digits)")
+    h = alpha / 2.0
+    h2 = h * h
+    pk = torch.ones_like(alpha)                            # (alpha / 2)^k / k!
+    out = []
+    for k in range(K):
+        term = pk.clone()
+        acc = term.clone()
+        for m in range(1, terms):
+            term = term * (-h2) / (m * (m + k))
+            acc = acc + term
+        out.append(acc)
+        pk = pk * h / (k + 1)
+    return torch.stack(out)
+
+
+def conv_corr64(N, G, dev):
+    """Grid correction c(t), t = 0 .. N/2, from the 28-point Gauss-Legendre rule (positive
+    half) of the ES kernel (SVDIntegratedImager.cu:1204-1216), FP64."""
+    import numpy as np
+    import torch
+    x, w = np.polynomial.legendre.leggauss(28)
+    pos = x > 0
+    x, w = torch.from_numpy(x[pos]).to(dev), torch.from_numpy(w[pos]).to(dev)
+    t = torch.arange(N // 2 + 1, device=dev, dtype=torch.float64) / G
+    k = es64(x)
+    corr = (k[None, :] * w[None, :] * torch.cos(math.pi * t[:, None] * ES_SUPPORT * x[None, :])).sum(1)
+    return corr * ES_SUPPORT / CONV_NORM
+
+
+def ref64_image(planes, rot, plan, N, cell, n_samples):
+    """Signed, grid-corrected image (N, N) of FP64 planes (SVDIntegratedImager.cu:1295-1445,
+    before |.|): unnormalised inverse FFT, then per pixel sum over slabs and terms of
+    Re[(k ? 2 : 1) i^k J_k(2 pi t3 half) e^{2 pi i t3 centre} M], times (-1)^(x' + y'), divided
+    by c(|x'|) c(|y'|) norm^2 and by S.  rot: FP64 rotation rows (numpy 3 x 3)."""
+    import torch
+    dev = planes.device
+    G = planes.shape[-1]
+    if G % 2:
+        raise ValueError("ref64_image: even grid sizes only")
+    K, nsl = int(plan["num_terms"]), int(plan["num_slabs"])
+    r3_min, width, half = float(plan["r3_min"]), float(plan["slab_width"]), float(plan["slab_half_range"])
+    cell = float(cell)
+    F = torch.fft.ifft2(planes, norm="forward")
+    r = torch.arange(N, device=dev) - N // 2               # x' = x - N/2, y' = y - N/2
+    M = F[:, G // 2 + r[:, None], G // 2 + r[None, :]]     # (P, y, x)
+    t1 = (cell * r.double())[:, None]                      # rows (y)
+    t2 = (cell * r.double())[None, :]                      # columns (x)
+    ra = 1.0 - (t1 + rot[0, 2]) ** 2 - (t2 + rot[1, 2]) ** 2
+    valid = ra >= 0
+    t3 = -rot[2, 2] + math.copysign(1.0, rot[2, 2]) * torch.sqrt(torch.clamp(ra, min=0.0))
+    J = bessel_j64(2.0 * math.pi * t3 * half, K)
+    val = torch.zeros(N, N, dtype=torch.float64, device=dev)
+    for ls in range(nsl):
+        centre = r3_min + (ls + 0.5) * width
+        e = torch.exp(1j * (2.0 * math.pi * t3 * centre))
+        for k in range(K):
+            coef = (2.0 if k else 1.0) * (1j ** k) * J[k] * e
+            val += (coef * M[ls * K + k]).real
+    sign = 1.0 - 2.0 * ((r[:, None] + r[None, :]) & 1).double()
+    conv = conv_corr64(N, G, dev)
+    val = val * sign / (conv[r.abs()][:, None] * conv[r.abs()][None, :] * CONV_NORM ** 2) / n_samples
+    return torch.where(valid, val, torch.zeros_like(val))
 
 
 def serve_main(args):
@@ -267,17 +439,17 @@ def precision_main(args):
     snap = synth.make_snapshot(cfg, device="cuda")
     dsnap = S.DeviceSnapshot(snap, "cuda")
     ctx = S.Context(0)
-    gridders = {"mfma": S.GRIDDER_MFMA, "plain-fp32": S.GRIDDER_TILED}
+    gridders = imager_variants(S)
     times = {}
     plan = None
     for name, g in gridders.items():
         image = torch.empty((cfg.image_size, cfg.image_size), dtype=torch.float32, device="cuda")
         for _ in range(2):
-            _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=g)
+            _, plan = S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, **g)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(3):
-            S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, gridder=g)
+            S.image_snapshot(ctx, dsnap, cfg.image_size, cfg.cell, image=image, **g)
         torch.cuda.synchronize()
         times[name] = (time.perf_counter() - t0) / 3 * 1e3
         log(f"precision[{name}]: {times[name]:.2f} ms per snapshot")
@@ -299,6 +471,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--gridder", default="mfma", choices=["mfma", "tiled", "atomic", "auto"])
+    ap.add_argument("--mfma-precision", default="default", choices=sorted(PRECISIONS),
+                    help="operand precision of the MFMA gridder (default: the library's choice)")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip timing the plain FP32 tiled imager")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -370,8 +544,9 @@ def main():
 
     ctx = S.Context(0)
     image = torch.empty((cfg.image_size, cfg.image_size), dtype=torch.float32, device="cuda")
+    gopts = {"mfma_precision": PRECISIONS[args.mfma_precision]}
     elapsed, stages, plan = time_gridder(S, ctx, dsnap, cfg, image, gridder, args.steps,
-                                         args.warmup, args.gridder)
+                                         args.warmup, args.gridder, **gopts)
     log(f"bench: plan {plan}")
     single_snaps_per_s = args.steps / elapsed
     ms_per_step = elapsed * 1e3 / args.steps
@@ -388,7 +563,7 @@ def main():
         log(f"bench: {batch} snapshots resident")
         images = [torch.empty_like(image) for _ in range(batch)]
         for i in range(args.warmup):
-            S.image_snapshot_batch(ctx, dsnaps, cfg.image_size, cfg.cell, images=images, gridder=gridder)
+            S.image_snapshot_batch(ctx, dsnaps, cfg.image_size, cfg.cell, images=images, gridder=gridder, **gopts)
             log(f"bench[batch {batch}]: warmup {i + 1}/{args.warmup} done")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -396,7 +571,7 @@ def main():
         # overlaps the previous call's last snapshots; the synchronize below waits for all.
         for _ in range(args.steps):
             S.image_snapshot_batch(ctx, dsnaps, cfg.image_size, cfg.cell, images=images, gridder=gridder,
-                                   wait=False)
+                                   wait=False, **gopts)
         torch.cuda.synchronize()
         el_b = time.perf_counter() - t0
         log(f"bench[batch {batch}]: {args.steps} steps in {el_b:.3f} s")
@@ -460,9 +635,7 @@ def main():
         out["speedup_single_vs_plain_fp32"] = single_snaps_per_s / (args.steps / el2)
     if not args.no_accuracy:
         log("bench: accuracy against the FP64 DFT ...")
-        gl = {args.gridder: gridder}
-        if not args.no_compare and args.gridder != "tiled":
-            gl["plain-fp32"] = S.GRIDDER_TILED
+        gl = imager_variants(S, (args.gridder, {"gridder": gridder, **gopts}))
         out["accuracy_vs_fp64_dft"] = accuracy(S, ctx, snap, cfg, gl)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline ...")

@@ -58,7 +58,7 @@ typedef struct svdi_snapshot {
 } svdi_snapshot;
 
 typedef enum svdi_gridder {
-    SVDI_GRIDDER_AUTO = 0,      /* fastest available */
+    SVDI_GRIDDER_AUTO = 0,      /* MFMA, or TILED for grids and sample counts MFMA does not take */
     SVDI_GRIDDER_ATOMIC = 1,    /* one thread per sample, device atomics (the reference's scheme) */
     SVDI_GRIDDER_TILED = 2,     /* samples binned into LDS-resident grid tiles, FP32 */
     SVDI_GRIDDER_MFMA = 3       /* 16 x 16 output windows held in MFMA accumulators, f16 hi/lo
@@ -75,6 +75,16 @@ typedef enum svdi_semantics {
     SVDI_SEMANTICS_ARCHIVE = 1      /* archive/FIPkernels.cu: one slab, smallest K in 1..32 */
 } svdi_semantics;
 
+/* Operand precision of the MFMA gridder's window contraction D += C B (C: the visibility times
+ * the Chebyshev term, B: the product of the two ES factors), accumulated in FP32. */
+typedef enum svdi_mfma_precision {
+    SVDI_PRECISION_DEFAULT = 0,     /* the library's choice: SPLIT3 */
+    SVDI_PRECISION_SPLIT3 = 1,      /* C and both ES factors in f16 hi + lo, B's rounding kept:
+                                       3 MFMAs per window row, ~22 significant bits per product */
+    SVDI_PRECISION_SPLITC = 2,      /* C in f16 hi + lo, B a single f16: 2 MFMAs per row */
+    SVDI_PRECISION_SINGLE = 3       /* C and B single f16: 1 MFMA per row */
+} svdi_mfma_precision;
+
 typedef struct svdi_options {
     int gridder;            /* svdi_gridder */
     int output;             /* svdi_output */
@@ -83,6 +93,7 @@ typedef struct svdi_options {
     float target_alpha;     /* <= 0: SVD_CHEB_TARGET_ALPHA from the environment, else 10 */
     size_t max_batch_slabs; /* 0: SVDIMAGER_MAX_BATCH_SLABS from the environment, else 25% of free HBM */
     int verbose;            /* 1: print the reference's Chebyshev report and "Time elapsed" to stdout */
+    int mfma_precision;     /* an svdi_mfma_precision value; the MFMA gridder only */
 } svdi_options;
 
 /* Planning results of one imaging call. */
@@ -99,8 +110,9 @@ typedef struct svdi_plan_info {
     size_t batch_slabs;
     size_t grid_size;
     size_t num_planes;      /* slabs x K */
-    size_t records;         /* (sample, tile) records gridded by the tiled gridder */
+    size_t records;         /* records gridded by the tiled or MFMA gridder (last slab batch) */
     float elapsed_ms;       /* device time from planning to the finished image */
+    int gridder;            /* svdi_gridder that ran (AUTO resolved) */
 } svdi_plan_info;
 
 int svdi_version(void);

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG, "libsvdimager.so")
 
 SVDI_OK = 0
 GRIDDER_AUTO, GRIDDER_ATOMIC, GRIDDER_TILED, GRIDDER_MFMA = 0, 1, 2, 3
+PRECISION_DEFAULT, PRECISION_SPLIT3, PRECISION_SPLITC, PRECISION_SINGLE = 0, 1, 2, 3
 OUTPUT_DIRTY, OUTPUT_SIGNED = 0, 1
 SEMANTICS_INTEGRATED, SEMANTICS_ARCHIVE = 0, 1
 
@@ -56,6 +57,7 @@ class Options(ctypes.Structure):
         ("gridder", ctypes.c_int), ("output", ctypes.c_int), ("semantics", ctypes.c_int),
         ("num_terms", ctypes.c_int), ("target_alpha", ctypes.c_float),
         ("max_batch_slabs", ctypes.c_size_t), ("verbose", ctypes.c_int),
+        ("mfma_precision", ctypes.c_int),
     ]
 
 
@@ -69,7 +71,7 @@ class PlanInfo(ctypes.Structure):
         ("term_error", ctypes.c_float), ("bucket_errors", ctypes.c_float * 4),
         ("batch_slabs", ctypes.c_size_t), ("grid_size", ctypes.c_size_t),
         ("num_planes", ctypes.c_size_t), ("records", ctypes.c_size_t),
-        ("elapsed_ms", ctypes.c_float),
+        ("elapsed_ms", ctypes.c_float), ("gridder", ctypes.c_int),
     ]
 
     def as_dict(self):

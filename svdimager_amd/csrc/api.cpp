@@ -151,6 +151,7 @@ struct CallState {
     float cell = 0.0f;
     size_t records = 0;
     const uint32_t* tile_totals = nullptr;
+    int gridder = SVDI_GRIDDER_AUTO;                       // resolved by image_enqueue
 };
 
 // Phase 1 of an imaging call: validate, enqueue the planning kernels and the read-back of the
@@ -162,6 +163,8 @@ void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     if (!(cell > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
     if (in.num_samples == 0) throw ArgError("Measurement set is empty");
     if (o.num_terms < 0 || o.num_terms > kMaxTerms) throw ArgError("num_terms must be in 0..32");
+    if (o.mfma_precision < SVDI_PRECISION_DEFAULT || o.mfma_precision > SVDI_PRECISION_SINGLE)
+        throw ArgError("unknown mfma_precision");
     if (N > 46340) throw ArgError("Image_Size too large");
     if (o.gridder != SVDI_GRIDDER_AUTO && o.gridder != SVDI_GRIDDER_ATOMIC && o.gridder != SVDI_GRIDDER_TILED &&
         o.gridder != SVDI_GRIDDER_MFMA)
@@ -267,6 +270,9 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     gp.r3_centre = hp.dev.r3_centre;
     gp.scale_f32 = cell * static_cast<float>(G);
     gp.scale_f64 = static_cast<double>(cell) * static_cast<double>(G);
+    gp.prec = o.mfma_precision == SVDI_PRECISION_SPLITC   ? 1
+              : o.mfma_precision == SVDI_PRECISION_SINGLE ? 2
+                                                          : 0;
     const float* v = ctx->pinned->vin;
     const float det = std::fabs(v[0] * v[4] - v[1] * v[3]);
     gp.det = det;
@@ -290,7 +296,13 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     ip.signed_out = o.output == SVDI_OUTPUT_SIGNED;
     const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
 
-    const int gridder = o.gridder == SVDI_GRIDDER_AUTO ? SVDI_GRIDDER_TILED : o.gridder;
+    // AUTO: the MFMA window gridder, or the FP32 tile gridder for the shapes it does not take
+    // (judged at the largest slab batch).
+    GridParams probe = gp;
+    probe.nslabs = static_cast<int>(hp.batch_slabs);
+    const int gridder = o.gridder != SVDI_GRIDDER_AUTO ? o.gridder
+                        : grid_windows_supported(in, probe) ? SVDI_GRIDDER_MFMA : SVDI_GRIDDER_TILED;
+    cs.gridder = gridder;
     for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
         const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
         gp.slab_start = static_cast<int>(start);
@@ -315,7 +327,30 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
         launch_image(ctx, colfft, vin_dev, conv, ip, image);
         stage_end(ctx, tok);
     }
+}
+
+// End of a call's work on `ctx`'s stream: the gridder totals (when `totals` is set) are copied
+// into page-locked memory; with `read_error` the device error word is too, and is then cleared
+// for the next call (without it the word keeps its bits until a later call reads it); ev_stop
+// is recorded.  No null-stream operation is involved, so other contexts' work and the caller's
+// streams are not waited for.
+void image_finish_enqueue(svdi_context* ctx, const uint32_t* totals, bool read_error) {
+    if (totals != nullptr)
+        SVDI_HIP(hipMemcpyAsync(ctx->pinned->totals, totals, sizeof ctx->pinned->totals, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    if (read_error) {
+        SVDI_HIP(hipMemcpyAsync(&ctx->pinned->error_word, ctx->dev_err, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        launch_fill(ctx, reinterpret_cast<float*>(ctx->dev_err), 1, 0.0f);
+    }
     SVDI_HIP(hipEventRecord(ctx->ev_stop, ctx->stream));
+}
+
+// After ev_stop has completed: a binning inconsistency detected on the device becomes an error.
+void check_error_word(const svdi_context* ctx) {
+    if (ctx->pinned->error_word & kErrBinMismatch)
+        throw std::runtime_error("gridder binning mismatch: a block's scatter pass emitted other records than "
+                                 "its count pass counted (the inputs changed during the call?)");
 }
 
 void fill_info(const CallState& cs, float ms, svdi_plan_info* info) {
@@ -339,19 +374,17 @@ void fill_info(const CallState& cs, float ms, svdi_plan_info* info) {
     info->num_planes = static_cast<size_t>(hp.dev.num_slabs) * static_cast<size_t>(hp.num_terms);
     info->records = cs.records;
     info->elapsed_ms = ms;
+    info->gridder = cs.gridder;
 }
 
 // Phase 3: wait for the call's work, then report.
 void image_complete(svdi_context* ctx, const svdi_options& o, CallState& cs, svdi_plan_info* info) {
     SVDI_HIP(hipEventSynchronize(ctx->ev_stop));
     finish_call(ctx);
+    check_error_word(ctx);
     float ms = 0.0f;
     SVDI_HIP(hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
-    if (cs.tile_totals != nullptr) {
-        uint32_t tot[2] = {0, 0};
-        SVDI_HIP(hipMemcpy(tot, cs.tile_totals, sizeof tot, hipMemcpyDeviceToHost));
-        cs.records = tot[0];                               // last slab batch
-    }
+    if (cs.tile_totals != nullptr) cs.records = ctx->pinned->totals[0];   // last slab batch
     if (o.verbose) std::cout << "Time elapsed: " << ms << " ms" << std::endl;
     fill_info(cs, ms, info);
 }
@@ -363,6 +396,7 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     image_begin(ctx, in, N, cell, o, cs);
     SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
     image_enqueue(ctx, in, o, cs, image);
+    image_finish_enqueue(ctx, cs.tile_totals, true);
     image_complete(ctx, o, cs, info);
 }
 
@@ -499,6 +533,10 @@ std::unique_ptr<svdi_context> create_context(int device) {
     SVDI_HIP(hipEventCreate(&c->ev_stop));
     SVDI_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
     SVDI_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->pinned), sizeof(PinnedPlan), hipHostMallocDefault));
+    std::memset(c->pinned, 0, sizeof(PinnedPlan));
+    SVDI_HIP(hipMalloc(reinterpret_cast<void**>(&c->dev_err), 256));
+    SVDI_HIP(hipMemsetAsync(c->dev_err, 0, 256, c->own_stream));
+    SVDI_HIP(hipStreamSynchronize(c->own_stream));
     hipDeviceProp_t prop;
     SVDI_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
@@ -547,11 +585,16 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
         }
         for (size_t j = 0; j < L; ++j) {
             svdi_context* lane = ctx->lanes[j].get();
-            SVDI_HIP(hipEventRecord(lane->ev_stop, lane->stream));
+            image_finish_enqueue(lane, nullptr, wait);
             SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_stop, 0));
         }
+        // The waiting form reports a binning inconsistency of this call (or of an earlier
+        // asynchronous call on the same lanes, whose error words are cleared only when read).
         if (wait)
-            for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamSynchronize(ctx->lanes[j]->stream));
+            for (size_t j = 0; j < L; ++j) {
+                SVDI_HIP(hipEventSynchronize(ctx->lanes[j]->ev_stop));
+                check_error_word(ctx->lanes[j].get());
+            }
         if (infos)
             for (size_t i = 0; i < count; ++i) fill_info(cs[i], 0.0f, &infos[i]);
     });
@@ -566,6 +609,7 @@ svdi_context::~svdi_context() {
     if (own_stream && own_stream != stream) (void)hipStreamSynchronize(own_stream);
     ws.release();
     if (pinned) (void)hipHostFree(pinned);
+    if (dev_err) (void)hipFree(dev_err);
     if (ev_start) (void)hipEventDestroy(ev_start);
     if (ev_stop) (void)hipEventDestroy(ev_stop);
     if (ev_plan) (void)hipEventDestroy(ev_plan);

@@ -44,12 +44,16 @@ int main(int argc, char* argv[]) {
     long imasize[2];
     float* visreal = read_fits_image(argv[1], imasize);
     if (!visreal) return 1;
+    const long n_visreal = imasize[0] * imasize[1];
     float* visimag = read_fits_image(argv[2], imasize);
     if (!visimag) return 1;
+    const long n_visimag = imasize[0] * imasize[1];
     float* bin = read_fits_image(argv[3], imasize);
     if (!bin) return 1;
+    const long n_bin = imasize[0] * imasize[1];
     float* vin = read_fits_image(argv[4], imasize);
     if (!vin) return 1;
+    const long n_vin = imasize[0] * imasize[1];
     size_t image_size, num_baselines;
     float cell_size;
     try {
@@ -60,9 +64,21 @@ int main(int argc, char* argv[]) {
         std::fprintf(stderr, "Invalid Image_Size / Number_of_Rows / Cell_Size\n");
         return 2;
     }
+    // FIpipe reads Number_of_Rows samples of Visreal / Visimag, 3 x Number_of_Rows of B and 9 of V
+    // (the reference copies them without checking, archive/pipe_interface.cpp:176-178).
+    if (num_baselines > static_cast<size_t>(n_visreal) || num_baselines > static_cast<size_t>(n_visimag) ||
+        3 * num_baselines > static_cast<size_t>(n_bin) || n_vin < 9) {
+        std::fprintf(stderr,
+                     "Number_of_Rows %zu does not fit the inputs (Visreal %ld, Visimag %ld, B %ld, V %ld values)\n",
+                     num_baselines, n_visreal, n_visimag, n_bin, n_vin);
+        return 1;
+    }
     float* dirty = static_cast<float*>(std::malloc(image_size * image_size * sizeof(float)));
     if (!dirty) return 1;
-    FIpipe(visreal, visimag, bin, vin, dirty, num_baselines, image_size, cell_size);
+    if (FIpipe(visreal, visimag, bin, vin, dirty, num_baselines, image_size, cell_size) != 0) {
+        std::free(dirty);
+        return 1;                                          // FIpipe has printed the reason
+    }
     const int status = svdi_fits_write(argv[8], dirty, static_cast<long>(image_size), static_cast<long>(image_size));
     if (status != SVDI_OK) {
         std::fprintf(stderr, "Error writing FITS image\n");

@@ -109,6 +109,7 @@ struct GridParams {
     float inv_det;          // 1 / |det| (vis weighting)
     float det;
     double inv_slab_width;  // 1 / slab_width
+    int prec;               // MFMA gridder operand precision: 0 split3, 1 split c, 2 single
 };
 
 // Grown-on-demand device buffers owned by a context.
@@ -132,11 +133,17 @@ class Workspace {
 }  // namespace svdi
 
 namespace svdi {
-// What the host reads back between planning and gridding, in one page-locked block.
+// What the host reads back, in one page-locked block: the plan and rotation rows between
+// planning and gridding, the gridder's totals and the consistency word at the end of a call.
 struct PinnedPlan {
     DevPlan plan;
     float vin[9];           // the rotation rows in FP32 (|det| of the first two rows is the vis scale)
+    uint32_t totals[8];     // the last slab batch's gridder totals ([0] = records)
+    uint32_t error_word;    // copy of the context's device error word (nonzero: binning mismatch)
 };
+
+// Bits of the device error word that the binning kernels set.
+constexpr uint32_t kErrBinMismatch = 1u;   // a block's scatter emitted other than it counted
 }  // namespace svdi
 
 struct svdi_context {
@@ -146,6 +153,8 @@ struct svdi_context {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr, ev_plan = nullptr;
     svdi::Workspace ws;
     svdi::PinnedPlan* pinned = nullptr;      // page-locked read-back of the plan
+    // Device word the binning kernels OR kErr* bits into; cleared after each read-back.
+    uint32_t* dev_err = nullptr;
     // Batched calls: child contexts, each with its own stream and workspace, that image
     // snapshots concurrently.
     std::vector<std::unique_ptr<svdi_context>> lanes;
@@ -199,6 +208,9 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
                                   float2* planes);
 const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                                     float2* planes);
+// Whether the window gridder takes this grid and sample count (else launch_grid_windows throws
+// UnsupportedError).
+bool grid_windows_supported(const SampleInput& in, const GridParams& gp);
 
 // fft.hip
 void fft_prepare(svdi_context* ctx, size_t G);

@@ -49,16 +49,36 @@ constexpr int kBinChanGroups = kBinThreads / kBinRowsPerSet;
 constexpr int kBinPerThread = 16;        // samples per thread, preprocessed input
 constexpr int kLdsBins = 36864;          // largest block-local histogram (144 KB of LDS)
 constexpr size_t kMaxBlockHistBytes = size_t(1) << 31;   // per-block histograms of all bands
-constexpr int kWaves = 12;               // wavefronts per workgroup of the window kernel (one per CU)
 constexpr int kChunk = 32;               // samples per MFMA reduction step
 constexpr int kStride = 36;              // floats per c staging row (32 samples + pad)
 constexpr int kStrideH = 40;             // halves per ES staging row (32 samples + pad, 16-byte rows)
 constexpr int kPad = 8;                  // ES staging rows beyond each window edge (-8 .. 23)
 constexpr int kStageRows = kWin + 2 * kPad;
 constexpr int kPlaneHalves = kStageRows * kStrideH;               // one ES plane (hi or lo)
-// Per wavefront: ku hi / lo and kv hi / lo planes (f16), then c[16] (f32).
-constexpr int kStageFloats = (4 * kPlaneHalves) / 2 + kWin * kStride;
 constexpr int kEsScaleLog2Sq = 14;      // ES factors scaled by 2^7 per axis (in the polynomial table)
+
+// Operand precision of the window contraction D_a += C B_a (C = c, B_a = ku_a kv).
+enum WinPrec : int {
+    kPrecSplit3 = 0,   // c and both ES factors in f16 hi + lo; B_a's product error kept:
+                       // hi*hi + lo*hi + hi*lo, ~22 significant bits per product, 3 MFMAs per row
+    kPrecSplitC = 1,   // c in f16 hi + lo, B_a = f16(f16(ku) f16(kv)): 2 MFMAs per row
+    kPrecSingle = 2,   // c and B_a single f16: 1 MFMA per row
+};
+template <int P>
+struct WinTraits {
+    static constexpr bool kEsLo = P == kPrecSplit3;       // ES factors staged as hi + lo planes
+    static constexpr bool kCLo = P != kPrecSingle;        // c split into hi + lo
+    static constexpr int kPlanes = kEsLo ? 4 : 2;
+    // Per wavefront: the ES planes (ku hi [lo], kv hi [lo], f16), then c[16] (f32).
+    static constexpr int kStageFloats = (kPlanes * kPlaneHalves) / 2 + kWin * kStride;
+    // Wavefronts per workgroup (one workgroup per CU): as many as the staging (and, at four per
+    // SIMD, 128 registers) allow.
+    static constexpr int kWaves = kEsLo ? 12 : 16;
+    // Accumulator slots of the workgroup reduction (within the staging area).
+    static constexpr int kSlots = kEsLo ? 6 : 4;
+    static_assert(kWaves % kSlots == 0, "reduction rounds");
+    static_assert(kSlots * kWin * 64 * 4 <= kStageFloats * kWaves, "reduction slots exceed the staging area");
+};
 // ES(1) 2^7, the ninth footprint cell of an integer position, as f16 hi + lo.
 constexpr _Float16 kEsPolyEdgeHi = static_cast<_Float16>(kEsPolyEdge);
 constexpr _Float16 kEsPolyEdgeLo = static_cast<_Float16>(kEsPolyEdge - static_cast<float>(kEsPolyEdgeHi));
@@ -67,6 +87,7 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx8 __attribute__((ext_vector_type(8)));
 
 struct WinGeom {
     int nw;          // windows per grid axis
@@ -164,6 +185,17 @@ __device__ __forceinline__ void block_max_to(float m, uint32_t* vmax_bits) {
     }
 }
 
+// Sum of v over the block's kBinThreads threads, returned to every thread (once per kernel).
+__device__ __forceinline__ uint32_t block_sum(uint32_t v) {
+    __shared__ uint32_t s_part[kBinThreads / 64];
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int w = 0; w < kBinThreads / 64; ++w) t += s_part[w];
+    return t;
+}
+
 // Bins [lo, lo + n) of LDS band blockIdx.y.
 __device__ __forceinline__ void lds_band(const WinGeom& wg, int& lo, int& n) {
     lo = static_cast<int>(blockIdx.y) * wg.band_bins;
@@ -171,12 +203,12 @@ __device__ __forceinline__ void lds_band(const WinGeom& wg, int& lo, int& n) {
 }
 
 // Bin counts: per (block, band) in an LDS histogram (dynamic LDS, band_bins words) written to
-// blockhist[block * nbins + bin] for the band's bins (without LDS histograms the counts go
-// straight to counts[bin]).
+// blockhist[block * nbins + bin] for the band's bins, and the (block, band)'s record total to
+// blocktot[band * blocks + block] (without LDS histograms the counts go straight to counts[bin]).
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
-            uint32_t* __restrict__ blockhist) {
+            uint32_t* __restrict__ blockhist, uint32_t* __restrict__ blocktot) {
     extern __shared__ uint32_t hist[];
     if (!wg.use_lds) {
         for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
@@ -188,11 +220,16 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     lds_band(wg, lo, n);
     for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = 0;
     __syncthreads();
+    uint32_t cnt = 0;
     for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
         const uint32_t b = static_cast<uint32_t>(bin - lo);
-        if (b < static_cast<uint32_t>(n)) atomicAdd(&hist[b], 1u);
+        if (b < static_cast<uint32_t>(n)) {
+            atomicAdd(&hist[b], 1u);
+            ++cnt;
+        }
     });
-    __syncthreads();
+    const uint32_t tot = block_sum(cnt);          // (its barrier also ends the histogram's atomics)
+    if (threadIdx.x == 0) blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x] = tot;
     uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
     for (int b = threadIdx.x; b < n; b += kBinThreads) row[b] = hist[b];
 }
@@ -265,11 +302,15 @@ k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, co
 // Records of this block's samples, in one pass per band: with LDS histograms the block's first
 // slot of each of the band's bins comes from blockhist (the offsets of k_bin_block_offsets) and
 // LDS cursors hand out the slots; otherwise every record takes a global cursor slot.
-// `limit` (the record buffer's capacity) only bounds the writes.
+// `limit` (the record buffer's capacity) only bounds the writes.  With LDS histograms the
+// (block, band)'s emitted records are checked against its count-pass total: a difference (the
+// inputs changed between the passes) would leave slots unwritten, so it sets kErrBinMismatch in
+// *err and the call fails instead of gridding stale records.
 template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ blockhist,
-              uint32_t* __restrict__ cursor, WRecords rec, uint32_t limit, uint32_t* __restrict__ vmax_bits) {
+              const uint32_t* __restrict__ blocktot, uint32_t* __restrict__ cursor, WRecords rec,
+              uint32_t limit, uint32_t* __restrict__ vmax_bits, uint32_t* __restrict__ err) {
     extern __shared__ uint32_t hist[];
     const int off = gp.G / 2;
     float m = 0.0f;
@@ -296,10 +337,17 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
     const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
     for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = row[b];
     __syncthreads();
+    uint32_t cnt = 0;
     for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
         const uint32_t b = static_cast<uint32_t>(bin - lo);
-        if (b < static_cast<uint32_t>(n)) emit(atomicAdd(&hist[b], 1u), f, h, wb);
+        if (b < static_cast<uint32_t>(n)) {
+            emit(atomicAdd(&hist[b], 1u), f, h, wb);
+            ++cnt;
+        }
     });
+    const uint32_t tot = block_sum(cnt);
+    if (threadIdx.x == 0 && tot != blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x])
+        atomicOr(err, kErrBinMismatch);
     block_max_to(m, vmax_bits);
 }
 
@@ -509,29 +557,37 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One window row a: B_a[r][b] = ku_r[a] kv_r[b] from the factors split into f16 hi + lo
-// (ku = uh + ul, kv = vh + vl), in packed f16: P = f16(uh vh), E = uh vh - P exactly (an fma of
-// 11-bit significands), lo = f16(uh vl + ul vh + E), hi = P, so |ku kv - hi - lo| <=
+// One window row a.  kPrecSplit3: B_a[r][b] = ku_r[a] kv_r[b] from the factors split into
+// f16 hi + lo (ku = uh + ul, kv = vh + vl), in packed f16: P = f16(uh vh), E = uh vh - P exactly
+// (an fma of 11-bit significands), lo = f16(uh vl + ul vh + E), hi = P, so |ku kv - hi - lo| <=
 // 2^-11 |lo| + |ul vl| ~ 2^-22 |ku kv| (products below 2^-24 / 2^-22 of the largest lose bits to
-// f16 subnormals: contributions under 1e-12 of the peak).  Then D_a += C B_a as hi*hi + lo*hi +
-// hi*lo.
+// f16 subnormals: contributions under 1e-12 of the peak); then D_a += C B_a as hi*hi + lo*hi +
+// hi*lo.  Otherwise B_a = f16(uh vh) (relative error <= 3 x 2^-11 against ku kv, from the three
+// roundings) and D_a += C_hi B_a [+ C_lo B_a].
+template <int P>
 __device__ __forceinline__ void window_row(floatx4& d, const _Float16* __restrict__ uh_row,
                                            const _Float16* __restrict__ ul_row, const half8& vh, const half8& vl,
                                            const half8& a_hi, const half8& a_lo) {
     const half8 uh = *reinterpret_cast<const half8*>(uh_row);
-    const half8 ul = *reinterpret_cast<const half8*>(ul_row);
-    const half8 p = uh * vh;
-    const half8 e = __builtin_elementwise_fma(uh, vh, -p);
-    const half8 b_lo = __builtin_elementwise_fma(ul, vh, __builtin_elementwise_fma(uh, vl, e));
-    const half8 b_hi = p;
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d, 0, 0, 0);
-    d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d, 0, 0, 0);
+    if constexpr (P == kPrecSplit3) {
+        const half8 ul = *reinterpret_cast<const half8*>(ul_row);
+        const half8 p = uh * vh;
+        const half8 e = __builtin_elementwise_fma(uh, vh, -p);
+        const half8 b_lo = __builtin_elementwise_fma(ul, vh, __builtin_elementwise_fma(uh, vl, e));
+        const half8 b_hi = p;
+        d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_hi, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b_hi, d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b_lo, d, 0, 0, 0);
+    } else {
+        const half8 b = uh * vh;
+        d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_hi, b, d, 0, 0, 0);
+        if constexpr (P == kPrecSplitC) d = __builtin_amdgcn_mfma_f32_16x16x32_f16(a_lo, b, d, 0, 0, 0);
+    }
 }
 
 // Per-wavefront constants of the window kernel.
 struct WinLane {
-    _Float16* s_kuh;    // ku hi, lo: [32 rows a = -8 .. 23][32 samples], f16
+    _Float16* s_kuh;    // ku hi, lo: [32 rows a = -8 .. 23][32 samples], f16 (lo: kPrecSplit3 only)
     _Float16* s_kul;
     _Float16* s_kvh;    // kv hi, lo: [32 columns b = -8 .. 23][32 samples], f16
     _Float16* s_kvl;
@@ -545,8 +601,10 @@ struct WinLane {
 
 // One chunk: the records [c0, c0 + 32) of item `it` (this lane's record in q) contracted
 // into the accumulators d for the terms kg .. kg + 7.
+template <int P>
 __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, int c0, int kg, int K,
                                              int cls_log2, const uint4& q, floatx4 (&d)[kWin]) {
+    using T = WinTraits<P>;
     const bool valid = c0 + L.rl < it.count;
     // Classes of the chunk's first and last records (wave-uniform), and the 4-row quarters of
     // the window that they touch: class j covers rows [c d, c d + c + 7], d = j - 8 / c.
@@ -562,15 +620,32 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     const int cls = 1 << cls_log2, dlo = -(8 >> cls_log2);
     const int q_lo = max(0, ((j_first + dlo) * cls) >> 2);
     const int q_hi = min(3, ((j_last + dlo) * cls + cls + 7) >> 2);
-    // ES factors along this lane's axis, as f16 hi + lo: the axis' 32 lanes zero the window's
-    // 16 rows / columns of both planes (one 64-byte row each), then every lane writes its
-    // sample's 9 cells ceil(p - 4) .. ceil(p - 4) + 8 (padded rows take those outside the
-    // window; the ninth is nonzero only for an integer p).
+    // ES factors along this lane's axis, as f16 (hi + lo for kPrecSplit3): the axis' 32 lanes
+    // zero the window's 16 rows / columns of the planes (a 64-byte row per lane, or half a row
+    // with one plane), then every lane writes its sample's 9 cells ceil(p - 4) .. ceil(p - 4) + 8
+    // (padded rows take those outside the window; the ninth is nonzero only for an integer p).
     {
-        _Float16* zrow = ((L.rl & 16) ? L.s_kl : L.s_kh) - L.rl + (kPad + (L.rl & 15)) * kStrideH;
         const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+        if constexpr (T::kEsLo) {
+            _Float16* zrow = ((L.rl & 16) ? L.s_kl : L.s_kh) - L.rl + (kPad + (L.rl & 15)) * kStrideH;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
+            for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
+        } else {
+            _Float16* zrow = L.s_kh - L.rl + (kPad + (L.rl & 15)) * kStrideH + ((L.rl & 16) ? 16 : 0);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
+        }
+    }
+    // Single-f16 B: each sample's two ES factors are scaled by its own factors du, dv in [1, 2)
+    // (from the visibility and s bits of its record) and c by 1 / (du dv).  Rounding ku, kv and
+    // their product to f16 then errs by an amount that is independent from sample to sample and
+    // averages out over the samples of a cell; with a fixed scale the kernel's flat peak sits at
+    // one f16 value for every sample and the rounding is biased (about 4e-6 of the gridded sum).
+    float dith_u = 1.0f, dith_v = 1.0f;
+    if constexpr (!T::kEsLo) {
+        const uint32_t h = (q.x ^ q.y ^ q.z) * 0x9E3779B1u;
+        dith_u = 1.0f + static_cast<float>(h >> 24) * (1.0f / 256.0f);
+        dith_v = 1.0f + static_cast<float>((h >> 16) & 0xffu) * (1.0f / 256.0f);
     }
     // A record's row position is relative to its class's home window row (the first window row
     // that reads the class); this window reads the classes j >= 16 / c as the row above their
@@ -586,6 +661,7 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         // quarter-support polynomial pair (j, 7 - j) in t = F - 1/2, for j = 8 ES(1) when F = 0
         // and zero otherwise.
         const float t = fmaf(static_cast<float>(fb), kRecFracInv, -0.5f), t2 = t * t;
+        const float dith = L.axis ? dith_v : dith_u;
         float v[8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -595,29 +671,40 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
             float o = kEsPolyOdd[j][kEsPolyDegree / 2 - 1];
 #pragma unroll
             for (int i = kEsPolyDegree / 2 - 2; i >= 0; --i) o = fmaf(o, t2, kEsPolyOdd[j][i]);
+            if constexpr (!T::kEsLo) {
+                e *= dith;
+                o *= dith;
+            }
             v[j] = fmaf(t, o, e);
             v[7 - j] = fmaf(-t, o, e);
         }
         _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
-        _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
         // hi of the pair (j, 7 - j) by one packed round-to-nearest conversion, each lo = f16(v - hi)
         // with the difference formed in FP32 (one mixed-precision fma per value).
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const half2v h = __builtin_convertvector((float2v){v[j], v[7 - j]}, half2v);
-            const half2v l = residual_f16x2(h, v[j], v[7 - j]);
             dh[j * kStrideH] = h[0];
             dh[(7 - j) * kStrideH] = h[1];
-            dl[j * kStrideH] = l[0];
-            dl[(7 - j) * kStrideH] = l[1];
+            if constexpr (T::kEsLo) {
+                _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
+                const half2v l = residual_f16x2(h, v[j], v[7 - j]);
+                dl[j * kStrideH] = l[0];
+                dl[(7 - j) * kStrideH] = l[1];
+            }
         }
         const bool edge = fb == 0u;
-        dh[8 * kStrideH] = edge ? kEsPolyEdgeHi : static_cast<_Float16>(0.0f);
-        dl[8 * kStrideH] = edge ? kEsPolyEdgeLo : static_cast<_Float16>(0.0f);
+        if constexpr (T::kEsLo) dh[8 * kStrideH] = edge ? kEsPolyEdgeHi : static_cast<_Float16>(0.0f);
+        else dh[8 * kStrideH] = static_cast<_Float16>(edge ? kEsPolyEdge * dith : 0.0f);
+        if constexpr (T::kEsLo) {
+            _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
+            dl[8 * kStrideH] = edge ? kEsPolyEdgeLo : static_cast<_Float16>(0.0f);
+        }
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
     {
-        const float vs = valid ? rec_v(L.axis ? q.y : q.x, q.w, L.vexp) : 0.0f;
+        float vs = valid ? rec_v(L.axis ? q.y : q.x, q.w, L.vexp) : 0.0f;
+        if constexpr (!T::kEsLo) vs *= __builtin_amdgcn_rcpf(dith_u * dith_v);
         const float s = rec_s(q.x, q.y, q.z), s2 = 2.0f * s;
         float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
         for (int k = 0; k < kg; ++k) {
@@ -640,18 +727,25 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     half8 a_hi, a_lo;
     {
         const float* row = L.s_c + L.n16 * kStride + L.g8;
-        split_hilo(*reinterpret_cast<const float4*>(row), *reinterpret_cast<const float4*>(row + 4), a_hi, a_lo);
+        const float4 x0 = *reinterpret_cast<const float4*>(row), x1 = *reinterpret_cast<const float4*>(row + 4);
+        if constexpr (T::kCLo) {
+            split_hilo(x0, x1, a_hi, a_lo);
+        } else {
+            a_hi = __builtin_convertvector((floatx8){x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w}, half8);
+            a_lo = a_hi;
+        }
     }
     const half8 vh = *reinterpret_cast<const half8*>(L.s_kvh + (kPad + L.n16) * kStrideH + L.g8);
-    const half8 vl = *reinterpret_cast<const half8*>(L.s_kvl + (kPad + L.n16) * kStrideH + L.g8);
+    half8 vl = vh;
+    if constexpr (T::kEsLo) vl = *reinterpret_cast<const half8*>(L.s_kvl + (kPad + L.n16) * kStrideH + L.g8);
     const _Float16* uhb = L.s_kuh + kPad * kStrideH + L.g8;
-    const _Float16* ulb = L.s_kul + kPad * kStrideH + L.g8;
+    const _Float16* ulb = T::kEsLo ? L.s_kul + kPad * kStrideH + L.g8 : uhb;
 #pragma unroll
     for (int qd = 0; qd < 4; ++qd) {
         if (qd >= q_lo && qd <= q_hi) {
 #pragma unroll
             for (int a = 4 * qd; a < 4 * qd + 4; ++a)
-                window_row(d[a], uhb + a * kStrideH, ulb + a * kStrideH, vh, vl, a_hi, a_lo);
+                window_row<P>(d[a], uhb + a * kStrideH, ulb + a * kStrideH, vh, vl, a_hi, a_lo);
         }
     }
     wave_lds_sync();
@@ -698,19 +792,29 @@ __device__ __forceinline__ void window_store(const WinOut& o, const WItem& it, c
 // requested before the current window is written, so the write-out overlaps the next item's
 // loads (vector memory operations complete in order).  The terms are done eight at a time
 // (one 16-row block of m = 2k + ri).
-__global__ void __launch_bounds__(kWaves * 64)
+template <int P>
+__global__ void __launch_bounds__(WinTraits<P>::kWaves * 64)
 k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
            WinGeom wg, float2* __restrict__ planes, float2* __restrict__ partial, float2* __restrict__ trash) {
+    using T = WinTraits<P>;
+    constexpr int kWaves = T::kWaves;
     extern __shared__ __align__(16) float stage_all[];
     __shared__ int s_item, s_chunk, s_witem;
     WinLane L;
     const int wave = threadIdx.x >> 6;
     L.lane = threadIdx.x & 63;
-    L.s_kuh = reinterpret_cast<_Float16*>(stage_all + wave * kStageFloats);
-    L.s_kul = L.s_kuh + kPlaneHalves;
-    L.s_kvh = L.s_kul + kPlaneHalves;
-    L.s_kvl = L.s_kvh + kPlaneHalves;
-    L.s_c = reinterpret_cast<float*>(L.s_kvl + kPlaneHalves);
+    L.s_kuh = reinterpret_cast<_Float16*>(stage_all + wave * T::kStageFloats);
+    if constexpr (T::kEsLo) {
+        L.s_kul = L.s_kuh + kPlaneHalves;
+        L.s_kvh = L.s_kul + kPlaneHalves;
+        L.s_kvl = L.s_kvh + kPlaneHalves;
+        L.s_c = reinterpret_cast<float*>(L.s_kvl + kPlaneHalves);
+    } else {
+        L.s_kvh = L.s_kuh + kPlaneHalves;
+        L.s_kul = L.s_kuh;
+        L.s_kvl = L.s_kvh;
+        L.s_c = reinterpret_cast<float*>(L.s_kvh + kPlaneHalves);
+    }
     L.rl = L.lane & 31;
     L.axis = L.lane >> 5;
     L.g8 = 8 * (L.lane >> 4);
@@ -770,16 +874,16 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                 const uint4 q = q_next;
                 const int c1 = claim_chunk();
                 if (c1 < it.count) q_next = rec.get(it.start + min(c1 + L.rl, it.count - 1));
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, d);
+                window_chunk<P>(L, it, c0, kg, K, wg.cls_log2, q, d);
                 c0 = c1;
             }
             // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
-            // wavefronts 0-5 store theirs in slots 0-5, wavefronts 6-11 add theirs to slot
-            // w - 6, and the write sums the six slots.  Slot layout red4[slot][a][lane] (float4
-            // = the lane's D[4 (lane / 16) + i][n16], i = 0..3).  The chunk counter is reset for
-            // the next term block or item once every wavefront is done with it.
-            constexpr int kSlots = kWaves / 2;
-            static_assert(kSlots * kWin * 64 * 4 <= kStageFloats * kWaves, "reduction slots exceed the staging area");
+            // wavefronts 0 .. S-1 store theirs in slots 0 .. S-1, then each following group of S
+            // wavefronts adds theirs to slot w mod S, one group per round, and the write sums the
+            // S slots.  Slot layout red4[slot][a][lane] (float4 = the lane's
+            // D[4 (lane / 16) + i][n16], i = 0..3).  The chunk counter is reset for the next term
+            // block or item once every wavefront is done with it.
+            constexpr int kSlots = T::kSlots;
             float4* red4 = reinterpret_cast<float4*>(stage_all);
             __syncthreads();
             if (wave_id < kSlots) {
@@ -789,15 +893,18 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             }
             if (threadIdx.x == 0) s_chunk = 0;
             __syncthreads();
-            if (wave_id >= kSlots) {
 #pragma unroll
-                for (int a = 0; a < kWin; ++a) {
-                    float4& r = red4[((wave_id - kSlots) * kWin + a) * 64 + lane];
-                    const float4 v = r;
-                    r = make_float4(v.x + d[a][0], v.y + d[a][1], v.z + d[a][2], v.w + d[a][3]);
+            for (int round = 1; round < kWaves / kSlots; ++round) {
+                if (wave_id >= round * kSlots && wave_id < (round + 1) * kSlots) {
+#pragma unroll
+                    for (int a = 0; a < kWin; ++a) {
+                        float4& r = red4[((wave_id - round * kSlots) * kWin + a) * 64 + lane];
+                        const float4 v = r;
+                        r = make_float4(v.x + d[a][0], v.y + d[a][1], v.z + d[a][2], v.w + d[a][3]);
+                    }
                 }
+                __syncthreads();
             }
-            __syncthreads();
             // Write: (row a, source lane sl) -> terms kg + 2 (sl / 16) + h, h = 0, 1, column sl % 16.
             for (int v = threadIdx.x; v < kWin * 64; v += kWaves * 64) {
                 const int a = v >> 6, sl = v & 63;
@@ -848,7 +955,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             for (int c0 = 0; c0 < it.count; c0 += kChunk) {
                 const uint4 q = q_next;
                 if (c0 + kChunk < it.count) q_next = rec.get(it.start + min(c0 + kChunk + L.rl, it.count - 1));
-                window_chunk(L, it, c0, kg, K, wg.cls_log2, q, d);
+                window_chunk<P>(L, it, c0, kg, K, wg.cls_log2, q, d);
             }
             // What comes next: this item's next term block, or the next item.  Its first
             // records are requested now, ahead of the stores below.
@@ -884,26 +991,41 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
 
 }  // namespace
 
+namespace {
+// Row-class height (log2) and bin count of a grid: classes of 4 cells when that histogram fits in
+// LDS, else of 8 (half the bins, so half the LDS bands when even those do not fit).
+int window_class_log2(const GridParams& gp, int nw) {
+    const size_t cols = static_cast<size_t>(gp.nslabs) * nw;
+    return cols * ((gp.G + 3) / 4) <= static_cast<size_t>(kLdsBins) ? 2 : 3;
+}
+size_t window_bins(const GridParams& gp, int nw, int cls_log2) {
+    const size_t nh = (static_cast<size_t>(gp.G) + (1u << cls_log2) - 1) >> cls_log2;
+    return static_cast<size_t>(gp.nslabs) * nw * nh;
+}
+}  // namespace
+
+bool grid_windows_supported(const SampleInput& in, const GridParams& gp) {
+    if (gp.G > 8192 || gp.G < 1) return false;
+    const int nw = (gp.G + kWin - 1) / kWin;
+    if (window_bins(gp, nw, window_class_log2(gp, nw)) >= (1ull << 30)) return false;
+    return 2 * in.num_samples < (1ull << 31);             // records: at most two per sample
+}
+
 const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                                     float2* planes) {
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
+    if (!grid_windows_supported(in, gp)) throw UnsupportedError("grid or sample count too large for the window gridder");
     WinGeom wg;
     wg.nw = (gp.G + kWin - 1) / kWin;
-    // Row classes of 4 cells when that histogram fits in LDS, else of 8 (half the bins, so half
-    // the LDS bands when even those do not fit).
-    const size_t cols = static_cast<size_t>(gp.nslabs) * wg.nw;
-    const bool fits4 = cols * ((gp.G + 3) / 4) <= static_cast<size_t>(kLdsBins);
-    wg.cls_log2 = fits4 ? 2 : 3;
+    wg.cls_log2 = window_class_log2(gp, wg.nw);
     wg.ncls = 24 >> wg.cls_log2;
     wg.nh = (gp.G + (1 << wg.cls_log2) - 1) >> wg.cls_log2;
     wg.nwin = wg.nw * wg.nw;
     wg.nwin_total = wg.nwin * gp.nslabs;
-    const size_t nbins_sz = static_cast<size_t>(gp.nslabs) * wg.nw * wg.nh;
-    if (nbins_sz >= (1ull << 30) || gp.G > 8192) throw UnsupportedError("grid too large for the window gridder");
+    const size_t nbins_sz = window_bins(gp, wg.nw, wg.cls_log2);
     wg.nbins = static_cast<int>(nbins_sz);
     const size_t S = in.num_samples;
     const size_t max_records = 2 * S;                      // one row class, at most two window columns
-    if (max_records >= (1ull << 31)) throw UnsupportedError("problem too large for the window gridder");
     const size_t nbins = nbins_sz;
     const size_t nwt = static_cast<size_t>(wg.nwin_total);
     // Wavefront items: one per window at most.  Workgroup pieces: at most one per window plus
@@ -927,7 +1049,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
 
     // counts[0, nbins) | counts[nbins + 2, +8) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 12) * sizeof(uint32_t)));
-    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, nhist * sizeof(uint32_t)));
+    // Per-block histograms, then the (block, band) record totals.
+    const size_t ntot = static_cast<size_t>(blocks) * bands;
+    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, (nhist + ntot) * sizeof(uint32_t)));
+    uint32_t* blocktot = blockhist + nhist;
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(
         ctx->ws.get(Workspace::kBinCursor, (nbins + 8 + nwt + (wg.use_lds ? kBinSegs * nbins : 0)) * sizeof(uint32_t)));
@@ -950,10 +1075,10 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
     if (in.from_snapshot) {
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_count<true>), bin_lds);
-        k_win_count<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
+        k_win_count<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, blocktot);
     } else {
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_count<false>), bin_lds);
-        k_win_count<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist);
+        k_win_count<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, blocktot);
     }
     SVDI_LAUNCH_CHECK();
     const unsigned seg_grid = static_cast<unsigned>((kBinSegs * nbins + 255) / 256);
@@ -975,20 +1100,27 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const uint32_t limit = static_cast<uint32_t>(max_records);
     if (in.from_snapshot) {
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_scatter<true>), bin_lds);
-        k_win_scatter<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
-                                                                         totals + 4);
+        k_win_scatter<true><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, blocktot, cursor,
+                                                                         rec, limit, totals + 4, ctx->dev_err);
     } else {
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_scatter<false>), bin_lds);
-        k_win_scatter<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, cursor, rec, limit,
-                                                                          totals + 4);
+        k_win_scatter<false><<<bin_grid, kBinThreads, bin_lds, ctx->stream>>>(in, gp, wg, blockhist, blocktot, cursor,
+                                                                          rec, limit, totals + 4, ctx->dev_err);
     }
     SVDI_LAUNCH_CHECK();
 
     const size_t tok = stage_begin(ctx, 5);
-    constexpr size_t lds = sizeof(float) * kStageFloats * kWaves;
-    allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_grid), lds);
     const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
-    k_win_grid<<<grid, kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
+    auto launch = [&](auto prec) {
+        constexpr int P = decltype(prec)::value;
+        using T = WinTraits<P>;
+        constexpr size_t lds = sizeof(float) * T::kStageFloats * T::kWaves;
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_grid<P>), lds);
+        k_win_grid<P><<<grid, T::kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
+    };
+    if (gp.prec == kPrecSingle) launch(std::integral_constant<int, kPrecSingle>());
+    else if (gp.prec == kPrecSplitC) launch(std::integral_constant<int, kPrecSplitC>());
+    else launch(std::integral_constant<int, kPrecSplit3>());
     SVDI_LAUNCH_CHECK();
     k_win_reduce<<<2048, 256, 0, ctx->stream>>>(boff, ioff, multi, totals, wg, gp.G, gp.K,
                                                 static_cast<uint32_t>(max_slots), partial, planes);

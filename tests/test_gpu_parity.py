@@ -19,6 +19,15 @@ import svdimager_amd as S
 pytestmark = pytest.mark.gpu
 
 
+# Archive dirty image (after |.| and the degree-trig SIN splat) against the oracle.  The
+# archive's p2p takes cos of an FP32 angle in degrees near 90 degrees: one ulp of that angle
+# (7.6e-6 degree) moves a splat target by about 1e-3 pixel at this cell size, host and device
+# libm may round it differently, and a bilinear splat moved by d pixels changes a pixel by up to
+# about 4 d of the values splatted onto it.  Measured on MI355X: 3.0e-3 over the image, 2.7e-3
+# over the central half, i.e. uniform (the signed image before the splat agrees to 1e-5).
+ARCHIVE_DIRTY = 5e-3
+
+
 def _peak(d):
     return float(np.nanmax(np.abs(d)))
 
@@ -98,6 +107,38 @@ def test_fipipedevice_matches_oracle_ref32(gpu_ctx, small_snap, oracle_pre):
         assert_parity(img, ref[key], key, central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
 
 
+def test_default_options_take_the_mfma_gridder(gpu_ctx, small_snap, small_dft, dsnap, oracle_pre):
+    """With default options (gridder AUTO) the snapshot path, FIpipeDevice and the archive FIpipe
+    path (svdi_fipipe_host, which the C++ FIpipe symbol calls with default options) run the MFMA
+    window gridder and meet their parity bounds."""
+    announce("default options -> MFMA gridder")
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    img, plan = S.image_snapshot(gpu_ctx, dsnap, N, cell, output=S.OUTPUT_SIGNED,
+                                 num_terms=small_snap["num_terms"])
+    assert plan["gridder"] == S.GRIDDER_MFMA
+    dft = small_dft["dft"]
+    peak = _peak(dft)
+    img = img.cpu().numpy()
+    assert _rel(img, dft, peak) <= 1.1e-4
+    assert _rel(_central(img, N), _central(dft, N), peak) <= 2e-5
+    t = {k: torch.from_numpy(np.ascontiguousarray(oracle_pre[k])).cuda()
+         for k in ("bin", "vin", "vis_re", "vis_im")}
+    ref = O.fipipe_device_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                oracle_pre["vis_im"], N, cell)
+    for output, key in ((S.OUTPUT_SIGNED, "signed"), (S.OUTPUT_DIRTY, "dirty")):
+        img, plan = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N, cell,
+                                   output=output)
+        assert plan["gridder"] == S.GRIDDER_MFMA
+        assert_parity(img.cpu().numpy(), ref[key], f"default {key}",
+                      central=DIRTY_CENTRAL if key == "dirty" else 1e-5)
+    aref = O.fipipe_archive_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
+                                  oracle_pre["vis_im"], N, cell)
+    aimg, plan = S.fipipe_host(gpu_ctx, oracle_pre["vis_re"], oracle_pre["vis_im"], oracle_pre["bin"],
+                               oracle_pre["vin"], N, cell, output=S.OUTPUT_SIGNED)
+    assert plan["gridder"] == S.GRIDDER_MFMA
+    assert_parity(aimg, aref["signed"], "default archive signed")
+
+
 def test_fipipe_symbol_matches_archive_oracle(small_snap, oracle_pre):
     announce("FIpipe C++ symbol small vs oracle archive")
     N, cell = small_snap["image_size"], small_snap["cell"]
@@ -109,15 +150,17 @@ def test_fipipe_symbol_matches_archive_oracle(small_snap, oracle_pre):
     assert rc == 0
     ref = O.fipipe_archive_ref32(oracle_pre["bin"], oracle_pre["vin"], oracle_pre["vis_re"],
                                  oracle_pre["vis_im"], N, cell)
-    err = _rel(dirty.reshape(N, N), ref["dirty"], _peak(ref["dirty"]))
-    print(f"archive dirty max rel {err:.3e}, K {ref['plan']['num_terms']}")
-    # The archive's p2p takes cos of an FP32 angle in degrees near 90 degrees; one ulp of that
-    # angle moves a splat target by ~1e-2 pixel, so host and device libm differ at 1e-3 here.
-    assert err <= 1e-2
+    peak = _peak(ref["dirty"])
+    d2 = dirty.reshape(N, N)
+    err = _rel(d2, ref["dirty"], peak)
+    err_c = _rel(_central(d2, N), _central(ref["dirty"], N), peak)
+    print(f"archive dirty max rel {err:.3e} central {err_c:.3e}, K {ref['plan']['num_terms']}")
+    assert err <= ARCHIVE_DIRTY and err_c <= ARCHIVE_DIRTY
     assert abs(float(dirty.sum()) - float(ref["dirty"].sum())) <= 1e-4 * float(np.abs(ref["dirty"]).sum())
-    ctx_img, plan = S.fipipe_host(S.Context(0), oracle_pre["vis_re"], oracle_pre["vis_im"],
-                                  oracle_pre["bin"], oracle_pre["vin"], N, cell,
-                                  output=S.OUTPUT_SIGNED)
+    with S.Context(0) as ctx:
+        ctx_img, plan = S.fipipe_host(ctx, oracle_pre["vis_re"], oracle_pre["vis_im"],
+                                      oracle_pre["bin"], oracle_pre["vin"], N, cell,
+                                      output=S.OUTPUT_SIGNED)
     assert plan["num_terms"] == ref["plan"]["num_terms"]
     assert_parity(ctx_img, ref["signed"], "archive signed")
 

@@ -125,6 +125,81 @@ def test_cli_usage_exit_code():
     assert r.returncode == 1
 
 
+def _cli_inputs(tmp_path, S):
+    from svdimager_amd import write_fits_2d
+    rng = np.random.default_rng(5)
+    paths = []
+    for name, arr in (("re", rng.normal(size=S)), ("im", rng.normal(size=S)),
+                      ("b", rng.normal(size=(S, 3))), ("v", np.eye(3))):
+        p = str(tmp_path / f"{name}.fits")
+        write_fits_2d(p, np.asarray(arr, np.float32).reshape(-1, 3) if name in ("b", "v")
+                      else np.asarray(arr, np.float32).reshape(1, -1))
+        paths.append(p)
+    return paths
+
+
+def test_cli_rejects_rows_beyond_the_inputs(tmp_path):
+    """Number_of_Rows larger than the FITS inputs hold is an error (exit 1, no output), checked
+    before any device work."""
+    cli = os.path.join(ROOT, "svdimager_amd", "bin", "sharedlibrary_gpu")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    paths = _cli_inputs(tmp_path, 10)
+    out = str(tmp_path / "out.fits")
+    for rows in ("11", "100"):
+        r = subprocess.run([cli, *paths, "16", rows, "1e-3", out], capture_output=True, text=True)
+        assert r.returncode == 1 and "does not fit" in r.stderr, r.stderr
+        assert not os.path.exists(out)
+
+
+def test_cli_fipipe_failure_writes_no_image(tmp_path):
+    """When FIpipe fails (here: no GPU), the CLI exits 1 and writes no output file."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    cli = os.path.join(ROOT, "svdimager_amd", "bin", "sharedlibrary_gpu")
+    if not os.path.exists(cli):
+        pytest.skip("CLI not built")
+    paths = _cli_inputs(tmp_path, 10)
+    out = str(tmp_path / "out.fits")
+    r = subprocess.run([cli, *paths, "16", "10", "1e-3", out], capture_output=True, text=True)
+    assert r.returncode == 1
+    assert not os.path.exists(out)
+
+
+CXX_CALLER = r"""
+#include <cstddef>
+#include <cstdio>
+// archive/pipe_interface.cpp:8, verbatim in meaning: C++ linkage, no header of this library.
+int FIpipe(float* Visreal, float* Visimag, float* Bin, float* Vin, float* dirty_image,
+           size_t num_baselines, size_t image_size, float cell_size);
+int main() {
+    float re[4] = {1, 1, 1, 1}, im[4] = {0, 0, 0, 0};
+    float b[12] = {0, 0, 0, 1, 1, 0, -1, 2, 0, 3, -1, 0};
+    float v[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float dirty[16 * 16];
+    const int rc = FIpipe(re, im, b, v, dirty, 4, 16, 1e-3f);
+    std::printf("FIpipe returned %d\n", rc);
+    return 0;
+}
+"""
+
+
+def test_cxx_caller_with_the_reference_prototype_links(tmp_path):
+    """A C++ caller that declares FIpipe exactly as archive/pipe_interface.cpp:8 compiles and links
+    against libsvdimager and the symbol resolves at run time (the call itself fails without a
+    GPU and returns nonzero; with one it returns 0)."""
+    src = tmp_path / "caller.cpp"
+    src.write_text(CXX_CALLER)
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", str(src), "-o", str(exe), f"-L{libdir}", "-lsvdimager",
+                    f"-Wl,-rpath,{libdir}"], check=True, capture_output=True, text=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "FIpipe returned" in r.stdout
+
+
 PROBE = r"""
 #include "svdi_math.hpp"
 extern "C" {
