@@ -172,6 +172,21 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snapshot, float*
                         size_t image_size, float cell_size, const svdi_options* opts,
                         svdi_plan_info* info);
 
+/* svdi_image_snapshot without any device allocation (no counterpart in the reference; for
+ * callers that cannot allocate: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The call's scratch (rotation, plan, binning records, planes, FFT buffers)
+ * is carved from a fixed device array of SVDI_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends; a snapshot or image that needs more returns
+ * SVDI_ERR_INVALID_ARGUMENT with the sizes in svdi_last_error() (about 2 M samples at 512^2
+ * fit).  Slabs are imaged one at a time unless opts->max_batch_slabs says otherwise.  Every
+ * context on a GPU shares that GPU's array: calls that overlap in time on one GPU (from two
+ * contexts or threads) race and must be serialised by the caller.  The call returns when the
+ * image is complete. */
+#define SVDI_NOALLOC_SCRATCH_BYTES (256u << 20)
+int svdi_image_snapshot_noalloc(svdi_context* ctx, const svdi_snapshot* snapshot, float* image,
+                                size_t image_size, float cell_size, const svdi_options* opts,
+                                svdi_plan_info* info);
+
 /* Batched svdi_image_snapshot (no counterpart in the reference, which images one snapshot per
  * process): `count` device-resident snapshots imaged into images[i] (device pointers,
  * image_size^2 floats each), up to four at a time on concurrent streams with their own

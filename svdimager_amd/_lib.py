@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
-    "_Z6FIpipePfS_S_S_S_mmf",
+    "svdi_image_snapshot_noalloc", "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
 
@@ -91,6 +91,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run svdimager_amd.build.build() first")
+        # torch's wheel carries its own HIP runtime, which its libraries load by file name
+        # (libamdhip64.so), while this library needs it by soname (libamdhip64.so.7).  Loading
+        # torch first makes this library bind to that same runtime; loaded the other way round,
+        # the process would hold two runtimes and torch would find no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, sz, f = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_float
         sigs = {
@@ -110,6 +118,9 @@ def lib():
             "svdi_image_snapshot": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, sz, f,
                                                    ctypes.POINTER(Options),
                                                    ctypes.POINTER(PlanInfo)]),
+            "svdi_image_snapshot_noalloc": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, sz, f,
+                                                           ctypes.POINTER(Options),
+                                                           ctypes.POINTER(PlanInfo)]),
             "svdi_image_snapshot_batch": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz,
                                                          ctypes.POINTER(ctypes.c_void_p), sz, f,
                                                          ctypes.POINTER(Options),

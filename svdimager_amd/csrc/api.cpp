@@ -20,10 +20,33 @@
 
 #include "svdi_internal.hpp"
 
+// Device scratch of the no-allocation entry point (svdi_image_snapshot_noalloc): a file-scope
+// array of which every GPU the library runs on holds its own copy until the process ends.  Host
+// code names it only in hipGetSymbolAddress, once per device at context creation; everywhere
+// else (kernel arguments, fills) the address returned there is used.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace svdi {
 
 namespace {
 thread_local std::string g_last_error;
+
+// The device address of `scratch` on each device, taken with that device current.
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_at;
+
+void* noalloc_scratch_address(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_at.size() <= static_cast<size_t>(device)) g_scratch_at.resize(device + 1, nullptr);
+    if (g_scratch_at[device] == nullptr) {
+        void* p = nullptr;
+        SVDI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_at[device] = p;
+    }
+    return g_scratch_at[device];
+}
 
 void set_error(const std::string& s) { g_last_error = s; }
 
@@ -491,6 +514,16 @@ void* Workspace::get(Slot s, size_t bytes) {
     if (bytes == 0) bytes = 16;
     bytes = (bytes + 255) & ~static_cast<size_t>(255);
     if (size_[s] >= bytes) return ptr_[s];
+    if (arena_ != nullptr) {
+        if (bytes > arena_cap_ - arena_used_)
+            throw ArgError("the call needs more device scratch than the no-allocation entry point's " +
+                           std::to_string(arena_cap_ >> 20) + " MiB (" + std::to_string(arena_used_) +
+                           " bytes in use, " + std::to_string(bytes) + " more requested)");
+        ptr_[s] = arena_ + arena_used_;
+        size_[s] = bytes;
+        arena_used_ += bytes;
+        return ptr_[s];
+    }
     if (ptr_[s] != nullptr) {
         SVDI_HIP(hipDeviceSynchronize());
         SVDI_HIP(hipFree(ptr_[s]));
@@ -502,6 +535,29 @@ void* Workspace::get(Slot s, size_t bytes) {
     ptr_[s] = p;
     size_[s] = bytes;
     return p;
+}
+
+void Workspace::begin_arena(void* base, size_t capacity) {
+    for (int i = 0; i < kSlots; ++i) {
+        kept_ptr_[i] = ptr_[i];
+        kept_size_[i] = size_[i];
+        ptr_[i] = nullptr;
+        size_[i] = 0;
+    }
+    arena_ = static_cast<char*>(base);
+    arena_cap_ = capacity & ~static_cast<size_t>(255);
+    arena_used_ = 0;
+}
+
+void Workspace::end_arena() {
+    for (int i = 0; i < kSlots; ++i) {
+        ptr_[i] = kept_ptr_[i];
+        size_[i] = kept_size_[i];
+        kept_ptr_[i] = nullptr;
+        kept_size_[i] = 0;
+    }
+    arena_ = nullptr;
+    arena_cap_ = arena_used_ = 0;
 }
 
 void Workspace::release() {
@@ -540,8 +596,34 @@ std::unique_ptr<svdi_context> create_context(int device) {
     hipDeviceProp_t prop;
     SVDI_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
+    c->noalloc_scratch = noalloc_scratch_address(device);
     return c;
 }
+
+// A call of the no-allocation entry point: the workspace carves its slots from the device's
+// scratch array, the cached grid-correction and twiddle tables are recomputed (the array holds
+// whatever the last call on this device left), and no timing events are created.  On leaving,
+// the context's own workspace is restored and the stream is drained (also after an error, so
+// that no kernel of this call still uses the array).
+struct NoAllocCall {
+    svdi_context* c;
+    bool profiling;
+    explicit NoAllocCall(svdi_context* ctx) : c(ctx), profiling(ctx->profiling) {
+        c->profiling = false;
+        c->conv_at = nullptr;
+        c->twiddle_at = nullptr;
+        c->ws.begin_arena(c->noalloc_scratch, kNoAllocScratchBytes);
+    }
+    ~NoAllocCall() {
+        (void)hipStreamSynchronize(c->stream);
+        c->ws.end_arena();
+        c->conv_at = nullptr;
+        c->twiddle_at = nullptr;
+        c->profiling = profiling;
+    }
+    NoAllocCall(const NoAllocCall&) = delete;
+    NoAllocCall& operator=(const NoAllocCall&) = delete;
+};
 
 // The batched call: enqueue every snapshot on the lanes, then wait for the lanes or not.
 int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
@@ -757,6 +839,24 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snap, float* ima
     });
 }
 
+
+int svdi_image_snapshot_noalloc(svdi_context* ctx, const svdi_snapshot* snap, float* image,
+                                size_t image_size, float cell_size, const svdi_options* opts,
+                                svdi_plan_info* info) {
+    return guarded([&] {
+        if (!ctx || !snap) throw ArgError("ctx / snapshot is null");
+        check_device_pointer(image, "image");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        if (ctx->noalloc_scratch == nullptr) throw std::runtime_error("the context has no scratch address");
+        begin_call(ctx);
+        NoAllocCall scope(ctx);
+        SampleInput in = snapshot_input(ctx, *snap);
+        svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        if (o.max_batch_slabs == 0) o.max_batch_slabs = 1;   // one slab's planes at a time
+        image_core(ctx, in, image_size, cell_size, o, info, image);
+    });
+}
 
 int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
                               float* const* images, size_t image_size, float cell_size,

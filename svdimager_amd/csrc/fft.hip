@@ -638,17 +638,24 @@ bool factorize(int n, FftStages& st) {
 
 }  // namespace
 
+// Twiddles e^{2 pi i t / G}, t < G, from FP64 sincospi, rounded to FP32.
+__global__ void k_twiddles(float2* __restrict__ tw, int G) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= G) return;
+    double sn, cs;
+    sincospi(2.0 * static_cast<double>(t) / static_cast<double>(G), &sn, &cs);
+    tw[t] = make_float2(static_cast<float>(cs), static_cast<float>(sn));
+}
+
+// The twiddle table of G in the context's workspace, computed on the stream (no host copy, no
+// synchronisation) when G or the table's address changed since the last call.
 void fft_prepare(svdi_context* ctx, size_t G) {
-    if (ctx->twiddle_n == G) return;
-    std::vector<float2> tw(G);
-    for (size_t t = 0; t < G; ++t) {
-        const double a = 2.0 * kPi * static_cast<double>(t) / static_cast<double>(G);
-        tw[t] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-    }
-    void* d = ctx->ws.get(Workspace::kTwiddles, G * sizeof(float2));
-    SVDI_HIP(hipMemcpyAsync(d, tw.data(), G * sizeof(float2), hipMemcpyHostToDevice, ctx->stream));
-    SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    float2* d = static_cast<float2*>(ctx->ws.get(Workspace::kTwiddles, G * sizeof(float2)));
+    if (ctx->twiddle_n == G && ctx->twiddle_at == d) return;
+    k_twiddles<<<static_cast<unsigned>((G + 255) / 256), 256, 0, ctx->stream>>>(d, static_cast<int>(G));
+    SVDI_LAUNCH_CHECK();
     ctx->twiddle_n = G;
+    ctx->twiddle_at = d;
 }
 
 void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size_t G,

@@ -112,7 +112,9 @@ struct GridParams {
     int prec;               // MFMA gridder operand precision: 0 split3, 1 split c, 2 single
 };
 
-// Grown-on-demand device buffers owned by a context.
+// Grown-on-demand device buffers owned by a context.  In arena mode (between begin_arena and
+// end_arena) the slots are carved instead, in request order, from a caller-given device range,
+// and nothing is allocated or freed; a request beyond the range throws ArgError.
 class Workspace {
   public:
     enum Slot {
@@ -123,11 +125,18 @@ class Workspace {
     void* get(Slot s, size_t bytes);
     void release();
     size_t bytes_held() const;
+    void begin_arena(void* base, size_t capacity);
+    void end_arena();
     ~Workspace() { release(); }
 
   private:
     void* ptr_[kSlots] = {};
     size_t size_[kSlots] = {};
+    // arena mode: the carved range, and the allocated slots put aside meanwhile
+    char* arena_ = nullptr;
+    size_t arena_cap_ = 0, arena_used_ = 0;
+    void* kept_ptr_[kSlots] = {};
+    size_t kept_size_[kSlots] = {};
 };
 
 }  // namespace svdi
@@ -155,11 +164,14 @@ struct svdi_context {
     svdi::PinnedPlan* pinned = nullptr;      // page-locked read-back of the plan
     // Device word the binning kernels OR kErr* bits into; cleared after each read-back.
     uint32_t* dev_err = nullptr;
+    // This device's no-allocation scratch (the address of the library's static array there).
+    void* noalloc_scratch = nullptr;
     // Batched calls: child contexts, each with its own stream and workspace, that image
     // snapshots concurrently.
     std::vector<std::unique_ptr<svdi_context>> lanes;
     std::mutex mu;
-    size_t twiddle_n = 0;
+    size_t twiddle_n = 0;                    // FFT twiddles held for G = twiddle_n at twiddle_at
+    const void* twiddle_at = nullptr;
     // Grid-correction table held in the workspace: computed for (N, G) at this address.
     size_t conv_n = 0, conv_g = 0;
     const float* conv_at = nullptr;

@@ -122,8 +122,10 @@ def FIpipeDevice(ctx: Context, bin_, vin, vis_re, vis_im, image_size: int, cell_
 
 
 def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_size: float,
-                   image=None, **options) -> Tuple[object, Dict]:
-    """Preprocess + image one device-resident snapshot; returns (image, plan dict)."""
+                   image=None, noalloc: bool = False, **options) -> Tuple[object, Dict]:
+    """Preprocess + image one device-resident snapshot; returns (image, plan dict).
+    noalloc=True calls svdi_image_snapshot_noalloc (no device allocation inside the call; the
+    image tensor, when not given, is still allocated here by torch)."""
     torch = _torch()
     dev = dsnap.tensors["uvw"].device
     N = int(image_size)
@@ -133,8 +135,9 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
     info = PlanInfo()
     _bind_stream(ctx, dev)
     s = dsnap.struct()
-    check(lib().svdi_image_snapshot(ctx.handle, ctypes.byref(s), _ptr(image), N,
-                                    float(cell_size), ctypes.byref(opts), ctypes.byref(info)))
+    fn = lib().svdi_image_snapshot_noalloc if noalloc else lib().svdi_image_snapshot
+    check(fn(ctx.handle, ctypes.byref(s), _ptr(image), N, float(cell_size), ctypes.byref(opts),
+             ctypes.byref(info)))
     return image, info.as_dict()
 
 
