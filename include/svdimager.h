@@ -206,6 +206,15 @@ int svdi_image_snapshot_batch_async(svdi_context* ctx, const svdi_snapshot* snap
                                     float* const* images, size_t image_size, float cell_size,
                                     const svdi_options* opts, svdi_plan_info* infos);
 
+/* Batched imaging of snapshots of mixed image sizes and cells (no counterpart in the reference;
+ * the batching form): snapshot i is imaged into images[i], image_sizes[i]^2 floats, with cell
+ * cell_sizes[i].  The snapshots are dealt to the concurrent lanes largest first.  wait != 0:
+ * returns when every image is done (as svdi_image_snapshot_batch); wait == 0: returns once the
+ * work is enqueued (as svdi_image_snapshot_batch_async). */
+int svdi_image_snapshots(svdi_context* ctx, const svdi_snapshot* snapshots, size_t count,
+                         float* const* images, const size_t* image_sizes, const float* cell_sizes,
+                         const svdi_options* opts, svdi_plan_info* infos, int wait);
+
 /* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
  * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
-    "svdi_image_snapshot_noalloc", "_Z6FIpipePfS_S_S_S_mmf",
+    "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
 
@@ -129,6 +129,12 @@ def lib():
                                                                ctypes.POINTER(ctypes.c_void_p), sz, f,
                                                                ctypes.POINTER(Options),
                                                                ctypes.POINTER(PlanInfo)]),
+            "svdi_image_snapshots": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz,
+                                                    ctypes.POINTER(ctypes.c_void_p),
+                                                    ctypes.POINTER(ctypes.c_size_t),
+                                                    ctypes.POINTER(ctypes.c_float),
+                                                    ctypes.POINTER(Options),
+                                                    ctypes.POINTER(PlanInfo), ctypes.c_int]),
             "svdi_fipipe_host": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, sz, sz, f,
                                                 ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
             "svdi_fits_write": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_long, ctypes.c_long]),

@@ -626,18 +626,42 @@ struct NoAllocCall {
 };
 
 // The batched call: enqueue every snapshot on the lanes, then wait for the lanes or not.
+// sizes[i] / cells[i]: snapshot i's image side and cell.  Snapshots are dealt to the lanes
+// largest first (cost ~ samples + 4 grid cells, the least-loaded lane next), and each lane
+// always has its next snapshot's plan queued behind its current work; the host enqueues the
+// gridding of whichever lane's plan is back first, waiting only when none is.
 int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
-                size_t image_size, float cell_size, const svdi_options* opts, svdi_plan_info* infos,
+                const size_t* sizes, const float* cells, const svdi_options* opts, svdi_plan_info* infos,
                 bool wait) {
     return guarded([&] {
-        if (!ctx || !snaps || !images) throw ArgError("ctx / snapshots / images is null");
-        for (size_t i = 0; i < count; ++i) check_device_pointer(images[i], "image");
+        if (!ctx || !snaps || !images || !sizes || !cells) throw ArgError("ctx / snapshots / images / sizes / cells is null");
+        for (size_t i = 0; i < count; ++i) {
+            check_device_pointer(images[i], "image");
+            if (sizes[i] == 0) throw ArgError("Image_Size must be greater than zero");
+            if (!(cells[i] > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
+        }
         if (count == 0) return;
         std::lock_guard<std::mutex> lk(ctx->mu);
         DeviceGuard g(ctx->device);
         const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
         const size_t L = std::min(count, kBatchLanes);
         while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
+        // Largest-first assignment of snapshots to lanes.
+        std::vector<size_t> order(count);
+        std::vector<double> cost(count);
+        for (size_t i = 0; i < count; ++i) {
+            order[i] = i;
+            const double G = static_cast<double>((sizes[i] * 3 + 1) / 2);
+            cost[i] = static_cast<double>(snaps[i].num_rows) * static_cast<double>(snaps[i].num_channels) + 4.0 * G * G;
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        std::vector<std::vector<size_t>> queue(L);
+        std::vector<double> load(L, 0.0);
+        for (size_t i : order) {
+            const size_t j = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+            queue[j].push_back(i);
+            load[j] += cost[i];
+        }
         // The lanes start after the caller's earlier work on the context stream, and the context
         // stream continues after the last lane's work.
         hipEvent_t ready = nullptr;
@@ -649,21 +673,41 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
         for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ready, 0));
         std::vector<CallState> cs(count);
         std::vector<SampleInput> in(count);
-        // Rounds of L snapshots, one per lane: every lane's planning is enqueued first, then each
-        // lane's gridding as soon as its plan is back, while the other lanes keep the GPU busy.
-        for (size_t r0 = 0; r0 < count; r0 += L) {
-            const size_t n = std::min(L, count - r0);
-            for (size_t j = 0; j < n; ++j) {
-                svdi_context* lane = ctx->lanes[j].get();
-                begin_call(lane);
-                in[r0 + j] = snapshot_input(lane, snaps[r0 + j]);
-                image_begin(lane, in[r0 + j], image_size, cell_size, o, cs[r0 + j]);
+        std::vector<size_t> next(L, 0);                        // position in the lane's queue
+        std::vector<long> planned(L, -1);                      // snapshot whose plan is queued
+        size_t order_of_plan = 0;
+        std::vector<size_t> plan_seq(L, 0);                    // when that plan was queued
+        auto begin_next = [&](size_t j) {
+            if (next[j] >= queue[j].size()) return;
+            const size_t i = queue[j][next[j]++];
+            svdi_context* lane = ctx->lanes[j].get();
+            begin_call(lane);
+            in[i] = snapshot_input(lane, snaps[i]);
+            image_begin(lane, in[i], sizes[i], cells[i], o, cs[i]);
+            planned[j] = static_cast<long>(i);
+            plan_seq[j] = order_of_plan++;
+        };
+        for (size_t j = 0; j < L; ++j) begin_next(j);
+        for (;;) {
+            long pick = -1;
+            for (size_t j = 0; j < L && pick < 0; ++j) {
+                if (planned[j] < 0) continue;
+                const hipError_t q = hipEventQuery(ctx->lanes[j]->ev_plan);
+                if (q == hipSuccess) pick = static_cast<long>(j);
+                else if (q != hipErrorNotReady) SVDI_HIP(q);
             }
-            for (size_t j = 0; j < n; ++j) {
-                svdi_context* lane = ctx->lanes[j].get();
-                SVDI_HIP(hipEventSynchronize(lane->ev_plan));
-                image_enqueue(lane, in[r0 + j], o, cs[r0 + j], images[r0 + j]);
+            if (pick < 0) {                                     // none back yet: the oldest plan
+                for (size_t j = 0; j < L; ++j)
+                    if (planned[j] >= 0 && (pick < 0 || plan_seq[j] < plan_seq[static_cast<size_t>(pick)]))
+                        pick = static_cast<long>(j);
+                if (pick < 0) break;                            // every lane is done
+                SVDI_HIP(hipEventSynchronize(ctx->lanes[static_cast<size_t>(pick)]->ev_plan));
             }
+            const size_t j = static_cast<size_t>(pick);
+            const size_t i = static_cast<size_t>(planned[j]);
+            image_enqueue(ctx->lanes[j].get(), in[i], o, cs[i], images[i]);
+            planned[j] = -1;
+            begin_next(j);
         }
         for (size_t j = 0; j < L; ++j) {
             svdi_context* lane = ctx->lanes[j].get();
@@ -861,13 +905,37 @@ int svdi_image_snapshot_noalloc(svdi_context* ctx, const svdi_snapshot* snap, fl
 int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
                               float* const* images, size_t image_size, float cell_size,
                               const svdi_options* opts, svdi_plan_info* infos) {
-    return svdi::image_batch(ctx, snaps, count, images, image_size, cell_size, opts, infos, true);
+    std::vector<size_t> sizes;
+    std::vector<float> cells;
+    try {
+        sizes.assign(count, image_size);
+        cells.assign(count, cell_size);
+    } catch (...) {
+        set_error("host allocation failed");
+        return SVDI_ERR_OUT_OF_MEMORY;
+    }
+    return svdi::image_batch(ctx, snaps, count, images, sizes.data(), cells.data(), opts, infos, true);
 }
 
 int svdi_image_snapshot_batch_async(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,
                                     float* const* images, size_t image_size, float cell_size,
                                     const svdi_options* opts, svdi_plan_info* infos) {
-    return svdi::image_batch(ctx, snaps, count, images, image_size, cell_size, opts, infos, false);
+    std::vector<size_t> sizes;
+    std::vector<float> cells;
+    try {
+        sizes.assign(count, image_size);
+        cells.assign(count, cell_size);
+    } catch (...) {
+        set_error("host allocation failed");
+        return SVDI_ERR_OUT_OF_MEMORY;
+    }
+    return svdi::image_batch(ctx, snaps, count, images, sizes.data(), cells.data(), opts, infos, false);
+}
+
+int svdi_image_snapshots(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
+                         const size_t* image_sizes, const float* cell_sizes, const svdi_options* opts,
+                         svdi_plan_info* infos, int wait) {
+    return svdi::image_batch(ctx, snaps, count, images, image_sizes, cell_sizes, opts, infos, wait != 0);
 }
 
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

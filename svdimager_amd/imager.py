@@ -141,21 +141,25 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
     return image, info.as_dict()
 
 
-def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size: int,
-                         cell_size: float, images=None, wait: bool = True,
-                         **options) -> Tuple[List, List[Dict]]:
+def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size, cell_size,
+                         images=None, wait: bool = True, **options) -> Tuple[List, List[Dict]]:
     """Image several device-resident snapshots in one call (concurrent lanes); returns (images,
-    plan dicts).  `images`: optional list of (N, N) float32 device tensors.  wait=False returns
-    once the work is enqueued (svdi_image_snapshot_batch_async): the images are complete when the
-    current stream reaches this point, and the snapshots must stay alive until then."""
+    plan dicts).  image_size / cell_size: one value for all, or one per snapshot (mixed sizes:
+    svdi_image_snapshots).  `images`: optional list of (N_i, N_i) float32 device tensors.
+    wait=False returns once the work is enqueued: the images are complete when the current
+    stream reaches this point, and the snapshots must stay alive until then."""
     torch = _torch()
     n = len(dsnaps)
     if n == 0:
         return [], []
     dev = dsnaps[0].tensors["uvw"].device
-    N = int(image_size)
+    mixed = isinstance(image_size, (list, tuple)) or isinstance(cell_size, (list, tuple))
+    sizes = [int(x) for x in image_size] if isinstance(image_size, (list, tuple)) else [int(image_size)] * n
+    cells = [float(x) for x in cell_size] if isinstance(cell_size, (list, tuple)) else [float(cell_size)] * n
+    if len(sizes) != n or len(cells) != n:
+        raise ValueError("one image size and cell per snapshot")
     if images is None:
-        images = [torch.empty((N, N), dtype=torch.float32, device=dev) for _ in range(n)]
+        images = [torch.empty((N, N), dtype=torch.float32, device=dev) for N in sizes]
     if len(images) != n:
         raise ValueError("one image per snapshot")
     opts = default_options(**options)
@@ -163,8 +167,14 @@ def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size:
     structs = (Snapshot * n)(*[d.struct() for d in dsnaps])
     ptrs = (ctypes.c_void_p * n)(*[_ptr(im) for im in images])
     _bind_stream(ctx, dev)
-    fn = lib().svdi_image_snapshot_batch if wait else lib().svdi_image_snapshot_batch_async
-    check(fn(ctx.handle, structs, n, ptrs, N, float(cell_size), ctypes.byref(opts), infos))
+    if mixed:
+        csz = (ctypes.c_size_t * n)(*sizes)
+        ccell = (ctypes.c_float * n)(*cells)
+        check(lib().svdi_image_snapshots(ctx.handle, structs, n, ptrs, csz, ccell, ctypes.byref(opts),
+                                         infos, 1 if wait else 0))
+    else:
+        fn = lib().svdi_image_snapshot_batch if wait else lib().svdi_image_snapshot_batch_async
+        check(fn(ctx.handle, structs, n, ptrs, sizes[0], cells[0], ctypes.byref(opts), infos))
     return images, [infos[i].as_dict() for i in range(n)]
 
 

@@ -65,6 +65,14 @@ def parity_errors(img, ref):
 # on the device vs glibc on the host, a few ulp apart, which moves each splat weight by ~1e-5.
 DIRTY_CENTRAL = 5e-5
 
+# Archive dirty image (after |.| and the degree-trig SIN splat) against the oracle.  The
+# archive's p2p takes cos of an FP32 angle in degrees near 90 degrees: one ulp of that angle
+# (7.6e-6 degree) moves a splat target by about 1e-3 pixel at this cell size, host and device
+# libm may round it differently, and a bilinear splat moved by d pixels changes a pixel by up to
+# about 4 d of the values splatted onto it.  Measured on MI355X: 3.0e-3 over the image, 2.7e-3
+# over the central half, i.e. uniform (the signed image before the splat agrees to 1e-5).
+ARCHIVE_DIRTY = 5e-3
+
 
 def assert_parity(img, ref, what="", whole=1e-4, central=1e-5):
     e, c = parity_errors(img, ref)

@@ -12,20 +12,13 @@ import numpy as np
 import pytest
 import torch
 
-from conftest import DIRTY_CENTRAL, announce, assert_parity
+from conftest import ARCHIVE_DIRTY, DIRTY_CENTRAL, announce, assert_parity
 from oracle import oracle as O
 import svdimager_amd as S
 
 pytestmark = pytest.mark.gpu
 
 
-# Archive dirty image (after |.| and the degree-trig SIN splat) against the oracle.  The
-# archive's p2p takes cos of an FP32 angle in degrees near 90 degrees: one ulp of that angle
-# (7.6e-6 degree) moves a splat target by about 1e-3 pixel at this cell size, host and device
-# libm may round it differently, and a bilinear splat moved by d pixels changes a pixel by up to
-# about 4 d of the values splatted onto it.  Measured on MI355X: 3.0e-3 over the image, 2.7e-3
-# over the central half, i.e. uniform (the signed image before the splat agrees to 1e-5).
-ARCHIVE_DIRTY = 5e-3
 
 
 def _peak(d):
