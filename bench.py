@@ -62,24 +62,48 @@ def algorithmic(cfg, S_eff, K, P):
     }
 
 
-def cpu_baseline(snap, cfg, n_snapshots):
-    """The oracle's FP32 restatement of the reference (OpenMP) on this host."""
+def host_cpu():
+    """CPU model and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
+
+
+def cpu_baseline(snap, cfg, n_snapshots, small=None):
+    """The oracle's FP32 restatement of the reference (OpenMP) on this host: `n_snapshots` runs
+    of the workload's snapshot (median), and the small CPU-runnable workload in full."""
     import numpy as np
     from oracle import oracle as O
     threads = O.lib().orc_num_threads()
-    times = []
-    for _ in range(n_snapshots):
+
+    def run(s, c):
         t0 = time.perf_counter()
-        pre = O.preprocess_ref32(snap)
-        O.fipipe_device_ref32(pre["bin"], pre["vin"], pre["vis_re"], pre["vis_im"],
-                              cfg.image_size, cfg.cell)
-        times.append(time.perf_counter() - t0)
+        pre = O.preprocess_ref32(s)
+        O.fipipe_device_ref32(pre["bin"], pre["vin"], pre["vis_re"], pre["vis_im"], c.image_size, c.cell)
+        return time.perf_counter() - t0
+
+    times = [run(snap, cfg) for _ in range(n_snapshots)]
     t = float(np.median(times))
-    return {"value": 1.0 / t, "unit": "snapshots/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n_snapshots} full snapshot(s) of the same workload "
-                      f"({cfg.num_samples} samples, {cfg.image_size}^2), median "
-                      f"{t:.2f} s each: FP32 preprocess + gridding + FP64 FFT + image pass",
-            "mvis_per_s": cfg.num_samples / t / 1e6}
+    out = {"value": 1.0 / t, "unit": "snapshots/s", "cores": int(threads), "kind": "port",
+           "sample": f"{n_snapshots} full snapshot(s) of the same workload "
+                     f"({cfg.num_samples} samples, {cfg.image_size}^2), median "
+                     f"{t:.2f} s each (runs {', '.join(f'{x:.2f}' for x in times)} s): FP32 "
+                     f"preprocess + gridding + FP64 FFT + image pass",
+           "mvis_per_s": cfg.num_samples / t / 1e6, **host_cpu()}
+    if small is not None:
+        s_snap, s_cfg = small
+        ts = [run(s_snap, s_cfg) for _ in range(3)]
+        out["small_workload"] = {"workload": s_cfg.name, "samples": s_cfg.num_samples,
+                                 "image_size": s_cfg.image_size, "seconds_median": float(np.median(ts)),
+                                 "snapshots_per_s": 1.0 / float(np.median(ts))}
+    return out
 
 
 def measured_traffic(kernel, workload):
@@ -477,7 +501,8 @@ def main():
                     help="skip timing the plain FP32 tiled imager")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-samples", type=int, default=1)
+    ap.add_argument("--cpu-samples", type=int, default=3,
+                    help="snapshots of the workload timed by the CPU baseline (median)")
     ap.add_argument("--batch", type=int, default=8,
                     help="snapshots per batched call (consecutive 10 s snapshots of the same "
                          "observation); 0 = time single-snapshot calls only")
@@ -624,15 +649,21 @@ def main():
         "path_roofline_cap_snapshots_per_s": HBM_PEAK_GBS * 1e9 / hbm_bytes,
         "roofline": roofline,
     }
-    if not args.no_compare and args.gridder != "tiled":
-        el2, st2, _ = time_gridder(S, ctx, dsnap, cfg, image, S.GRIDDER_TILED, args.steps,
-                                   args.warmup, "plain-fp32")
-        out["plain_fp32"] = {"snapshots_per_s": args.steps / el2,
-                             "ms_per_step": el2 * 1e3 / args.steps,
-                             "mvis_per_s": cfg.num_samples * args.steps / el2 / 1e6,
-                             "stages_ms": st2}
-        out["speedup_vs_plain_fp32"] = snaps_per_s / (args.steps / el2)
-        out["speedup_single_vs_plain_fp32"] = single_snaps_per_s / (args.steps / el2)
+    if not args.no_compare and args.gridder not in ("tiled", "atomic"):
+        # The plain FP32 imagers: the FP32 tile gridder, and the reference's own schedule (one
+        # thread per sample, a device atomic per cell and term, SVDIntegratedImager.cu:1276-1291).
+        # plain_fp32 is the faster of the two.
+        plain = {}
+        for name, g, steps in (("tiled", S.GRIDDER_TILED, args.steps),
+                               ("atomic", S.GRIDDER_ATOMIC, max(1, min(args.steps, 5)))):
+            el2, st2, _ = time_gridder(S, ctx, dsnap, cfg, image, g, steps, min(args.warmup, 2),
+                                       f"plain-fp32-{name}")
+            plain[name] = {"snapshots_per_s": steps / el2, "ms_per_step": el2 * 1e3 / steps,
+                           "mvis_per_s": cfg.num_samples * steps / el2 / 1e6, "stages_ms": st2}
+        best = max(plain, key=lambda k: plain[k]["snapshots_per_s"])
+        out["plain_fp32"] = dict(plain[best], gridder=best, both=plain)
+        out["speedup_vs_plain_fp32"] = snaps_per_s / plain[best]["snapshots_per_s"]
+        out["speedup_single_vs_plain_fp32"] = single_snaps_per_s / plain[best]["snapshots_per_s"]
     if not args.no_accuracy:
         log("bench: accuracy against the FP64 DFT ...")
         gl = imager_variants(S, (args.gridder, {"gridder": gridder, **gopts}))
@@ -640,7 +671,8 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline ...")
         host = {k: snap[k] for k in snap}
-        out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_samples)
+        small = (synth.make_snapshot(synth.SMALL), synth.SMALL)
+        out["cpu_baseline"] = cpu_baseline(host, cfg, args.cpu_samples, small)
     print(json.dumps(out), flush=True)
     ctx.close()
 

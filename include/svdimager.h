@@ -222,6 +222,16 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
                      size_t num_baselines, size_t image_size, float cell_size,
                      const svdi_options* opts, svdi_plan_info* info);
 
+/* The library's batched 2D inverse FFT on its own (the transform that replaces
+ * cufftExecC2C(..., CUFFT_INVERSE), SVDIntegratedImager.cu:1658-1676,1713), for cross-checking
+ * against another FFT: planes (num_planes x grid_size x grid_size complex64, device, row-major)
+ * -> out (num_planes x rows x cols complex64, device) = rows [row_lo, row_lo + rows) and columns
+ * [col_lo, col_lo + cols) of each unnormalised inverse transform, sum over (a, b) of
+ * plane[a][b] e^{+2 pi i (a r + b c) / grid_size}.  The imager keeps the central image_size
+ * rows and columns of an even grid, the whole plane of an odd one.  Returns when done. */
+int svdi_fft2_inverse(svdi_context* ctx, const void* planes, size_t num_planes, size_t grid_size,
+                      size_t row_lo, size_t rows, size_t col_lo, size_t cols, void* out);
+
 /* FITS primary-HDU images, BITPIX -32 (write_fits_2d, SVDIntegratedImager.cu:153-176; the
  * archive's read_fits_image / write_fits_image, pipe_interface.cpp:12-137).  write replaces
  * an existing file.  read returns 1-D (naxes[1] = 1) or 2-D images as float; free with

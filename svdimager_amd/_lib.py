@@ -29,7 +29,8 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot", "svdi_fipipe_host", "svdi_fits_write", "svdi_fits_read", "svdi_free",
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
-    "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "_Z6FIpipePfS_S_S_S_mmf",
+    "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
+    "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
 
@@ -135,6 +136,7 @@ def lib():
                                                     ctypes.POINTER(ctypes.c_float),
                                                     ctypes.POINTER(Options),
                                                     ctypes.POINTER(PlanInfo), ctypes.c_int]),
+            "svdi_fft2_inverse": (ctypes.c_int, [vp, vp, sz, sz, sz, sz, sz, sz, vp]),
             "svdi_fipipe_host": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, sz, sz, f,
                                                 ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
             "svdi_fits_write": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_long, ctypes.c_long]),

@@ -973,6 +973,26 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
     });
 }
 
+int svdi_fft2_inverse(svdi_context* ctx, const void* planes, size_t num_planes, size_t grid_size,
+                      size_t row_lo, size_t rows, size_t col_lo, size_t cols, void* out) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        check_device_pointer(planes, "planes");
+        check_device_pointer(out, "out");
+        if (num_planes == 0 || grid_size == 0 || rows == 0 || cols == 0) throw ArgError("empty transform");
+        if (grid_size > 69510) throw ArgError("grid_size too large");
+        if (row_lo + rows > grid_size || col_lo + cols > grid_size) throw ArgError("window outside the plane");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        begin_call(ctx);
+        float2* rowfft = static_cast<float2*>(
+            ctx->ws.get(Workspace::kRowFft, num_planes * grid_size * cols * sizeof(float2)));
+        launch_fft2_inverse(ctx, static_cast<const float2*>(planes), num_planes, grid_size, row_lo, rows, col_lo,
+                            cols, rowfft, static_cast<float2*>(out));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int svdi_set_profiling(svdi_context* ctx, int enable) {
     return guarded([&] {
         if (!ctx) throw ArgError("ctx is null");

@@ -178,6 +178,23 @@ def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size,
     return images, [infos[i].as_dict() for i in range(n)]
 
 
+def fft2_inverse(ctx: Context, planes, row_lo: int = 0, rows: Optional[int] = None,
+                 col_lo: int = 0, cols: Optional[int] = None):
+    """The library's unnormalised 2D inverse FFT of (P, G, G) complex64 device planes, window
+    rows [row_lo, row_lo + rows) x columns [col_lo, col_lo + cols); returns (P, rows, cols)."""
+    torch = _torch()
+    P, G, G2 = planes.shape
+    if G != G2 or planes.dtype != torch.complex64 or not planes.is_contiguous():
+        raise ValueError("planes: contiguous (P, G, G) complex64")
+    rows = G - row_lo if rows is None else int(rows)
+    cols = G - col_lo if cols is None else int(cols)
+    out = torch.empty((P, rows, cols), dtype=torch.complex64, device=planes.device)
+    _bind_stream(ctx, planes.device)
+    check(lib().svdi_fft2_inverse(ctx.handle, _ptr(planes), P, G, int(row_lo), rows, int(col_lo), cols,
+                                  _ptr(out)))
+    return out
+
+
 def fipipe_host(ctx: Context, vis_re: np.ndarray, vis_im: np.ndarray, bin_: np.ndarray,
                 vin: np.ndarray, image_size: int, cell_size: float, **options):
     """FIpipe semantics on host numpy arrays through a given context; returns (image, plan)."""
