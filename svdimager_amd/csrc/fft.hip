@@ -346,15 +346,10 @@ __device__ __forceinline__ void wave_lds_fence() {
 }
 
 // kTwStride: the table holds e^{2 pi i t / (1536 kTwStride)} (W_1536^m = tw[kTwStride m]).
-template <int kTwStride = 1, class Load, class Store>
-__device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restrict__ tw, int lane,
-                                             Load&& load, Store&& store) {
-    float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-#pragma unroll
-        for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
-    wave_lds_fence();                              // the tile may be where the loads came from
+// fft1536_wave_regs takes the lane's inputs already in registers: y[v][u] = x[64 (3u + v) + l].
+template <int kTwStride = 1, class Store>
+__device__ __forceinline__ void fft1536_wave_regs(float2 (&y)[3][8], float2* tile, const float2* __restrict__ tw,
+                                                  int lane, Store&& store) {
     // 24-point DFT over n1 = 3u + v, k1 = s + 8t: 8-point DFTs over u, times W_24^{v s}, then
     // 3-point DFTs over v.  W_24^m = tw[64 m].
 #pragma unroll
@@ -411,6 +406,18 @@ __device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restr
 #pragma unroll
         for (int d = 0; d < 8; ++d) store(p + 192 * d, z[j][d]);
     }
+}
+
+template <int kTwStride = 1, class Load, class Store>
+__device__ __forceinline__ void fft1536_wave(float2* tile, const float2* __restrict__ tw, int lane,
+                                             Load&& load, Store&& store) {
+    float2 y[3][8];                                // y[v][u] = x[64 (3u + v) + l]
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) y[v][u] = load(64 * (3 * u + v) + lane);
+    wave_lds_fence();                              // the tile may be where the loads came from
+    fft1536_wave_regs<kTwStride>(y, tile, tw, lane, store);
 }
 
 // Row pass for G = 1536: one wavefront per row of the P x G rows; loads stride 64 per lane and
@@ -542,6 +549,112 @@ k_fft_rows_3072(const float2* __restrict__ in, float2* __restrict__ out, const f
             dst_a[k1] = cadd(xn1, xm1);
             dst_b[k1] = cmul(csub(xn1, xm1), wn);
         }
+    }
+}
+
+// Row pass for G = 6144 = 4 x 1536: one workgroup per row, wavefront j transforming the samples
+// x[4 m + j] (m < 1536, the 1536-point wavefront transform with a stride-4 twiddle table) into
+// F_j, then a radix-4 step: with W = e^{2 pi i / 6144} and k = k0 + 1536 q (k0 < 1536, q < 4),
+// X[k] = sum_j W^{j k} F_j[k0] = sum_j i^{j q} (W^{j k0} F_j[k0]).  Only the columns
+// col_lo .. col_lo + cols - 1 are stored; the column pass is the general one.
+__global__ void __launch_bounds__(256)
+k_fft_rows_6144(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int col_lo, int cols) {
+    constexpr int G = 6144, H = 1536;
+    __shared__ float2 tile_all[4][24 * kTileStride];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const size_t row = blockIdx.x;
+    const float2* src = in + row * G;
+    float2* tile = tile_all[wave];
+    fft1536_wave<4>(tile, tw, lane, [&](int m) { return src[4 * m + wave]; },
+                    [&](int k, float2 v) { tile[k] = v; });
+    __syncthreads();
+    float2* dst = out + row * static_cast<size_t>(cols) - col_lo;
+    for (int k0 = threadIdx.x; k0 < H; k0 += 256) {
+        const float2 g0 = tile_all[0][k0];
+        const float2 g1 = cmul(tile_all[1][k0], tw[k0]);
+        const float2 g2 = cmul(tile_all[2][k0], tw[2 * k0]);
+        const float2 g3 = cmul(tile_all[3][k0], tw[3 * k0]);
+        const float2 s02 = cadd(g0, g2), d02 = csub(g0, g2);
+        const float2 s13 = cadd(g1, g3), d13 = csub(g1, g3);
+        const float2 id13 = make_float2(-d13.y, d13.x);          // i (g1 - g3)
+        const float2 x[4] = {cadd(s02, s13), cadd(d02, id13), csub(s02, s13), csub(d02, id13)};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + H * q;
+            if (k >= col_lo && k < col_lo + cols) dst[k] = x[q];
+        }
+    }
+}
+
+// Column pass for G = 6144 = 4 x 1536: decimation in frequency along the column, then four
+// 1536-point transforms.  With W = e^{2 pi i / 6144} and n < 1536,
+//   y_q[n] = W^{n q} sum_m i^{m q} x[n + 1536 m],   X[4 k + q] = sum_n y_q[n] W_1536^{n k}.
+// One workgroup per (plane, strip of kColStrip6 columns): the strip is read into LDS (column c's
+// rows 1536 m .. 1536 m + 1535 in its region m), wavefront (c, q) forms its 24 values of y_q in
+// registers from the four regions, and after a workgroup barrier transforms them in place in
+// region q (the region doubles as the exchange tile); the output rows of the window are written
+// back as row segments of the strip.
+constexpr int kColStrip6 = 3;
+constexpr int kCols6Threads = 64 * 4 * kColStrip6;
+constexpr size_t kCols6Lds = sizeof(float2) * kColStrip6 * 4 * kColLen;
+
+__global__ void __launch_bounds__(kCols6Threads)
+k_fft_cols_6144(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+                int cols, int row_lo, int rows, int strips) {
+    constexpr int G = 6144, H = 1536;
+    constexpr int kRowsPerPass = kCols6Threads / kColStrip6;      // 256
+    constexpr int kPasses = G / kRowsPerPass;                      // 24
+    extern __shared__ float2 lds6[];                               // [column][region m][kColLen]
+    const int tile = xcd_local_tile(blockIdx.x, gridDim.x);
+    const int plane = tile / strips;
+    const int c0 = (tile - plane * strips) * kColStrip6;
+    const int nc = min(kColStrip6, cols - c0);
+    {
+        const float2* src = in + static_cast<size_t>(plane) * G * cols + c0;
+        const int c = threadIdx.x % kColStrip6, r0 = threadIdx.x / kColStrip6;
+        if (c < nc) {
+            float2 v[kPasses];
+#pragma unroll
+            for (int j = 0; j < kPasses; ++j) v[j] = src[static_cast<size_t>(r0 + kRowsPerPass * j) * cols + c];
+#pragma unroll
+            for (int j = 0; j < kPasses; ++j) {
+                const int r = r0 + kRowsPerPass * j;
+                lds6[(c * 4 + r / H) * kColLen + r % H] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = wave >> 2, q = wave & 3;
+    float2 y[3][8];
+    if (c < nc) {
+        const float2* col = lds6 + c * 4 * kColLen;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const int n = 64 * (3 * u + v) + lane;
+                const float2 x0 = col[n], x1 = col[kColLen + n], x2 = col[2 * kColLen + n], x3 = col[3 * kColLen + n];
+                float2 t;
+                if (q == 0) t = cadd(cadd(x0, x2), cadd(x1, x3));
+                else if (q == 1) t = cadd(csub(x0, x2), mul_i(csub(x1, x3)));
+                else if (q == 2) t = csub(cadd(x0, x2), cadd(x1, x3));
+                else t = csub(csub(x0, x2), mul_i(csub(x1, x3)));
+                y[v][u] = q == 0 ? t : cmul(t, tw[n * q]);
+            }
+    }
+    __syncthreads();                               // every region read before any is rewritten
+    if (c < nc) {
+        float2* region = lds6 + (c * 4 + q) * kColLen;
+        fft1536_wave_regs<4>(y, region, tw, lane, [&](int k, float2 v) { region[k] = v; });
+    }
+    __syncthreads();
+    float2* dst = out + static_cast<size_t>(plane) * rows * cols + c0;
+    for (int t = threadIdx.x; t < rows * kColStrip6; t += kCols6Threads) {
+        const int r = t / kColStrip6, cc = t - r * kColStrip6;
+        const int gr = row_lo + r;                                 // output row 4 k + q
+        if (cc < nc) dst[static_cast<size_t>(r) * cols + cc] = lds6[(cc * 4 + (gr & 3)) * kColLen + (gr >> 2)];
     }
 }
 
@@ -705,7 +818,10 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
     auto cols_fn = inplace ? &k_fft_cols<true> : &k_fft_cols<false>;
     allow_dynamic_lds(reinterpret_cast<const void*>(rows_fn), lds);
     allow_dynamic_lds(reinterpret_cast<const void*>(cols_fn), lds);
-    if (G == 3072) {
+    if (G == 6144) {
+        k_fft_rows_6144<<<static_cast<unsigned>(P * G), 256, 0, ctx->stream>>>(
+            planes, row_out, tw, static_cast<int>(col_lo), static_cast<int>(cols));
+    } else if (G == 3072) {
         k_fft_rows_3072<<<static_cast<unsigned>(P * 1536), 256, 0, ctx->stream>>>(
             planes, row_out, tw, static_cast<int>(col_lo), static_cast<int>(cols));
     } else if (G == 1536) {
@@ -719,7 +835,12 @@ void launch_fft2_inverse(svdi_context* ctx, const float2* planes, size_t P, size
                                                    static_cast<int>(cols));
     }
     SVDI_LAUNCH_CHECK();
-    if (G == 3072) {
+    if (G == 6144) {
+        const int strips = static_cast<int>((cols + kColStrip6 - 1) / kColStrip6);
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_fft_cols_6144), kCols6Lds);
+        k_fft_cols_6144<<<static_cast<unsigned>(strips * P), kCols6Threads, kCols6Lds, ctx->stream>>>(
+            row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
+    } else if (G == 3072) {
         const int strips = static_cast<int>((cols + kColStrip - 1) / kColStrip);
         k_fft_cols_1536<2><<<static_cast<unsigned>(strips * 2 * P), 64 * kColStrip, 0, ctx->stream>>>(
             row_out, out, tw, static_cast<int>(cols), static_cast<int>(row_lo), static_cast<int>(rows), strips);
