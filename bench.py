@@ -236,7 +236,7 @@ def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64, all_pixels=True):
     return out
 
 
-PRECISIONS = {"default": 0, "split3": 1, "splitc": 2, "single": 3}
+PRECISIONS = {"default": 0, "split3": 1, "splitc": 2, "single": 3, "bf16": 4}
 
 
 def imager_variants(S, headline=None):
@@ -245,7 +245,7 @@ def imager_variants(S, headline=None):
     out = {}
     if headline is not None:
         out[headline[0]] = headline[1]
-    for name in ("split3", "splitc", "single"):
+    for name in ("split3", "splitc", "single", "bf16"):
         key = f"mfma-{name}"
         if key not in out:
             out[key] = {"gridder": S.GRIDDER_MFMA, "mfma_precision": PRECISIONS[name]}

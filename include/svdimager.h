@@ -82,7 +82,9 @@ typedef enum svdi_mfma_precision {
     SVDI_PRECISION_SPLIT3 = 1,      /* C and both ES factors in f16 hi + lo, B's rounding kept:
                                        3 MFMAs per window row, ~22 significant bits per product */
     SVDI_PRECISION_SPLITC = 2,      /* C in f16 hi + lo, B a single f16: 2 MFMAs per row */
-    SVDI_PRECISION_SINGLE = 3       /* C and B single f16: 1 MFMA per row */
+    SVDI_PRECISION_SINGLE = 3,      /* C and B single f16: 1 MFMA per row */
+    SVDI_PRECISION_BF16 = 4         /* C and B in bf16 hi + lo (B's product formed in FP32):
+                                       3 bf16 MFMAs per row, ~16 significant bits per product */
 } svdi_mfma_precision;
 
 typedef struct svdi_options {

@@ -6,7 +6,7 @@ snapshots.  Importing the package does not load the library; the first imaging c
 raises if the library has not been built.
 """
 from ._lib import (GRIDDER_ATOMIC, GRIDDER_AUTO, GRIDDER_MFMA, GRIDDER_TILED, OUTPUT_DIRTY, OUTPUT_SIGNED,
-                   PRECISION_DEFAULT, PRECISION_SINGLE, PRECISION_SPLIT3, PRECISION_SPLITC,
+                   PRECISION_BF16, PRECISION_DEFAULT, PRECISION_SINGLE, PRECISION_SPLIT3, PRECISION_SPLITC,
                    SEMANTICS_ARCHIVE, SEMANTICS_INTEGRATED, STAGES, Context, SvdiError)
 from .imager import (DeviceSnapshot, FIpipe, FIpipeDevice, fft2_inverse, fipipe_host, image_snapshot,
                      image_snapshot_batch,
@@ -18,6 +18,6 @@ __all__ = [
     "image_snapshot", "image_snapshot_batch", "preprocess_measurement_set_gpu", "read_fits_image", "snapshot_geometry",
     "write_fits_2d",
     "GRIDDER_ATOMIC", "GRIDDER_AUTO", "GRIDDER_MFMA", "GRIDDER_TILED", "OUTPUT_DIRTY", "OUTPUT_SIGNED",
-    "PRECISION_DEFAULT", "PRECISION_SINGLE", "PRECISION_SPLIT3", "PRECISION_SPLITC",
+    "PRECISION_BF16", "PRECISION_DEFAULT", "PRECISION_SINGLE", "PRECISION_SPLIT3", "PRECISION_SPLITC",
     "SEMANTICS_ARCHIVE", "SEMANTICS_INTEGRATED", "STAGES",
 ]

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(PKG, "libsvdimager.so")
 
 SVDI_OK = 0
 GRIDDER_AUTO, GRIDDER_ATOMIC, GRIDDER_TILED, GRIDDER_MFMA = 0, 1, 2, 3
-PRECISION_DEFAULT, PRECISION_SPLIT3, PRECISION_SPLITC, PRECISION_SINGLE = 0, 1, 2, 3
+PRECISION_DEFAULT, PRECISION_SPLIT3, PRECISION_SPLITC, PRECISION_SINGLE, PRECISION_BF16 = 0, 1, 2, 3, 4
 OUTPUT_DIRTY, OUTPUT_SIGNED = 0, 1
 SEMANTICS_INTEGRATED, SEMANTICS_ARCHIVE = 0, 1
 

@@ -186,7 +186,7 @@ void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     if (!(cell > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
     if (in.num_samples == 0) throw ArgError("Measurement set is empty");
     if (o.num_terms < 0 || o.num_terms > kMaxTerms) throw ArgError("num_terms must be in 0..32");
-    if (o.mfma_precision < SVDI_PRECISION_DEFAULT || o.mfma_precision > SVDI_PRECISION_SINGLE)
+    if (o.mfma_precision < SVDI_PRECISION_DEFAULT || o.mfma_precision > SVDI_PRECISION_BF16)
         throw ArgError("unknown mfma_precision");
     if (N > 46340) throw ArgError("Image_Size too large");
     if (o.gridder != SVDI_GRIDDER_AUTO && o.gridder != SVDI_GRIDDER_ATOMIC && o.gridder != SVDI_GRIDDER_TILED &&
@@ -295,6 +295,7 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     gp.scale_f64 = static_cast<double>(cell) * static_cast<double>(G);
     gp.prec = o.mfma_precision == SVDI_PRECISION_SPLITC   ? 1
               : o.mfma_precision == SVDI_PRECISION_SINGLE ? 2
+              : o.mfma_precision == SVDI_PRECISION_BF16   ? 3
                                                           : 0;
     const float* v = ctx->pinned->vin;
     const float det = std::fabs(v[0] * v[4] - v[1] * v[3]);

@@ -63,19 +63,28 @@ enum WinPrec : int {
                        // hi*hi + lo*hi + hi*lo, ~22 significant bits per product, 3 MFMAs per row
     kPrecSplitC = 1,   // c in f16 hi + lo, B_a = f16(f16(ku) f16(kv)): 2 MFMAs per row
     kPrecSingle = 2,   // c and B_a single f16: 1 MFMA per row
+    kPrecBf16 = 3,     // c and B_a in bf16 hi + lo, B_a's product formed in FP32: hi*hi + lo*hi +
+                       // hi*lo on the bf16 MFMA, ~16 significant bits per product
 };
+constexpr int kStrideF = 36;                               // floats per FP32 ES staging row
+constexpr int kPlaneFloats = kStageRows * kStrideF;        // one FP32 ES plane
 template <int P>
 struct WinTraits {
-    static constexpr bool kEsLo = P == kPrecSplit3;       // ES factors staged as hi + lo planes
-    static constexpr bool kCLo = P != kPrecSingle;        // c split into hi + lo
+    static constexpr bool kBf16 = P == kPrecBf16;          // FP32 ES planes, bf16 operands
+    static constexpr bool kEsLo = P == kPrecSplit3;        // ES factors staged as f16 hi + lo planes
+    static constexpr bool kCLo = P != kPrecSingle;         // c split into hi + lo
+    static constexpr bool kDither = P == kPrecSplitC || P == kPrecSingle;   // single-f16 ES product
     static constexpr int kPlanes = kEsLo ? 4 : 2;
-    // Per wavefront: the ES planes (ku hi [lo], kv hi [lo], f16), then c[16] (f32).
-    static constexpr int kStageFloats = (kPlanes * kPlaneHalves) / 2 + kWin * kStride;
-    // Wavefronts per workgroup (one workgroup per CU): as many as the staging (and, at four per
-    // SIMD, 128 registers) allow.
-    static constexpr int kWaves = kEsLo ? 12 : 16;
+    // Per wavefront: the ES planes (ku hi [lo], kv hi [lo], f16; or ku, kv in FP32), then c[16]
+    // (f32).
+    static constexpr int kStageFloats =
+        (kBf16 ? 2 * kPlaneFloats : (kPlanes * kPlaneHalves) / 2) + kWin * kStride;
+    // Wavefronts per workgroup (one workgroup per CU): as many as the staging allows.  At 12 the
+    // kernel keeps its ~150 registers without spills; 16 (four per SIMD) would need 128 and a
+    // staging area without row padding, which measured slower (542 against 500 us for split3).
+    static constexpr int kWaves = (kEsLo || kBf16) ? 12 : 16;
     // Accumulator slots of the workgroup reduction (within the staging area).
-    static constexpr int kSlots = kEsLo ? 6 : 4;
+    static constexpr int kSlots = (kEsLo || kBf16) ? 6 : 4;
     static_assert(kWaves % kSlots == 0, "reduction rounds");
     static_assert(kSlots * kWin * 64 * 4 <= kStageFloats * kWaves, "reduction slots exceed the staging area");
 };
@@ -88,6 +97,8 @@ typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 struct WinGeom {
     int nw;          // windows per grid axis
@@ -551,6 +562,35 @@ __device__ __forceinline__ void split_hilo(const float4& x0, const float4& x1, h
     }
 }
 
+// x = hi + lo in bf16: hi = RN(x), lo = RN(x - hi) (the difference exact in FP32), so
+// |x - hi - lo| <= 2^-9 |x - hi| <= 2^-18 |x|; two values per packed conversion.
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+        const bf16x2 h = __builtin_convertvector((float2v){x[i], x[i + 1]}, bf16x2);
+        const bf16x2 l = __builtin_convertvector((float2v){x[i] - static_cast<float>(h[0]),
+                                                           x[i + 1] - static_cast<float>(h[1])}, bf16x2);
+        hi[i] = h[0];
+        hi[i + 1] = h[1];
+        lo[i] = l[0];
+        lo[i + 1] = l[1];
+    }
+}
+
+// One window row a with bf16 operands: B_a = ku_a kv formed in FP32 and split into bf16 hi + lo,
+// then D_a += C_hi B_hi + C_lo B_hi + C_hi B_lo.
+__device__ __forceinline__ void window_row_bf16(floatx4& d, const float* __restrict__ u_row, const float (&vf)[8],
+                                                const bf16x8& a_hi, const bf16x8& a_lo) {
+    const float4 u0 = *reinterpret_cast<const float4*>(u_row), u1 = *reinterpret_cast<const float4*>(u_row + 4);
+    const float p[8] = {u0.x * vf[0], u0.y * vf[1], u0.z * vf[2], u0.w * vf[3],
+                        u1.x * vf[4], u1.y * vf[5], u1.z * vf[6], u1.w * vf[7]};
+    bf16x8 b_hi, b_lo;
+    split_bf16(p, b_hi, b_lo);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, d, 0, 0, 0);
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -594,6 +634,9 @@ struct WinLane {
     float* s_c;         // [16 m][32 samples]
     _Float16* s_kh;     // this lane's axis (ku or kv): hi and lo planes
     _Float16* s_kl;
+    float* s_kuf;       // FP32 ES planes (kPrecBf16): ku, kv, and this lane's axis
+    float* s_kvf;
+    float* s_kf;
     int lane, rl, axis, g8, n16;
     int vexp;           // log2 of the c scale
     float inv;
@@ -626,7 +669,11 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     // (padded rows take those outside the window; the ninth is nonzero only for an integer p).
     {
         const uint4 z = make_uint4(0u, 0u, 0u, 0u);
-        if constexpr (T::kEsLo) {
+        if constexpr (T::kBf16) {
+            float* zrow = L.s_kf - L.rl + (kPad + (L.rl & 15)) * kStrideF + ((L.rl & 16) ? 16 : 0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
+        } else if constexpr (T::kEsLo) {
             _Float16* zrow = ((L.rl & 16) ? L.s_kl : L.s_kh) - L.rl + (kPad + (L.rl & 15)) * kStrideH;
 #pragma unroll
             for (int k = 0; k < 4; ++k) reinterpret_cast<uint4*>(zrow)[k] = z;
@@ -642,7 +689,7 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
     // averages out over the samples of a cell; with a fixed scale the kernel's flat peak sits at
     // one f16 value for every sample and the rounding is biased (about 4e-6 of the gridded sum).
     float dith_u = 1.0f, dith_v = 1.0f;
-    if constexpr (!T::kEsLo) {
+    if constexpr (T::kDither) {
         const uint32_t h = (q.x ^ q.y ^ q.z) * 0x9E3779B1u;
         dith_u = 1.0f + static_cast<float>(h >> 24) * (1.0f / 256.0f);
         dith_v = 1.0f + static_cast<float>((h >> 16) & 0xffu) * (1.0f / 256.0f);
@@ -671,13 +718,19 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
             float o = kEsPolyOdd[j][kEsPolyDegree / 2 - 1];
 #pragma unroll
             for (int i = kEsPolyDegree / 2 - 2; i >= 0; --i) o = fmaf(o, t2, kEsPolyOdd[j][i]);
-            if constexpr (!T::kEsLo) {
+            if constexpr (T::kDither) {
                 e *= dith;
                 o *= dith;
             }
             v[j] = fmaf(t, o, e);
             v[7 - j] = fmaf(-t, o, e);
         }
+        if constexpr (T::kBf16) {
+            float* df = L.s_kf + (first + kPad) * kStrideF;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) df[j * kStrideF] = v[j];
+            df[8 * kStrideF] = fb == 0u ? kEsPolyEdge : 0.0f;
+        } else {
         _Float16* dh = L.s_kh + (first + kPad) * kStrideH;
         // hi of the pair (j, 7 - j) by one packed round-to-nearest conversion, each lo = f16(v - hi)
         // with the difference formed in FP32 (one mixed-precision fma per value).
@@ -700,11 +753,12 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
             _Float16* dl = L.s_kl + (first + kPad) * kStrideH;
             dl[8 * kStrideH] = edge ? kEsPolyEdgeLo : static_cast<_Float16>(0.0f);
         }
+        }
     }
     // c rows 2k + axis for the terms kg .. kg + 7.
     {
         float vs = valid ? rec_v(L.axis ? q.y : q.x, q.w, L.vexp) : 0.0f;
-        if constexpr (!T::kEsLo) vs *= __builtin_amdgcn_rcpf(dith_u * dith_v);
+        if constexpr (T::kDither) vs *= __builtin_amdgcn_rcpf(dith_u * dith_v);
         const float s = rec_s(q.x, q.y, q.z), s2 = 2.0f * s;
         float t0 = 1.0f, t1 = s;                      // T_k, T_{k+1}
         for (int k = 0; k < kg; ++k) {
@@ -724,6 +778,32 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
 
     // A (16 x 32): lane holds A[n16][g8 + i]; B (32 x 16): lane holds B[g8 + i][n16];
     // D (16 x 16): lane holds D[4 (lane / 16) + i][n16].
+    if constexpr (T::kBf16) {
+        bf16x8 a_hi, a_lo;
+        {
+            const float* row = L.s_c + L.n16 * kStride + L.g8;
+            const float4 x0 = *reinterpret_cast<const float4*>(row), x1 = *reinterpret_cast<const float4*>(row + 4);
+            const float x[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+            split_bf16(x, a_hi, a_lo);
+        }
+        float vf[8];
+        {
+            const float* vrow = L.s_kvf + (kPad + L.n16) * kStrideF + L.g8;
+            const float4 v0 = *reinterpret_cast<const float4*>(vrow), v1 = *reinterpret_cast<const float4*>(vrow + 4);
+            vf[0] = v0.x; vf[1] = v0.y; vf[2] = v0.z; vf[3] = v0.w;
+            vf[4] = v1.x; vf[5] = v1.y; vf[6] = v1.z; vf[7] = v1.w;
+        }
+        const float* ub = L.s_kuf + kPad * kStrideF + L.g8;
+#pragma unroll
+        for (int qd = 0; qd < 4; ++qd) {
+            if (qd >= q_lo && qd <= q_hi) {
+#pragma unroll
+                for (int a = 4 * qd; a < 4 * qd + 4; ++a) window_row_bf16(d[a], ub + a * kStrideF, vf, a_hi, a_lo);
+            }
+        }
+        wave_lds_sync();
+        return;
+    }
     half8 a_hi, a_lo;
     {
         const float* row = L.s_c + L.n16 * kStride + L.g8;
@@ -804,7 +884,13 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     const int wave = threadIdx.x >> 6;
     L.lane = threadIdx.x & 63;
     L.s_kuh = reinterpret_cast<_Float16*>(stage_all + wave * T::kStageFloats);
-    if constexpr (T::kEsLo) {
+    L.s_kuf = L.s_kvf = L.s_kf = nullptr;
+    if constexpr (T::kBf16) {
+        L.s_kuf = stage_all + wave * T::kStageFloats;
+        L.s_kvf = L.s_kuf + kPlaneFloats;
+        L.s_c = L.s_kvf + kPlaneFloats;
+        L.s_kul = L.s_kvh = L.s_kvl = L.s_kuh;
+    } else if constexpr (T::kEsLo) {
         L.s_kul = L.s_kuh + kPlaneHalves;
         L.s_kvh = L.s_kul + kPlaneHalves;
         L.s_kvl = L.s_kvh + kPlaneHalves;
@@ -821,6 +907,7 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     L.n16 = L.lane & 15;
     L.s_kh = (L.axis ? L.s_kvh : L.s_kuh) + L.rl;
     L.s_kl = (L.axis ? L.s_kvl : L.s_kul) + L.rl;
+    if constexpr (T::kBf16) L.s_kf = (L.axis ? L.s_kvf : L.s_kuf) + L.rl;
     // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
     {
         const float vmax = __uint_as_float(totals[4]);
@@ -1118,7 +1205,8 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_grid<P>), lds);
         k_win_grid<P><<<grid, T::kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
     };
-    if (gp.prec == kPrecSingle) launch(std::integral_constant<int, kPrecSingle>());
+    if (gp.prec == kPrecBf16) launch(std::integral_constant<int, kPrecBf16>());
+    else if (gp.prec == kPrecSingle) launch(std::integral_constant<int, kPrecSingle>());
     else if (gp.prec == kPrecSplitC) launch(std::integral_constant<int, kPrecSplitC>());
     else launch(std::integral_constant<int, kPrecSplit3>());
     SVDI_LAUNCH_CHECK();
