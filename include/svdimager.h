@@ -115,6 +115,9 @@ typedef struct svdi_plan_info {
     size_t records;         /* records gridded by the tiled or MFMA gridder (last slab batch) */
     float elapsed_ms;       /* device time from planning to the finished image */
     int gridder;            /* svdi_gridder that ran (AUTO resolved) */
+    float r3_centre;        /* (r3_min + r3_max) / 2 */
+    float inv_slab_half_range;
+    float vis_det;          /* |V00 V11 - V01 V10| of the rotation rows (the visibility scale) */
 } svdi_plan_info;
 
 int svdi_version(void);
@@ -188,6 +191,21 @@ int svdi_image_snapshot(svdi_context* ctx, const svdi_snapshot* snapshot, float*
 int svdi_image_snapshot_noalloc(svdi_context* ctx, const svdi_snapshot* snapshot, float* image,
                                 size_t image_size, float cell_size, const svdi_options* opts,
                                 svdi_plan_info* info);
+
+/* svdi_image_snapshot on a plan an earlier call reported for the same geometry (no counterpart in
+ * the reference; for re-imaging fixed uvw and frequencies with new visibilities, and for HIP
+ * graph capture): the planning kernels and the host's wait for their result are skipped, the
+ * plan's slabs, K and slab batches are used as given, and the call returns once the work is
+ * enqueued on the context's stream (the image is complete when the stream reaches that point).
+ * Nothing in the call waits on the host or records a memset, so it can be captured into a
+ * graph on the context's stream (svdi_set_stream): under capture the workspace must already
+ * hold the shape (one earlier call of the same sizes on the context), else the call fails with
+ * SVDI_ERR_INVALID_ARGUMENT instead of allocating.  The plan must come from a call on the same
+ * uvw, frequencies and flags, image size and cell: samples outside its r3 range would be
+ * imaged with the wrong w-terms. */
+int svdi_image_snapshot_planned(svdi_context* ctx, const svdi_snapshot* snapshot, float* image,
+                                size_t image_size, float cell_size, const svdi_options* opts,
+                                const svdi_plan_info* plan);
 
 /* Batched svdi_image_snapshot (no counterpart in the reference, which images one snapshot per
  * process): `count` device-resident snapshots imaged into images[i] (device pointers,

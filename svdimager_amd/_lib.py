@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = [
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
     "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
+    "svdi_image_snapshot_planned",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -73,7 +74,20 @@ class PlanInfo(ctypes.Structure):
         ("batch_slabs", ctypes.c_size_t), ("grid_size", ctypes.c_size_t),
         ("num_planes", ctypes.c_size_t), ("records", ctypes.c_size_t),
         ("elapsed_ms", ctypes.c_float), ("gridder", ctypes.c_int),
+        ("r3_centre", ctypes.c_float), ("inv_slab_half_range", ctypes.c_float),
+        ("vis_det", ctypes.c_float),
     ]
+
+    @classmethod
+    def from_dict(cls, d):
+        p = cls()
+        for name, _ in cls._fields_:
+            if name == "bucket_errors":
+                for i, v in enumerate(d[name]):
+                    p.bucket_errors[i] = v
+            else:
+                setattr(p, name, d[name])
+        return p
 
     def as_dict(self):
         d = {}
@@ -137,6 +151,9 @@ def lib():
                                                     ctypes.POINTER(Options),
                                                     ctypes.POINTER(PlanInfo), ctypes.c_int]),
             "svdi_fft2_inverse": (ctypes.c_int, [vp, vp, sz, sz, sz, sz, sz, sz, vp]),
+            "svdi_image_snapshot_planned": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, sz, f,
+                                                           ctypes.POINTER(Options),
+                                                           ctypes.POINTER(PlanInfo)]),
             "svdi_fipipe_host": (ctypes.c_int, [vp, vp, vp, vp, vp, vp, sz, sz, f,
                                                 ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
             "svdi_fits_write": (ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_long, ctypes.c_long]),

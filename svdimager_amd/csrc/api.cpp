@@ -175,6 +175,7 @@ struct CallState {
     size_t records = 0;
     const uint32_t* tile_totals = nullptr;
     int gridder = SVDI_GRIDDER_AUTO;                       // resolved by image_enqueue
+    float det = 0.0f;                                      // |V00 V11 - V01 V10| of the rotation rows
 };
 
 // Phase 1 of an imaging call: validate, enqueue the planning kernels and the read-back of the
@@ -213,14 +214,13 @@ void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
 }
 
-// Phase 2, once ev_plan has completed: choose K and the slab batches on the host, then enqueue
-// gridding, FFT and image pass per slab batch, and record ev_stop.  The image (N x N floats,
-// device) is overwritten.
-void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+// Phase 2a, once ev_plan has completed: take the plan and the rotation rows from the read-back,
+// choose K and the slab batches on the host.
+void image_take_plan(svdi_context* ctx, const svdi_options& o, CallState& cs) {
     HostPlan& hp = cs.hp;
-    const size_t N = cs.N, G = hp.grid_size;
-    const float cell = cs.cell;
     hp.dev = ctx->pinned->plan;
+    const float* v = ctx->pinned->vin;
+    cs.det = std::fabs(v[0] * v[4] - v[1] * v[3]);
 
     // K: bucket rule (SVDIntegratedImager.cu:1577-1603) or smallest passing K (FIPkernels.cu:301-318).
     if (hp.semantics == Semantics::Integrated) {
@@ -253,9 +253,18 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
         hp.term_error = hp.semantics == Semantics::Archive ? bits_to_float(hp.dev.term_err_bits[o.num_terms - 1])
                                                            : hp.term_error;
     }
+    hp.batch_slabs = choose_batch_slabs(static_cast<size_t>(hp.dev.num_slabs), hp.num_terms,
+                                        hp.grid_size * hp.grid_size, o.max_batch_slabs);
+}
+
+// Phase 2b, with the plan in cs: enqueue gridding, FFT and image pass per slab batch.  The image
+// (N x N floats, device) is overwritten.
+void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+    HostPlan& hp = cs.hp;
+    const size_t N = cs.N, G = hp.grid_size;
+    const float cell = cs.cell;
     const int K = hp.num_terms;
     const size_t total_slabs = static_cast<size_t>(hp.dev.num_slabs);
-    hp.batch_slabs = choose_batch_slabs(total_slabs, K, G * G, o.max_batch_slabs);
 
     if (G % 2 == 0) {
         hp.row_lo = G / 2 - N / 2;
@@ -297,10 +306,8 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
               : o.mfma_precision == SVDI_PRECISION_SINGLE ? 2
               : o.mfma_precision == SVDI_PRECISION_BF16   ? 3
                                                           : 0;
-    const float* v = ctx->pinned->vin;
-    const float det = std::fabs(v[0] * v[4] - v[1] * v[3]);
-    gp.det = det;
-    gp.inv_det = 1.0f / det;
+    gp.det = cs.det;
+    gp.inv_det = 1.0f / cs.det;
 
     ImageParams ip = {};
     ip.N = static_cast<int>(N);
@@ -399,6 +406,9 @@ void fill_info(const CallState& cs, float ms, svdi_plan_info* info) {
     info->records = cs.records;
     info->elapsed_ms = ms;
     info->gridder = cs.gridder;
+    info->r3_centre = hp.dev.r3_centre;
+    info->inv_slab_half_range = hp.dev.inv_slab_half_range;
+    info->vis_det = cs.det;
 }
 
 // Phase 3: wait for the call's work, then report.
@@ -419,6 +429,7 @@ void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     CallState cs;
     image_begin(ctx, in, N, cell, o, cs);
     SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
+    image_take_plan(ctx, o, cs);
     image_enqueue(ctx, in, o, cs, image);
     image_finish_enqueue(ctx, cs.tile_totals, true);
     image_complete(ctx, o, cs, info);
@@ -515,6 +526,9 @@ void* Workspace::get(Slot s, size_t bytes) {
     if (bytes == 0) bytes = 16;
     bytes = (bytes + 255) & ~static_cast<size_t>(255);
     if (size_[s] >= bytes) return ptr_[s];
+    if (frozen_)
+        throw ArgError("the workspace would have to grow during stream capture: make one call of this shape on "
+                       "the context before capturing");
     if (arena_ != nullptr) {
         if (bytes > arena_cap_ - arena_used_)
             throw ArgError("the call needs more device scratch than the no-allocation entry point's " +
@@ -706,6 +720,7 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
             }
             const size_t j = static_cast<size_t>(pick);
             const size_t i = static_cast<size_t>(planned[j]);
+            image_take_plan(ctx->lanes[j].get(), o, cs[i]);
             image_enqueue(ctx->lanes[j].get(), in[i], o, cs[i], images[i]);
             planned[j] = -1;
             begin_next(j);
@@ -900,6 +915,66 @@ int svdi_image_snapshot_noalloc(svdi_context* ctx, const svdi_snapshot* snap, fl
         svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
         if (o.max_batch_slabs == 0) o.max_batch_slabs = 1;   // one slab's planes at a time
         image_core(ctx, in, image_size, cell_size, o, info, image);
+    });
+}
+
+int svdi_image_snapshot_planned(svdi_context* ctx, const svdi_snapshot* snap, float* image,
+                                size_t image_size, float cell_size, const svdi_options* opts,
+                                const svdi_plan_info* plan) {
+    return guarded([&] {
+        if (!ctx || !snap || !plan) throw ArgError("ctx / snapshot / plan is null");
+        check_device_pointer(image, "image");
+        const size_t N = image_size, G = (N * 3 + 1) / 2;
+        if (N == 0 || N > 46340) throw ArgError("Image_Size out of range");
+        if (!(cell_size > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
+        if (plan->grid_size != G) throw ArgError("the plan was made for another image size");
+        if (plan->num_terms < 1 || plan->num_terms > kMaxTerms || plan->num_slabs < 1 || plan->batch_slabs < 1 ||
+            plan->batch_slabs > static_cast<size_t>(plan->num_slabs) || !(plan->slab_width > 0.0f) ||
+            !(plan->slab_half_range > 0.0f) || !(plan->vis_det > 0.0f))
+            throw ArgError("the plan is not one an imaging call reported");
+        svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        if (o.semantics != SVDI_SEMANTICS_INTEGRATED) throw ArgError("planned calls take the integrated semantics");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        // Under stream capture the workspace must already hold this shape (no allocation).
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        SVDI_HIP(hipStreamIsCapturing(ctx->stream, &capture));
+        struct Freeze {
+            svdi_context* c;
+            bool profiling;
+            Freeze(svdi_context* ctx, bool on) : c(ctx), profiling(ctx->profiling) {
+                c->profiling = false;
+                c->ws.set_frozen(on);
+            }
+            ~Freeze() {
+                c->ws.set_frozen(false);
+                c->profiling = profiling;
+            }
+        } freeze(ctx, capture != hipStreamCaptureStatusNone);
+        begin_call(ctx);
+        SampleInput in = snapshot_input(ctx, *snap);
+        CallState cs;
+        cs.N = N;
+        cs.cell = cell_size;
+        cs.hp.semantics = Semantics::Integrated;
+        cs.hp.image_size = N;
+        cs.hp.grid_size = G;
+        cs.hp.cell = cell_size;
+        DevPlan& d = cs.hp.dev;
+        d = DevPlan();
+        d.r3_min = plan->r3_min;
+        d.r3_max = plan->r3_max;
+        d.num_slabs = plan->num_slabs;
+        d.slab_width = plan->slab_width;
+        d.slab_half_range = plan->slab_half_range;
+        d.inv_slab_half_range = plan->inv_slab_half_range;
+        d.r3_centre = plan->r3_centre;
+        d.slab_alpha_max = plan->slab_alpha_max;
+        cs.hp.num_terms = plan->num_terms;
+        cs.hp.term_error = plan->term_error;
+        cs.hp.batch_slabs = plan->batch_slabs;
+        cs.det = plan->vis_det;
+        image_enqueue(ctx, in, o, cs, image);
     });
 }
 

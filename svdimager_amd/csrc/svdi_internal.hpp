@@ -127,12 +127,15 @@ class Workspace {
     size_t bytes_held() const;
     void begin_arena(void* base, size_t capacity);
     void end_arena();
+    // Frozen: a slot that would have to grow throws ArgError instead of allocating.
+    void set_frozen(bool on) { frozen_ = on; }
     ~Workspace() { release(); }
 
   private:
     void* ptr_[kSlots] = {};
     size_t size_[kSlots] = {};
     // arena mode: the carved range, and the allocated slots put aside meanwhile
+    bool frozen_ = false;
     char* arena_ = nullptr;
     size_t arena_cap_ = 0, arena_used_ = 0;
     void* kept_ptr_[kSlots] = {};

@@ -122,10 +122,14 @@ def FIpipeDevice(ctx: Context, bin_, vin, vis_re, vis_im, image_size: int, cell_
 
 
 def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_size: float,
-                   image=None, noalloc: bool = False, **options) -> Tuple[object, Dict]:
+                   image=None, noalloc: bool = False, plan: Optional[Dict] = None,
+                   **options) -> Tuple[object, Dict]:
     """Preprocess + image one device-resident snapshot; returns (image, plan dict).
     noalloc=True calls svdi_image_snapshot_noalloc (no device allocation inside the call; the
-    image tensor, when not given, is still allocated here by torch)."""
+    image tensor, when not given, is still allocated here by torch).  plan=<a plan dict an
+    earlier call on the same geometry returned> calls svdi_image_snapshot_planned: no planning,
+    no host wait, capturable in a CUDA/HIP graph; it returns (image, plan) with the image
+    complete when torch's current stream reaches this point."""
     torch = _torch()
     dev = dsnap.tensors["uvw"].device
     N = int(image_size)
@@ -135,6 +139,11 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
     info = PlanInfo()
     _bind_stream(ctx, dev)
     s = dsnap.struct()
+    if plan is not None:
+        pin = PlanInfo.from_dict(plan)
+        check(lib().svdi_image_snapshot_planned(ctx.handle, ctypes.byref(s), _ptr(image), N,
+                                                float(cell_size), ctypes.byref(opts), ctypes.byref(pin)))
+        return image, plan
     fn = lib().svdi_image_snapshot_noalloc if noalloc else lib().svdi_image_snapshot
     check(fn(ctx.handle, ctypes.byref(s), _ptr(image), N, float(cell_size), ctypes.byref(opts),
              ctypes.byref(info)))

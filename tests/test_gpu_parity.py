@@ -253,3 +253,31 @@ def test_invalid_arguments(gpu_ctx, small_snap, dsnap):
         S.image_snapshot(gpu_ctx, dsnap, 64, -1.0)
     with pytest.raises(S.SvdiError):
         S.image_snapshot(gpu_ctx, dsnap, 64, 1e-4, num_terms=33)
+
+
+# Where the sampled Chebyshev error of K = 8 terms crosses the reference's 1e-2 target (FP64;
+# the oracle pins the reference's FP32 rule within 1e-5 of it, tests/test_oracle_pins.py).
+K8_THRESHOLD = 4.0946
+
+
+@pytest.mark.parametrize("side", [-1, 1])
+def test_bucket_choice_next_to_the_k8_threshold(gpu_ctx, small_snap, oracle_pre, side):
+    """The slab alpha driven 0.1% below / above the K = 8 threshold (r3 scaled, one slab): the
+    device picks K = 8 / 16 as the reference's FP32 bucket rule does on the same alpha."""
+    N, cell = small_snap["image_size"], small_snap["cell"]
+    b = np.ascontiguousarray(oracle_pre["bin"], np.float32).copy()
+    stats = O.cheb_stats(b, oracle_pre["vin"], N, cell)
+    alpha0 = 2.0 * np.pi * float(stats[2]) * 0.5 * (float(stats[1]) - float(stats[0]))
+    target = K8_THRESHOLD * (1.0 + 1e-3 * side)
+    b[:, 2] *= np.float32(target / alpha0)
+    announce(f"bucket choice at alpha {target:.5f}")
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda()
+         for k, v in (("bin", b), ("vin", oracle_pre["vin"]), ("vis_re", oracle_pre["vis_re"]),
+                      ("vis_im", oracle_pre["vis_im"]))}
+    _, plan = S.FIpipeDevice(gpu_ctx, t["bin"], t["vin"], t["vis_re"], t["vis_im"], N, cell,
+                             output=S.OUTPUT_SIGNED)
+    assert plan["num_slabs"] == 1
+    assert abs(plan["slab_alpha_max"] / target - 1.0) < 1e-4
+    expected = 8 if side < 0 else 16
+    assert plan["num_terms"] == expected
+    assert O.select_bucket(O.bucket_errors_ref32(plan["slab_alpha_max"])) == expected

@@ -442,3 +442,39 @@ def test_archive_dirty_conserves_flux_through_the_fp64_map(small_snap):
     ref = np.abs(dft[inside & valid]).sum()
     assert float(res["dirty"].sum()) == pytest.approx(ref, rel=2e-4)
     assert res["dirty"].min() >= 0.0
+
+
+def _sampled_error_f64(alpha, K):
+    """max over 257 alphas in [0, alpha] x 1024 s in [-1, 1] of |e^{i a s} - sum_{n<K} (n?2:1) i^n
+    J_n(a) T_n(s)|, in FP64 with scipy's J_n (the sampling of SVDIntegratedImager.cu:1125-1202)."""
+    from scipy import special
+    a = np.linspace(0.0, alpha, 257)
+    s = np.linspace(-1.0, 1.0, 1024)
+    J = np.stack([special.jv(n, a) for n in range(K)])
+    T = np.stack([np.cos(n * np.arccos(s)) for n in range(K)])
+    coef = np.array([1.0 if n == 0 else 2.0 for n in range(K)]) * (1j ** np.arange(K))
+    approx = np.einsum("n,na,ns->as", coef, J, T)
+    return float(np.abs(np.exp(1j * a[:, None] * s[None, :]) - approx).max())
+
+
+def _threshold(err_fn, lo=0.1, hi=30.0, steps=50):
+    for _ in range(steps):
+        mid = 0.5 * (lo + hi)
+        if err_fn(mid) <= 1e-2:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+def test_bucket_rule_thresholds_match_fp64():
+    """The reference's bucket rule (first K in {8, 16, 24, 32} with sampled error <= 1e-2,
+    SVDIntegratedImager.cu:1577-1603) switches from K = 8 to 16 at alpha ~ 4.0946 and from 16 to
+    24 at ~ 10.869.  Its FP32 series (the oracle's restatement) puts those switches within 1e-5
+    and 1e-4 (relative) of where the FP64 sampled error crosses 1e-2, so an imager that evaluates
+    the same error in FP64 picks another K only inside those intervals."""
+    for K, idx, tol in ((8, 0, 1e-5), (16, 1, 1e-4)):
+        a64 = _threshold(lambda a: _sampled_error_f64(a, K))
+        a32 = _threshold(lambda a: float(O.bucket_errors_ref32(a)[idx]))
+        assert abs(a32 - a64) <= tol * a64, (K, a32, a64)
+    assert abs(_threshold(lambda a: _sampled_error_f64(a, 8)) - 4.0946) < 1e-3
