@@ -237,6 +237,13 @@ def accuracy(S, ctx, snap, cfg, gridders, n_pixels=64, all_pixels=True):
 
 
 PRECISIONS = {"default": 0, "split3": 1, "splitc": 2, "single": 3, "bf16": 4}
+PRECISION_DTYPES = {
+    "default": "f16 hi/lo MFMA (visibility term and both ES factors split), f32 accumulate, f64 geometry",
+    "split3": "f16 hi/lo MFMA (visibility term and both ES factors split), f32 accumulate, f64 geometry",
+    "splitc": "f16 MFMA (visibility term hi/lo, ES product one f16), f32 accumulate, f64 geometry",
+    "single": "f16 MFMA (single f16 operands), f32 accumulate, f64 geometry",
+    "bf16": "bf16 hi/lo MFMA (visibility term and ES product split), f32 accumulate, f64 geometry",
+}
 
 
 def imager_variants(S, headline=None):
@@ -637,13 +644,13 @@ def main():
         "metric": METRIC, "value": snaps_per_s, "unit": "snapshots/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f16 hi/lo MFMA, f32 accumulate, f64 geometry" if mfma else "f32, f64 geometry",
+        "dtype": PRECISION_DTYPES[args.mfma_precision] if mfma else "f32, f64 geometry",
         "data": "synthetic",
         "config": {"workload": cfg.name, "stations": cfg.n_stations, "rows": cfg.num_rows,
                    "channels": cfg.n_channels, "samples": cfg.num_samples,
                    "image_size": cfg.image_size, "grid_size": cfg.grid_size, "cell": cfg.cell,
                    "num_terms": K, "num_slabs": plan["num_slabs"],
-                   "gridder": args.gridder, "batch": batch,
+                   "gridder": args.gridder, "mfma_precision": args.mfma_precision, "batch": batch,
                    "batch_calls": "pipelined (svdi_image_snapshot_batch_async)" if batch > 0 else None,
                    "parallelism": "single-gpu"},
         "mvis_per_s": cfg.num_samples * snaps_per_s / 1e6,
@@ -655,6 +662,16 @@ def main():
         "path_roofline_cap_snapshots_per_s": HBM_PEAK_GBS * 1e9 / hbm_bytes,
         "roofline": roofline,
     }
+    if not args.no_compare and mfma:
+        # The other operand precisions of the MFMA gridder, single calls (their accuracy is in
+        # accuracy_vs_fp64_dft).
+        modes = {}
+        for name in ("split3", "splitc", "single", "bf16"):
+            el3, st3, _ = time_gridder(S, ctx, dsnap, cfg, image, S.GRIDDER_MFMA, args.steps, min(args.warmup, 2),
+                                       f"mfma-{name}", mfma_precision=PRECISIONS[name])
+            modes[name] = {"snapshots_per_s": args.steps / el3, "k_win_grid_ms": st3["grid_accumulate"],
+                           "grid_stage_ms": st3["grid"], "dtype": PRECISION_DTYPES[name]}
+        out["mfma_precision_modes"] = modes
     if not args.no_compare and args.gridder not in ("tiled", "atomic"):
         # The plain FP32 imagers: the FP32 tile gridder, and the reference's own schedule (one
         # thread per sample, a device atomic per cell and term, SVDIntegratedImager.cu:1276-1291).
@@ -674,6 +691,23 @@ def main():
         log("bench: accuracy against the FP64 DFT ...")
         gl = imager_variants(S, (args.gridder, {"gridder": gridder, **gopts}))
         out["accuracy_vs_fp64_dft"] = accuracy(S, ctx, snap, cfg, gl)
+    # The host-buffer boundary (FIpipe on host Bin / Visreal / Visimag, archive semantics): the
+    # PCIe-inclusive single-call time, the rate a caller of the C++ FIpipe symbol sees.
+    pre = S.preprocess_measurement_set_gpu(ctx, dsnap)
+    host_in = {k: pre[k].cpu().numpy() for k in ("bin", "vin", "vis_re", "vis_im")}
+    del pre
+    S.fipipe_host(ctx, host_in["vis_re"], host_in["vis_im"], host_in["bin"], host_in["vin"],
+                  cfg.image_size, cfg.cell)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        S.fipipe_host(ctx, host_in["vis_re"], host_in["vis_im"], host_in["bin"], host_in["vin"],
+                      cfg.image_size, cfg.cell)
+    fip_ms = (time.perf_counter() - t0) / 3 * 1e3
+    h2d = sum(host_in[k].nbytes for k in host_in)
+    out["fipipe_host"] = {"ms_per_call": fip_ms, "h2d_bytes": int(h2d),
+                          "d2h_bytes": 4 * cfg.image_size * cfg.image_size,
+                          "note": "PCIe-inclusive: host Bin / Visreal / Visimag / Vin in, dirty image out"}
+    log(f"bench: FIpipe on host buffers {fip_ms:.2f} ms per call ({h2d / 1e6:.0f} MB in)")
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline ...")
         host = {k: snap[k] for k in snap}
