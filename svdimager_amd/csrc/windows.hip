@@ -605,12 +605,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 // hi*lo.  Otherwise B_a = f16(uh vh) (relative error <= 3 x 2^-11 against ku kv, from the three
 // roundings) and D_a += C_hi B_a [+ C_lo B_a].
 template <int P>
-__device__ __forceinline__ void window_row(floatx4& d, const _Float16* __restrict__ uh_row,
-                                           const _Float16* __restrict__ ul_row, const half8& vh, const half8& vl,
-                                           const half8& a_hi, const half8& a_lo) {
-    const half8 uh = *reinterpret_cast<const half8*>(uh_row);
+__device__ __forceinline__ void window_row_regs(floatx4& d, const half8& uh, const half8& ul, const half8& vh,
+                                                const half8& vl, const half8& a_hi, const half8& a_lo) {
     if constexpr (P == kPrecSplit3) {
-        const half8 ul = *reinterpret_cast<const half8*>(ul_row);
         const half8 p = uh * vh;
         const half8 e = __builtin_elementwise_fma(uh, vh, -p);
         const half8 b_lo = __builtin_elementwise_fma(ul, vh, __builtin_elementwise_fma(uh, vl, e));
@@ -823,9 +820,20 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
 #pragma unroll
     for (int qd = 0; qd < 4; ++qd) {
         if (qd >= q_lo && qd <= q_hi) {
+            // Each row's ES values are requested one row ahead, so that their LDS latency
+            // overlaps the previous row's products and MFMAs.
+            half8 uh_n = *reinterpret_cast<const half8*>(uhb + 4 * qd * kStrideH);
+            half8 ul_n = uh_n;
+            if constexpr (T::kEsLo) ul_n = *reinterpret_cast<const half8*>(ulb + 4 * qd * kStrideH);
 #pragma unroll
-            for (int a = 4 * qd; a < 4 * qd + 4; ++a)
-                window_row<P>(d[a], uhb + a * kStrideH, ulb + a * kStrideH, vh, vl, a_hi, a_lo);
+            for (int a = 4 * qd; a < 4 * qd + 4; ++a) {
+                const half8 uh = uh_n, ul = ul_n;
+                if (a + 1 < 4 * qd + 4) {
+                    uh_n = *reinterpret_cast<const half8*>(uhb + (a + 1) * kStrideH);
+                    if constexpr (T::kEsLo) ul_n = *reinterpret_cast<const half8*>(ulb + (a + 1) * kStrideH);
+                }
+                window_row_regs<P>(d[a], uh, ul, vh, vl, a_hi, a_lo);
+            }
         }
     }
     wave_lds_sync();
