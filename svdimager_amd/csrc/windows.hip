@@ -143,7 +143,7 @@ struct WRecords {
 // weights sum to zero), so both passes keep the same samples.
 template <bool kSnap, bool kVis, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
-                                                const WinGeom& wg, Fn&& fn) {
+                                                const WinGeom& wg, size_t blk, Fn&& fn) {
     auto bins_of = [&](const Footprint& f) {
         const int h = f.a0 >> wg.cls_log2;
         const int col0 = f.local_slab * wg.nw;
@@ -153,7 +153,7 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
     if constexpr (kSnap) {
         constexpr int kBatch = 4;
         for (int set = 0; set < kBinRowSets; ++set) {
-        const size_t row = blockIdx.x * (size_t)kBinRows + set * kBinRowsPerSet + (threadIdx.x % kBinRowsPerSet);
+        const size_t row = blk * kBinRows + set * kBinRowsPerSet + (threadIdx.x % kBinRowsPerSet);
         if (row >= in.num_rows) return;
         const double ru = in.rot[3 * row], rv = in.rot[3 * row + 1], rw = in.rot[3 * row + 2];
         for (size_t c0 = threadIdx.x / kBinRowsPerSet; c0 < in.num_channels; c0 += kBatch * kBinChanGroups) {
@@ -172,7 +172,7 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
         }
         }
     } else {
-        const size_t base = blockIdx.x * (size_t)kBinThreads * kBinPerThread + threadIdx.x;
+        const size_t base = blk * kBinThreads * kBinPerThread + threadIdx.x;
         for (int j = 0; j < kBinPerThread; ++j) {
             const size_t idx = base + static_cast<size_t>(j) * kBinThreads;
             if (idx >= in.num_samples) return;
@@ -221,8 +221,9 @@ __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
             uint32_t* __restrict__ blockhist, uint32_t* __restrict__ blocktot) {
     extern __shared__ uint32_t hist[];
+    const size_t blk = blockIdx.x;
     if (!wg.use_lds) {
-        for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
+        for_sample_bins<kSnap, false>(in, gp, wg, blk, [&](const Footprint&, int bin, int, int) {
             atomicAdd(&counts[bin], 1u);
         });
         return;
@@ -232,7 +233,7 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = 0;
     __syncthreads();
     uint32_t cnt = 0;
-    for_sample_bins<kSnap, false>(in, gp, wg, [&](const Footprint&, int bin, int, int) {
+    for_sample_bins<kSnap, false>(in, gp, wg, blk, [&](const Footprint&, int bin, int, int) {
         const uint32_t b = static_cast<uint32_t>(bin - lo);
         if (b < static_cast<uint32_t>(n)) {
             atomicAdd(&hist[b], 1u);
@@ -240,8 +241,8 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
         }
     });
     const uint32_t tot = block_sum(cnt);          // (its barrier also ends the histogram's atomics)
-    if (threadIdx.x == 0) blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x] = tot;
-    uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
+    if (threadIdx.x == 0) blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blk] = tot;
+    uint32_t* row = blockhist + blk * wg.nbins + lo;
     for (int b = threadIdx.x; b < n; b += kBinThreads) row[b] = hist[b];
 }
 
@@ -336,8 +337,9 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         const int base2 = kSnap ? f.b0 - wb * kWin : off - wb * kWin;
         rec.put(slot, rec_encode(base1, f.q1, base2, f.q2, f.s, f.vw.x, f.vw.y));
     };
+    const size_t blk = blockIdx.x;
     if (!wg.use_lds) {
-        for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+        for_sample_bins<kSnap, true>(in, gp, wg, blk, [&](const Footprint& f, int bin, int h, int wb) {
             emit(atomicAdd(&cursor[bin], 1u), f, h, wb);
         });
         block_max_to(m, vmax_bits);
@@ -345,11 +347,11 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
     }
     int lo, n;
     lds_band(wg, lo, n);
-    const uint32_t* row = blockhist + static_cast<size_t>(blockIdx.x) * wg.nbins + lo;
+    const uint32_t* row = blockhist + blk * wg.nbins + lo;
     for (int b = threadIdx.x; b < n; b += kBinThreads) hist[b] = row[b];
     __syncthreads();
     uint32_t cnt = 0;
-    for_sample_bins<kSnap, true>(in, gp, wg, [&](const Footprint& f, int bin, int h, int wb) {
+    for_sample_bins<kSnap, true>(in, gp, wg, blk, [&](const Footprint& f, int bin, int h, int wb) {
         const uint32_t b = static_cast<uint32_t>(bin - lo);
         if (b < static_cast<uint32_t>(n)) {
             emit(atomicAdd(&hist[b], 1u), f, h, wb);
@@ -357,7 +359,7 @@ k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restr
         }
     });
     const uint32_t tot = block_sum(cnt);
-    if (threadIdx.x == 0 && tot != blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blockIdx.x])
+    if (threadIdx.x == 0 && tot != blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blk])
         atomicOr(err, kErrBinMismatch);
     block_max_to(m, vmax_bits);
 }
@@ -389,14 +391,17 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
 // depend on timing).  Both queues run largest first, by octave classes: class c holds items
 // with count * 2^c <= cap < count * 2^(c+1) (cap = kCoopMin for wavefront items, kCoopCap for
 // workgroup items; the last class also the smaller and empty ones), so that the wavefronts and
-// workgroups finish together.
+// workgroups finish together.  The two limits were measured (SKA-Low snapshot at 1024^2, whose
+// 9216 windows hold 1 to 1.8 M records each): below ~1000 records a workgroup item spends a
+// large part of its time summing the wavefronts' accumulators, and pieces much above ~25000
+// records leave the last ones running alone at the end of the launch.
 // icount: [kItemClasses][nwin_total] wavefront items, [kItemClasses][nwin_total] workgroup items,
 // [nwin_total] partial slots; one exclusive scan gives the queue positions and slot numbers.
 // totals[0] = records, [1] = wavefront items, [2] = (unused), [3] = workgroup items,
 // [4] = max |v| bits, [5] = workgroup queue head, [6] = windows with several pieces.
 constexpr int kItemClasses = 8;
-constexpr uint32_t kCoopMin = 256;
-constexpr uint32_t kCoopCap = 12288;
+constexpr uint32_t kCoopMin = 1024;
+constexpr uint32_t kCoopCap = 24576;
 
 __device__ __forceinline__ int item_class(uint32_t count, uint32_t cap) {
     if (count == 0) return kItemClasses - 1;
@@ -481,15 +486,21 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
 }
 
 // Windows with several pieces: the sum of their partial windows, in piece order, into the
-// planes.  One unit = (listed window, term); 256 threads = the window's cells.
-__global__ void __launch_bounds__(256)
+// planes.  One unit = (listed window, term) per workgroup of kReduceGroups x 256 threads: group g
+// sums the pieces g, g + kReduceGroups, ... of every cell (the densest windows have over a
+// hundred pieces, which one thread per cell would request one after another), and the groups'
+// sums are added in group order, so the result does not depend on timing.
+constexpr int kReduceGroups = 4;
+__global__ void __launch_bounds__(kReduceGroups * 256)
 k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ ioff,
              const uint32_t* __restrict__ multi, const uint32_t* __restrict__ totals, WinGeom wg,
              int G, int K, uint32_t max_slots, const float2* __restrict__ partial, float2* __restrict__ planes) {
+    __shared__ float2 s_sum[kReduceGroups - 1][256];
     const int n = 2 * kItemClasses * wg.nwin_total;
     const uint32_t units = totals[6] * static_cast<uint32_t>(K);
     const size_t plane_size = static_cast<size_t>(G) * G;
     const size_t piece_size = static_cast<size_t>(K) * kWin * kWin;
+    const int cell = threadIdx.x & 255, grp = threadIdx.x >> 8;
     for (uint32_t u = blockIdx.x; u < units; u += gridDim.x) {
         const int w = static_cast<int>(multi[u / K]);
         const int k = static_cast<int>(u % K);
@@ -498,41 +509,46 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
         const uint32_t pieces = coop_pieces(end - x.start());
         const uint32_t first = ioff[n + w] - ioff[n];
         if (first + pieces > max_slots) continue;          // never: the slot count bounds the pieces
-        const int ls = w / wg.nwin, r = w - ls * wg.nwin;
-        const int wa = r / wg.nw, wb = r - wa * wg.nw;
-        const int cell = threadIdx.x;
-        const int a = wa * kWin + cell / kWin, b = wb * kWin + cell % kWin;
-        const float2* src = partial + first * piece_size + static_cast<size_t>(k) * kWin * kWin + cell;
-        // Sixteen pieces' values requested at a time (the densest windows have many pieces),
-        // summed into four accumulators in a fixed order.
+        const float2* src = partial + (first + grp) * piece_size + static_cast<size_t>(k) * kWin * kWin + cell;
+        const size_t step = kReduceGroups * piece_size;
+        // This group's pieces, eight requested at a time, summed into four accumulators in a
+        // fixed order.
+        const uint32_t mine = pieces > static_cast<uint32_t>(grp) ? (pieces - grp + kReduceGroups - 1) / kReduceGroups : 0;
         float2 acc[4] = {};
         uint32_t p = 0;
-        for (; p + 16 <= pieces; p += 16) {
-            float2 v[16];
+        for (; p + 8 <= mine; p += 8) {
+            float2 v[8];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) v[j] = src[(p + j) * piece_size];
+            for (int j = 0; j < 8; ++j) v[j] = src[(p + j) * step];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
+            for (int j = 0; j < 8; ++j) {
                 acc[j & 3].x += v[j].x;
                 acc[j & 3].y += v[j].y;
             }
         }
-        for (; p + 4 <= pieces; p += 4) {
+        for (; p < mine; ++p) {
+            const float2 v = src[p * step];
+            acc[p & 3].x += v.x;
+            acc[p & 3].y += v.y;
+        }
+        const float2 sum = make_float2((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x),
+                                       (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y));
+        if (grp > 0) s_sum[grp - 1][cell] = sum;
+        __syncthreads();
+        if (grp == 0) {
+            float2 t = sum;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float2 v = src[(p + j) * piece_size];
-                acc[j].x += v.x;
-                acc[j].y += v.y;
+            for (int g = 0; g < kReduceGroups - 1; ++g) {
+                t.x += s_sum[g][cell].x;
+                t.y += s_sum[g][cell].y;
             }
+            const int ls = w / wg.nwin, r = w - ls * wg.nwin;
+            const int wa = r / wg.nw, wb = r - wa * wg.nw;
+            const int a = wa * kWin + cell / kWin, b = wb * kWin + cell % kWin;
+            if (a < G && b < G)
+                planes[(static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(a) * G + b] = t;
         }
-        for (; p < pieces; ++p) {
-            const float2 v = src[p * piece_size];
-            acc[0].x += v.x;
-            acc[0].y += v.y;
-        }
-        if (a < G && b < G)
-            planes[(static_cast<size_t>(ls) * K + k) * plane_size + static_cast<size_t>(a) * G + b] =
-                make_float2((acc[0].x + acc[1].x) + (acc[2].x + acc[3].x), (acc[0].y + acc[1].y) + (acc[2].y + acc[3].y));
+        __syncthreads();
     }
 }
 
@@ -1218,7 +1234,7 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     else if (gp.prec == kPrecSplitC) launch(std::integral_constant<int, kPrecSplitC>());
     else launch(std::integral_constant<int, kPrecSplit3>());
     SVDI_LAUNCH_CHECK();
-    k_win_reduce<<<2048, 256, 0, ctx->stream>>>(boff, ioff, multi, totals, wg, gp.G, gp.K,
+    k_win_reduce<<<2048, kReduceGroups * 256, 0, ctx->stream>>>(boff, ioff, multi, totals, wg, gp.G, gp.K,
                                                 static_cast<uint32_t>(max_slots), partial, planes);
     SVDI_LAUNCH_CHECK();
     stage_end(ctx, tok);
