@@ -235,6 +235,38 @@ int svdi_image_snapshots(svdi_context* ctx, const svdi_snapshot* snapshots, size
                          float* const* images, const size_t* image_sizes, const float* cell_sizes,
                          const svdi_options* opts, svdi_plan_info* infos, int wait);
 
+/* Work shape of the MFMA window gridder (no counterpart in the reference, whose gridder has no
+ * shape to tune; the batching form picks it per image size with svdi_autotune).  The gridder
+ * contracts 16 x 16 windows of the grid; a window's records are contracted by one wavefront
+ * when there are at most coop_min of them, else in pieces of at most coop_cap records each
+ * taken by a whole workgroup.  Records are binned by row classes of 2^class_log2 rows.  Zero
+ * fields select the library's defaults (coop_min 1024, coop_cap 24576, class_log2 2 when its
+ * histogram fits in LDS, else 3).  The shape changes how the FP32 sums are ordered, not what
+ * is summed: images agree to rounding. */
+typedef struct svdi_tuning {
+    uint32_t coop_min;          /* 0, or 32 .. 65536 */
+    uint32_t coop_cap;          /* 0, or 4096 .. 16777216 */
+    int32_t class_log2;         /* 0, 2 or 3 */
+    float grid_ms;              /* svdi_autotune: the grid stage's time with this shape, ms */
+    float default_ms;           /* svdi_autotune: the grid stage's time with the defaults, ms */
+} svdi_tuning;
+
+/* The shape the context's imaging calls use for grids of grid_size (= ceil(3 image_size / 2));
+ * grid_size 0 sets the shape for every size without one of its own.  tuning NULL removes the
+ * entry.  Batched calls pass the context's shapes to their lanes. */
+int svdi_set_tuning(svdi_context* ctx, size_t grid_size, const svdi_tuning* tuning);
+/* The shape in use for grid_size (all fields 0 when the defaults are). */
+int svdi_get_tuning(svdi_context* ctx, size_t grid_size, svdi_tuning* tuning);
+
+/* Autotuner (no counterpart in the reference): images `snapshot` at image_size / cell_size with
+ * the MFMA gridder under each candidate shape (coop_min 256 .. 2048, coop_cap 12288 .. 49152, both
+ * class heights), `repeats` times each (>= 1), times the grid stage with HIP events, and keeps the
+ * fastest for this grid size on the context (svdi_set_tuning); *best receives it with its time
+ * and the defaults' time.  K, the slabs and ES support stay the reference's: the search changes
+ * only the work decomposition.  Allocates an image-sized scratch buffer for the trial images. */
+int svdi_autotune(svdi_context* ctx, const svdi_snapshot* snapshot, size_t image_size, float cell_size,
+                  const svdi_options* opts, int repeats, svdi_tuning* best);
+
 /* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
  * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = [
     "svdi_set_profiling", "svdi_last_stage_times", "svdi_snapshot_geometry",
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
     "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
-    "svdi_image_snapshot_planned",
+    "svdi_image_snapshot_planned", "svdi_set_tuning", "svdi_get_tuning", "svdi_autotune",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -95,6 +95,17 @@ class PlanInfo(ctypes.Structure):
             v = getattr(self, name)
             d[name] = list(v) if name == "bucket_errors" else v
         return d
+
+
+class Tuning(ctypes.Structure):
+    """svdi_tuning: the MFMA window gridder's work shape (0 fields: the library's defaults)."""
+    _fields_ = [
+        ("coop_min", ctypes.c_uint32), ("coop_cap", ctypes.c_uint32), ("class_log2", ctypes.c_int32),
+        ("grid_ms", ctypes.c_float), ("default_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 _lib = None
@@ -163,6 +174,10 @@ def lib():
             "svdi_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
             "svdi_last_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
                                                      ctypes.c_int]),
+            "svdi_set_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
+            "svdi_get_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
+            "svdi_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz, f, ctypes.POINTER(Options),
+                                             ctypes.c_int, ctypes.POINTER(Tuning)]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -221,6 +236,19 @@ class Context:
         arr = (ctypes.c_float * len(STAGES))()
         check(lib().svdi_last_stage_times(self._h, arr, len(STAGES)))
         return {name: float(arr[i]) for i, name in enumerate(STAGES)}
+
+    def set_tuning(self, grid_size: int, coop_min: int = 0, coop_cap: int = 0, class_log2: int = 0) -> None:
+        """The MFMA gridder's work shape for grids of grid_size (0: every size without its own)."""
+        t = Tuning(coop_min=coop_min, coop_cap=coop_cap, class_log2=class_log2)
+        check(lib().svdi_set_tuning(self._h, grid_size, ctypes.byref(t)))
+
+    def clear_tuning(self, grid_size: int) -> None:
+        check(lib().svdi_set_tuning(self._h, grid_size, None))
+
+    def get_tuning(self, grid_size: int) -> dict:
+        t = Tuning()
+        check(lib().svdi_get_tuning(self._h, grid_size, ctypes.byref(t)))
+        return t.as_dict()
 
     def close(self) -> None:
         if self._h:

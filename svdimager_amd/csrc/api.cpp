@@ -257,6 +257,16 @@ void image_take_plan(svdi_context* ctx, const svdi_options& o, CallState& cs) {
                                         hp.grid_size * hp.grid_size, o.max_batch_slabs);
 }
 
+// The MFMA gridder's work shape for grid size G: the context's entry for G, else its entry for
+// every size, else the defaults (all zero).
+svdi_tuning tuning_for(const svdi_context* ctx, size_t G) {
+    auto it = ctx->tuning.find(G);
+    if (it == ctx->tuning.end()) it = ctx->tuning.find(0);
+    if (it != ctx->tuning.end()) return it->second;
+    svdi_tuning t = {};
+    return t;
+}
+
 // Phase 2b, with the plan in cs: enqueue gridding, FFT and image pass per slab batch.  The image
 // (N x N floats, device) is overwritten.
 void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
@@ -308,6 +318,10 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
                                                           : 0;
     gp.det = cs.det;
     gp.inv_det = 1.0f / cs.det;
+    const svdi_tuning tune = tuning_for(ctx, G);
+    gp.coop_min = tune.coop_min;
+    gp.coop_cap = tune.coop_cap;
+    gp.class_log2 = tune.class_log2;
 
     ImageParams ip = {};
     ip.N = static_cast<int>(N);
@@ -661,6 +675,7 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
         const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
         const size_t L = std::min(count, kBatchLanes);
         while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
+        for (size_t j = 0; j < L; ++j) ctx->lanes[j]->tuning = ctx->tuning;
         // Largest-first assignment of snapshots to lanes.
         std::vector<size_t> order(count);
         std::vector<double> cost(count);
@@ -1066,6 +1081,110 @@ int svdi_fft2_inverse(svdi_context* ctx, const void* planes, size_t num_planes, 
         launch_fft2_inverse(ctx, static_cast<const float2*>(planes), num_planes, grid_size, row_lo, rows, col_lo,
                             cols, rowfft, static_cast<float2*>(out));
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int svdi_set_tuning(svdi_context* ctx, size_t grid_size, const svdi_tuning* tuning) {
+    return guarded([&] {
+        if (!ctx) throw ArgError("ctx is null");
+        if (tuning) {
+            if (tuning->coop_min != 0 && (tuning->coop_min < 32 || tuning->coop_min > 65536))
+                throw ArgError("coop_min must be 0 or in 32 .. 65536");
+            if (tuning->coop_cap != 0 && (tuning->coop_cap < 4096 || tuning->coop_cap > (1u << 24)))
+                throw ArgError("coop_cap must be 0 or in 4096 .. 16777216");
+            if (tuning->class_log2 != 0 && tuning->class_log2 != 2 && tuning->class_log2 != 3)
+                throw ArgError("class_log2 must be 0, 2 or 3");
+        }
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (tuning) ctx->tuning[grid_size] = *tuning;
+        else ctx->tuning.erase(grid_size);
+    });
+}
+
+int svdi_get_tuning(svdi_context* ctx, size_t grid_size, svdi_tuning* tuning) {
+    return guarded([&] {
+        if (!ctx || !tuning) throw ArgError("ctx / tuning is null");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        *tuning = tuning_for(ctx, grid_size);
+    });
+}
+
+int svdi_autotune(svdi_context* ctx, const svdi_snapshot* snap, size_t image_size, float cell_size,
+                  const svdi_options* opts, int repeats, svdi_tuning* best) {
+    return guarded([&] {
+        if (!ctx || !snap || !best) throw ArgError("ctx / snapshot / best is null");
+        if (repeats < 1 || repeats > 100) throw ArgError("repeats must be in 1 .. 100");
+        const size_t N = image_size, G = (N * 3 + 1) / 2;
+        if (N == 0 || N > 46340) throw ArgError("Image_Size out of range");
+        if (!(cell_size > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        o.gridder = SVDI_GRIDDER_MFMA;
+        o.verbose = 0;
+        float* image = nullptr;
+        SVDI_HIP(hipMalloc(&image, N * N * sizeof(float)));
+        struct Restore {
+            svdi_context* c;
+            float* image;
+            bool profiling;
+            ~Restore() {
+                (void)hipStreamSynchronize(c->stream);
+                (void)hipFree(image);
+                c->profiling = profiling;
+            }
+        } restore{ctx, image, ctx->profiling};
+        ctx->profiling = true;
+        const bool had_entry = ctx->tuning.count(G) != 0;
+        const svdi_tuning saved = had_entry ? ctx->tuning[G] : svdi_tuning{};
+        // The grid stage's best time over `repeats` calls with shape t (stage [2]: binning, item
+        // lists, the window contraction and the piece sums).
+        auto time_shape = [&](const svdi_tuning& t) {
+            ctx->tuning[G] = t;
+            float ms = 0.0f;
+            for (int r = 0; r < repeats; ++r) {
+                begin_call(ctx);
+                SampleInput in = snapshot_input(ctx, *snap);
+                image_core(ctx, in, N, cell_size, o, nullptr, image);
+                const float m = ctx->stage_ms[2];
+                if (r == 0 || m < ms) ms = m;
+            }
+            return ms;
+        };
+        try {
+            const svdi_tuning defaults = {};
+            (void)time_shape(defaults);                 // the workspace grows to this shape first
+            const float default_ms = time_shape(defaults);
+            svdi_tuning pick = defaults;
+            pick.grid_ms = default_ms;
+            for (uint32_t cmin : {256u, 512u, 1024u, 2048u})
+                for (uint32_t ccap : {12288u, 24576u, 49152u})
+                    for (int cls : {2, 3}) {
+                        svdi_tuning t = {};
+                        t.coop_min = cmin;
+                        t.coop_cap = ccap;
+                        t.class_log2 = cls;
+                        const float ms = time_shape(t);
+                        if (ms < pick.grid_ms) {
+                            pick = t;
+                            pick.grid_ms = ms;
+                        }
+                    }
+            // The defaults once more (the first timings of a context run cold): a shape replaces
+            // them only when it beats their better time.
+            const float default_ms2 = std::min(default_ms, time_shape(defaults));
+            if (!(pick.grid_ms < default_ms2)) {
+                pick = defaults;
+                pick.grid_ms = default_ms2;
+            }
+            pick.default_ms = default_ms2;
+            ctx->tuning[G] = pick;
+            *best = pick;
+        } catch (...) {
+            if (had_entry) ctx->tuning[G] = saved;
+            else ctx->tuning.erase(G);
+            throw;
+        }
     });
 }
 

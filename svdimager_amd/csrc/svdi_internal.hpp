@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -110,6 +111,9 @@ struct GridParams {
     float det;
     double inv_slab_width;  // 1 / slab_width
     int prec;               // MFMA gridder operand precision: 0 split3, 1 split c, 2 single
+    // MFMA gridder work shape (svdi_tuning; 0 = the default).
+    uint32_t coop_min, coop_cap;
+    int class_log2;
 };
 
 // Grown-on-demand device buffers owned by a context.  In arena mode (between begin_arena and
@@ -185,6 +189,8 @@ struct svdi_context {
     size_t events_used = 0;
     std::vector<std::pair<int, size_t>> stage_marks;   // (stage, index of start event)
     float stage_ms[SVDI_NUM_STAGES] = {};
+    // MFMA gridder work shapes by grid size (0: every size without its own entry).
+    std::map<size_t, svdi_tuning> tuning;
     svdi_context() = default;
     svdi_context(const svdi_context&) = delete;
     svdi_context& operator=(const svdi_context&) = delete;

@@ -110,6 +110,8 @@ struct WinGeom {
     int nbins;       // nslabs * nw * nh, ordered (slab, window column, row class)
     int use_lds;     // block histograms in LDS (else one global atomic per record)
     int band_bins;   // bins per LDS band (<= kLdsBins); band y covers [y band_bins, ...)
+    uint32_t coop_min;   // records up to which a window is one wavefront item
+    uint32_t coop_cap;   // records per workgroup piece
 };
 
 // A work item: records [start, start + count) of window `win`; bnd[j] (j = 1 .. ncls - 1) is the
@@ -384,24 +386,25 @@ __device__ __forceinline__ WinRange window_range(const uint32_t* __restrict__ bo
     return x;
 }
 
-// Work items.  A window with at most kCoopMin records is one wavefront item.  A larger one is
-// taken by whole workgroups: pieces of at most kCoopCap records whose chunks the workgroup's
+// Work items.  A window with at most coop_min records is one wavefront item.  A larger one is
+// taken by whole workgroups: pieces of at most coop_cap records whose chunks the workgroup's
 // wavefronts share, their accumulators summed in LDS.  A window with several pieces writes one
 // partial window per piece, and k_win_reduce adds them in piece order (so the result does not
 // depend on timing).  Both queues run largest first, by octave classes: class c holds items
-// with count * 2^c <= cap < count * 2^(c+1) (cap = kCoopMin for wavefront items, kCoopCap for
+// with count * 2^c <= cap < count * 2^(c+1) (cap = coop_min for wavefront items, coop_cap for
 // workgroup items; the last class also the smaller and empty ones), so that the wavefronts and
-// workgroups finish together.  The two limits were measured (SKA-Low snapshot at 1024^2, whose
-// 9216 windows hold 1 to 1.8 M records each): below ~1000 records a workgroup item spends a
-// large part of its time summing the wavefronts' accumulators, and pieces much above ~25000
-// records leave the last ones running alone at the end of the launch.
+// workgroups finish together.  The default limits were measured (SKA-Low snapshot at 1024^2,
+// whose 9216 windows hold 1 to 1.8 M records each): below ~1000 records a workgroup item spends
+// a large part of its time summing the wavefronts' accumulators, and pieces much above ~25000
+// records leave the last ones running alone at the end of the launch; a context's tuning
+// (svdi_autotune) may replace them per grid size.
 // icount: [kItemClasses][nwin_total] wavefront items, [kItemClasses][nwin_total] workgroup items,
 // [nwin_total] partial slots; one exclusive scan gives the queue positions and slot numbers.
 // totals[0] = records, [1] = wavefront items, [2] = (unused), [3] = workgroup items,
 // [4] = max |v| bits, [5] = workgroup queue head, [6] = windows with several pieces.
 constexpr int kItemClasses = 8;
-constexpr uint32_t kCoopMin = 1024;
-constexpr uint32_t kCoopCap = 24576;
+constexpr uint32_t kCoopMinDefault = 1024;
+constexpr uint32_t kCoopCapDefault = 24576;
 
 __device__ __forceinline__ int item_class(uint32_t count, uint32_t cap) {
     if (count == 0) return kItemClasses - 1;
@@ -410,7 +413,9 @@ __device__ __forceinline__ int item_class(uint32_t count, uint32_t cap) {
     return c;
 }
 
-__device__ __forceinline__ uint32_t coop_pieces(uint32_t c) { return (c + kCoopCap - 1) / kCoopCap; }
+__device__ __forceinline__ uint32_t coop_pieces(const WinGeom& wg, uint32_t c) {
+    return (c + wg.coop_cap - 1) / wg.coop_cap;
+}
 
 __global__ void __launch_bounds__(256)
 k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, uint32_t* __restrict__ icount,
@@ -423,15 +428,15 @@ k_win_item_counts(const uint32_t* __restrict__ boff, WinGeom wg, uint32_t* __res
     const WinRange x = window_range(boff, wg, w, end);
     const uint32_t c = end - x.start();
     for (int k = 0; k <= 2 * kItemClasses; ++k) icount[k * nwt + w] = 0u;
-    if (c <= kCoopMin) {
-        icount[item_class(c, kCoopMin) * nwt + w] = 1u;
+    if (c <= wg.coop_min) {
+        icount[item_class(c, wg.coop_min) * nwt + w] = 1u;
         return;
     }
-    const uint32_t full = c / kCoopCap, rem = c - full * kCoopCap;
+    const uint32_t full = c / wg.coop_cap, rem = c - full * wg.coop_cap;
     uint32_t* gcount = icount + kItemClasses * nwt;
     gcount[w] += full;
-    if (rem > 0) gcount[item_class(rem, kCoopCap) * nwt + w] += 1u;
-    const uint32_t pieces = coop_pieces(c);
+    if (rem > 0) gcount[item_class(rem, wg.coop_cap) * nwt + w] += 1u;
+    const uint32_t pieces = coop_pieces(wg, c);
     if (pieces > 1) {
         icount[2 * kItemClasses * nwt + w] = pieces;
         multi[atomicAdd(&totals[6], 1u)] = static_cast<uint32_t>(w);
@@ -475,12 +480,12 @@ k_win_item_write(const uint32_t* __restrict__ boff, const uint32_t* __restrict__
     t.count = static_cast<int>(c);
     t.slot = -1;
     if (coop) {
-        const uint32_t full = c / kCoopCap;
+        const uint32_t full = c / wg.coop_cap;
         const uint32_t q = i - ioff[lo];
         const uint32_t piece = (cls == 0 && q < full) ? q : full;
-        t.start = static_cast<int>(x.start() + piece * kCoopCap);
-        t.count = static_cast<int>(piece < full ? kCoopCap : c - full * kCoopCap);
-        if (coop_pieces(c) > 1) t.slot = static_cast<int>(ioff[n + w] - ioff[n] + piece);
+        t.start = static_cast<int>(x.start() + piece * wg.coop_cap);
+        t.count = static_cast<int>(piece < full ? wg.coop_cap : c - full * wg.coop_cap);
+        if (coop_pieces(wg, c) > 1) t.slot = static_cast<int>(ioff[n + w] - ioff[n] + piece);
     }
     items[i] = t;
 }
@@ -506,7 +511,7 @@ k_win_reduce(const uint32_t* __restrict__ boff, const uint32_t* __restrict__ iof
         const int k = static_cast<int>(u % K);
         uint32_t end;
         const WinRange x = window_range(boff, wg, w, end);
-        const uint32_t pieces = coop_pieces(end - x.start());
+        const uint32_t pieces = coop_pieces(wg, end - x.start());
         const uint32_t first = ioff[n + w] - ioff[n];
         if (first + pieces > max_slots) continue;          // never: the slot count bounds the pieces
         const float2* src = partial + (first + grp) * piece_size + static_cast<size_t>(k) * kWin * kWin + cell;
@@ -768,7 +773,8 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         }
         }
     }
-    // c rows 2k + axis for the terms kg .. kg + 7.
+    // c rows 2k + axis for the terms kg .. kg + 7.  Terms k >= K (when K is not a multiple of 8)
+    // are contracted like the others and dropped at write-out (T_k stays in [-1, 1]).
     {
         float vs = valid ? rec_v(L.axis ? q.y : q.x, q.w, L.vexp) : 0.0f;
         if constexpr (T::kDither) vs *= __builtin_amdgcn_rcpf(dith_u * dith_v);
@@ -781,7 +787,7 @@ __device__ __forceinline__ void window_chunk(const WinLane& L, const WItem& it, 
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            L.s_c[(2 * j + L.axis) * kStride + L.rl] = kg + j < K ? t0 * vs : 0.0f;
+            L.s_c[(2 * j + L.axis) * kStride + L.rl] = t0 * vs;
             const float t2 = s2 * t1 - t0;
             t0 = t1;
             t1 = t2;
@@ -1115,10 +1121,15 @@ size_t window_bins(const GridParams& gp, int nw, int cls_log2) {
 }
 }  // namespace
 
+// The class height in use: the tuned one, else by LDS fit.
+static int effective_class_log2(const GridParams& gp, int nw) {
+    return gp.class_log2 == 2 || gp.class_log2 == 3 ? gp.class_log2 : window_class_log2(gp, nw);
+}
+
 bool grid_windows_supported(const SampleInput& in, const GridParams& gp) {
     if (gp.G > 8192 || gp.G < 1) return false;
     const int nw = (gp.G + kWin - 1) / kWin;
-    if (window_bins(gp, nw, window_class_log2(gp, nw)) >= (1ull << 30)) return false;
+    if (window_bins(gp, nw, effective_class_log2(gp, nw)) >= (1ull << 30)) return false;
     return 2 * in.num_samples < (1ull << 31);             // records: at most two per sample
 }
 
@@ -1128,7 +1139,9 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     if (!grid_windows_supported(in, gp)) throw UnsupportedError("grid or sample count too large for the window gridder");
     WinGeom wg;
     wg.nw = (gp.G + kWin - 1) / kWin;
-    wg.cls_log2 = window_class_log2(gp, wg.nw);
+    wg.cls_log2 = effective_class_log2(gp, wg.nw);
+    wg.coop_min = gp.coop_min != 0 ? gp.coop_min : kCoopMinDefault;
+    wg.coop_cap = gp.coop_cap != 0 ? gp.coop_cap : kCoopCapDefault;
     wg.ncls = 24 >> wg.cls_log2;
     wg.nh = (gp.G + (1 << wg.cls_log2) - 1) >> wg.cls_log2;
     wg.nwin = wg.nw * wg.nw;
@@ -1140,11 +1153,11 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     const size_t nbins = nbins_sz;
     const size_t nwt = static_cast<size_t>(wg.nwin_total);
     // Wavefront items: one per window at most.  Workgroup pieces: at most one per window plus
-    // reads / kCoopCap (a window reads each record at most twice).  Partial slots: only windows
-    // with two or more pieces have them, at most 2 reads / kCoopCap.
+    // reads / coop_cap (a window reads each record at most twice).  Partial slots: only windows
+    // with two or more pieces have them, at most 2 reads / coop_cap.
     const size_t max_reads = 2 * max_records;
-    const size_t max_items = 2 * nwt + max_reads / kCoopCap + 1;
-    const size_t max_slots = 2 * max_reads / kCoopCap + 1;
+    const size_t max_items = 2 * nwt + max_reads / wg.coop_cap + 1;
+    const size_t max_slots = 2 * max_reads / wg.coop_cap + 1;
     unsigned blocks;
     if (in.from_snapshot) blocks = static_cast<unsigned>((in.num_rows + kBinRows - 1) / kBinRows);
     else blocks = static_cast<unsigned>((S + kBinThreads * kBinPerThread - 1) / (kBinThreads * kBinPerThread));

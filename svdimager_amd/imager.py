@@ -150,6 +150,22 @@ def image_snapshot(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_si
     return image, info.as_dict()
 
 
+def autotune(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_size: float, repeats: int = 2,
+             **options) -> Dict:
+    """svdi_autotune: time the MFMA gridder's candidate work shapes on this snapshot and size,
+    keep the fastest on the context for this grid size, and return it (with its grid-stage time
+    and the defaults' time, ms)."""
+    from ._lib import Tuning
+    dev = dsnap.tensors["uvw"].device
+    opts = default_options(**options)
+    best = Tuning()
+    _bind_stream(ctx, dev)
+    s = dsnap.struct()
+    check(lib().svdi_autotune(ctx.handle, ctypes.byref(s), int(image_size), float(cell_size), ctypes.byref(opts),
+                              int(repeats), ctypes.byref(best)))
+    return best.as_dict()
+
+
 def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size, cell_size,
                          images=None, wait: bool = True, **options) -> Tuple[List, List[Dict]]:
     """Image several device-resident snapshots in one call (concurrent lanes); returns (images,

@@ -66,6 +66,11 @@ def test_null_context_is_an_error(L):
         assert L.svdi_last_error()
     assert L.svdi_workspace_bytes(None, ctypes.byref(n)) != 0 and n.value == 7
     assert L.svdi_trim(None) != 0
+    from svdimager_amd._lib import Tuning
+    t = Tuning()
+    assert L.svdi_set_tuning(None, 0, ctypes.byref(t)) == 1 and L.svdi_last_error()
+    assert L.svdi_get_tuning(None, 0, ctypes.byref(t)) == 1
+    assert L.svdi_autotune(None, None, 1024, 1e-5, None, 1, ctypes.byref(t)) == 1
 
 
 def test_fits_round_trip(tmp_path):
