@@ -235,6 +235,20 @@ int svdi_image_snapshots(svdi_context* ctx, const svdi_snapshot* snapshots, size
                          float* const* images, const size_t* image_sizes, const float* cell_sizes,
                          const svdi_options* opts, svdi_plan_info* infos, int wait);
 
+/* svdi_image_snapshots on plans earlier calls reported (no counterpart in the reference; the
+ * batching form's graph): plans[i] is snapshot i's svdi_plan_info from a call on the same uvw,
+ * frequencies, flags, image size and cell (as svdi_image_snapshot_planned).  Every snapshot's
+ * work is enqueued on the concurrent lanes without any wait on the host; the lanes fork from the
+ * context stream and the context stream joins them, so the call returns at once and the images
+ * are complete when that stream reaches this point.  Nothing in the call allocates, waits or
+ * records a memset when the lanes and their workspaces already hold the batch's shapes (an
+ * earlier batched call of the same snapshots and sizes on the context), so the whole batch can be
+ * captured into one graph on the context's stream (svdi_set_stream); under capture a context
+ * without those lanes or shapes fails with SVDI_ERR_INVALID_ARGUMENT. */
+int svdi_image_snapshots_planned(svdi_context* ctx, const svdi_snapshot* snapshots, size_t count,
+                                 float* const* images, const size_t* image_sizes, const float* cell_sizes,
+                                 const svdi_options* opts, const svdi_plan_info* plans);
+
 /* Work shape of the MFMA window gridder (no counterpart in the reference, whose gridder has no
  * shape to tune; the batching form picks it per image size with svdi_autotune).  The gridder
  * contracts 16 x 16 windows of the grid; a window's records are contracted by one wavefront

@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
     "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
     "svdi_image_snapshot_planned", "svdi_set_tuning", "svdi_get_tuning", "svdi_autotune",
+    "svdi_image_snapshots_planned",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -174,6 +175,11 @@ def lib():
             "svdi_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
             "svdi_last_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
                                                      ctypes.c_int]),
+            "svdi_image_snapshots_planned": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz,
+                                                            ctypes.POINTER(ctypes.c_void_p),
+                                                            ctypes.POINTER(ctypes.c_size_t),
+                                                            ctypes.POINTER(ctypes.c_float),
+                                                            ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
             "svdi_set_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
             "svdi_get_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
             "svdi_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz, f, ctypes.POINTER(Options),

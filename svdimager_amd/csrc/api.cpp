@@ -603,7 +603,10 @@ size_t Workspace::bytes_held() const {
     return t;
 }
 
-std::unique_ptr<svdi_context> create_context(int device) {
+// blocking: the context's stream is ordered with the null stream (a caller's context); lanes of
+// batched calls take non-blocking streams, ordered only by the fork / join events with their
+// parent's stream (so that their work can also be captured into a graph with it).
+std::unique_ptr<svdi_context> create_context(int device, bool blocking = true) {
     int count = 0;
     SVDI_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) throw ArgError("no such HIP device");
@@ -612,7 +615,7 @@ std::unique_ptr<svdi_context> create_context(int device) {
     c->device = device;
     // A blocking stream: ordered with the null stream both ways, so a caller that works on the
     // null stream (torch's default stream) needs no synchronisation of its own around a call.
-    SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamDefault));
+    SVDI_HIP(hipStreamCreateWithFlags(&c->own_stream, blocking ? hipStreamDefault : hipStreamNonBlocking));
     c->stream = c->own_stream;
     SVDI_HIP(hipEventCreate(&c->ev_start));
     SVDI_HIP(hipEventCreate(&c->ev_stop));
@@ -654,6 +657,80 @@ struct NoAllocCall {
     NoAllocCall& operator=(const NoAllocCall&) = delete;
 };
 
+// The host state of a call on a plan an earlier call reported (svdi_plan_info) for image size N
+// and cell `cell`; throws ArgError for a plan no imaging call could have reported.
+CallState planned_state(size_t N, float cell, const svdi_plan_info& plan) {
+    const size_t G = (N * 3 + 1) / 2;
+    if (N == 0 || N > 46340) throw ArgError("Image_Size out of range");
+    if (!(cell > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
+    if (plan.grid_size != G) throw ArgError("the plan was made for another image size");
+    if (plan.num_terms < 1 || plan.num_terms > kMaxTerms || plan.num_slabs < 1 || plan.batch_slabs < 1 ||
+        plan.batch_slabs > static_cast<size_t>(plan.num_slabs) || !(plan.slab_width > 0.0f) ||
+        !(plan.slab_half_range > 0.0f) || !(plan.vis_det > 0.0f))
+        throw ArgError("the plan is not one an imaging call reported");
+    CallState cs;
+    cs.N = N;
+    cs.cell = cell;
+    cs.hp.semantics = Semantics::Integrated;
+    cs.hp.image_size = N;
+    cs.hp.grid_size = G;
+    cs.hp.cell = cell;
+    DevPlan& d = cs.hp.dev;
+    d = DevPlan();
+    d.r3_min = plan.r3_min;
+    d.r3_max = plan.r3_max;
+    d.num_slabs = plan.num_slabs;
+    d.slab_width = plan.slab_width;
+    d.slab_half_range = plan.slab_half_range;
+    d.inv_slab_half_range = plan.inv_slab_half_range;
+    d.r3_centre = plan.r3_centre;
+    d.slab_alpha_max = plan.slab_alpha_max;
+    cs.hp.num_terms = plan.num_terms;
+    cs.hp.term_error = plan.term_error;
+    cs.hp.batch_slabs = plan.batch_slabs;
+    cs.det = plan.vis_det;
+    return cs;
+}
+
+// For the duration of a planned call: no stage events, and under stream capture a frozen
+// workspace (a request for a shape it does not already hold fails instead of allocating).
+struct PlannedScope {
+    svdi_context* c;
+    bool profiling;
+    PlannedScope(svdi_context* ctx, bool capturing) : c(ctx), profiling(ctx->profiling) {
+        c->profiling = false;
+        c->ws.set_frozen(capturing);
+    }
+    ~PlannedScope() {
+        c->ws.set_frozen(false);
+        c->profiling = profiling;
+    }
+    PlannedScope(const PlannedScope&) = delete;
+    PlannedScope& operator=(const PlannedScope&) = delete;
+};
+
+// Snapshots dealt to L lanes largest first (cost ~ samples + 4 grid cells, the least-loaded lane
+// next): queue[j] lists lane j's snapshots in order.
+std::vector<std::vector<size_t>> deal_to_lanes(const svdi_snapshot* snaps, size_t count, const size_t* sizes,
+                                               size_t L) {
+    std::vector<size_t> order(count);
+    std::vector<double> cost(count);
+    for (size_t i = 0; i < count; ++i) {
+        order[i] = i;
+        const double G = static_cast<double>((sizes[i] * 3 + 1) / 2);
+        cost[i] = static_cast<double>(snaps[i].num_rows) * static_cast<double>(snaps[i].num_channels) + 4.0 * G * G;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<size_t>> queue(L);
+    std::vector<double> load(L, 0.0);
+    for (size_t i : order) {
+        const size_t j = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+        queue[j].push_back(i);
+        load[j] += cost[i];
+    }
+    return queue;
+}
+
 // The batched call: enqueue every snapshot on the lanes, then wait for the lanes or not.
 // sizes[i] / cells[i]: snapshot i's image side and cell.  Snapshots are dealt to the lanes
 // largest first (cost ~ samples + 4 grid cells, the least-loaded lane next), and each lane
@@ -674,24 +751,9 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
         DeviceGuard g(ctx->device);
         const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
         const size_t L = std::min(count, kBatchLanes);
-        while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device));
+        while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device, false));
         for (size_t j = 0; j < L; ++j) ctx->lanes[j]->tuning = ctx->tuning;
-        // Largest-first assignment of snapshots to lanes.
-        std::vector<size_t> order(count);
-        std::vector<double> cost(count);
-        for (size_t i = 0; i < count; ++i) {
-            order[i] = i;
-            const double G = static_cast<double>((sizes[i] * 3 + 1) / 2);
-            cost[i] = static_cast<double>(snaps[i].num_rows) * static_cast<double>(snaps[i].num_channels) + 4.0 * G * G;
-        }
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
-        std::vector<std::vector<size_t>> queue(L);
-        std::vector<double> load(L, 0.0);
-        for (size_t i : order) {
-            const size_t j = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
-            queue[j].push_back(i);
-            load[j] += cost[i];
-        }
+        const std::vector<std::vector<size_t>> queue = deal_to_lanes(snaps, count, sizes, L);
         // The lanes start after the caller's earlier work on the context stream, and the context
         // stream continues after the last lane's work.
         hipEvent_t ready = nullptr;
@@ -754,6 +816,55 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
             }
         if (infos)
             for (size_t i = 0; i < count; ++i) fill_info(cs[i], 0.0f, &infos[i]);
+    });
+}
+
+// The batched call on known plans: every snapshot's work is enqueued on the lanes with no wait
+// on the host (each lane takes its snapshots in turn), and the context stream waits for the
+// lanes.  Under stream capture on the context stream the lanes' streams join the capture
+// through the fork / join events, so the whole batch becomes one graph.
+int image_batch_planned(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
+                        const size_t* sizes, const float* cells, const svdi_options* opts,
+                        const svdi_plan_info* plans) {
+    return guarded([&] {
+        if (!ctx || !snaps || !images || !sizes || !cells || !plans)
+            throw ArgError("ctx / snapshots / images / sizes / cells / plans is null");
+        std::vector<CallState> cs(count);
+        for (size_t i = 0; i < count; ++i) {
+            check_device_pointer(images[i], "image");
+            cs[i] = planned_state(sizes[i], cells[i], plans[i]);
+        }
+        if (count == 0) return;
+        const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
+        if (o.semantics != SVDI_SEMANTICS_INTEGRATED) throw ArgError("planned calls take the integrated semantics");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        SVDI_HIP(hipStreamIsCapturing(ctx->stream, &capture));
+        const bool capturing = capture != hipStreamCaptureStatusNone;
+        const size_t L = std::min(count, kBatchLanes);
+        if (ctx->lanes.size() < L) {
+            if (capturing) throw ArgError("under capture the context needs its lanes from an earlier batched call");
+            while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device, false));
+        }
+        for (size_t j = 0; j < L; ++j) ctx->lanes[j]->tuning = ctx->tuning;
+        const std::vector<std::vector<size_t>> queue = deal_to_lanes(snaps, count, sizes, L);
+        // Fork: the lanes start after the caller's earlier work on the context stream (an event
+        // of the context's, so that nothing is created under capture).
+        SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
+        for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ctx->ev_plan, 0));
+        for (size_t j = 0; j < L; ++j) {
+            svdi_context* lane = ctx->lanes[j].get();
+            PlannedScope scope(lane, capturing);
+            for (size_t i : queue[j]) {
+                begin_call(lane);
+                SampleInput in = snapshot_input(lane, snaps[i]);
+                image_enqueue(lane, in, o, cs[i], images[i]);
+            }
+            // Join: the context stream continues after the lane's last snapshot.
+            SVDI_HIP(hipEventRecord(lane->ev_plan, lane->stream));
+            SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_plan, 0));
+        }
     });
 }
 
@@ -939,14 +1050,7 @@ int svdi_image_snapshot_planned(svdi_context* ctx, const svdi_snapshot* snap, fl
     return guarded([&] {
         if (!ctx || !snap || !plan) throw ArgError("ctx / snapshot / plan is null");
         check_device_pointer(image, "image");
-        const size_t N = image_size, G = (N * 3 + 1) / 2;
-        if (N == 0 || N > 46340) throw ArgError("Image_Size out of range");
-        if (!(cell_size > 0.0f)) throw ArgError("Cell_Size must be greater than zero");
-        if (plan->grid_size != G) throw ArgError("the plan was made for another image size");
-        if (plan->num_terms < 1 || plan->num_terms > kMaxTerms || plan->num_slabs < 1 || plan->batch_slabs < 1 ||
-            plan->batch_slabs > static_cast<size_t>(plan->num_slabs) || !(plan->slab_width > 0.0f) ||
-            !(plan->slab_half_range > 0.0f) || !(plan->vis_det > 0.0f))
-            throw ArgError("the plan is not one an imaging call reported");
+        CallState cs = planned_state(image_size, cell_size, *plan);
         svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
         if (o.semantics != SVDI_SEMANTICS_INTEGRATED) throw ArgError("planned calls take the integrated semantics");
         std::lock_guard<std::mutex> lk(ctx->mu);
@@ -954,43 +1058,17 @@ int svdi_image_snapshot_planned(svdi_context* ctx, const svdi_snapshot* snap, fl
         // Under stream capture the workspace must already hold this shape (no allocation).
         hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
         SVDI_HIP(hipStreamIsCapturing(ctx->stream, &capture));
-        struct Freeze {
-            svdi_context* c;
-            bool profiling;
-            Freeze(svdi_context* ctx, bool on) : c(ctx), profiling(ctx->profiling) {
-                c->profiling = false;
-                c->ws.set_frozen(on);
-            }
-            ~Freeze() {
-                c->ws.set_frozen(false);
-                c->profiling = profiling;
-            }
-        } freeze(ctx, capture != hipStreamCaptureStatusNone);
+        PlannedScope scope(ctx, capture != hipStreamCaptureStatusNone);
         begin_call(ctx);
         SampleInput in = snapshot_input(ctx, *snap);
-        CallState cs;
-        cs.N = N;
-        cs.cell = cell_size;
-        cs.hp.semantics = Semantics::Integrated;
-        cs.hp.image_size = N;
-        cs.hp.grid_size = G;
-        cs.hp.cell = cell_size;
-        DevPlan& d = cs.hp.dev;
-        d = DevPlan();
-        d.r3_min = plan->r3_min;
-        d.r3_max = plan->r3_max;
-        d.num_slabs = plan->num_slabs;
-        d.slab_width = plan->slab_width;
-        d.slab_half_range = plan->slab_half_range;
-        d.inv_slab_half_range = plan->inv_slab_half_range;
-        d.r3_centre = plan->r3_centre;
-        d.slab_alpha_max = plan->slab_alpha_max;
-        cs.hp.num_terms = plan->num_terms;
-        cs.hp.term_error = plan->term_error;
-        cs.hp.batch_slabs = plan->batch_slabs;
-        cs.det = plan->vis_det;
         image_enqueue(ctx, in, o, cs, image);
     });
+}
+
+int svdi_image_snapshots_planned(svdi_context* ctx, const svdi_snapshot* snapshots, size_t count,
+                                 float* const* images, const size_t* image_sizes, const float* cell_sizes,
+                                 const svdi_options* opts, const svdi_plan_info* plans) {
+    return svdi::image_batch_planned(ctx, snapshots, count, images, image_sizes, cell_sizes, opts, plans);
 }
 
 int svdi_image_snapshot_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count,

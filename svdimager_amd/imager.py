@@ -167,12 +167,16 @@ def autotune(ctx: Context, dsnap: DeviceSnapshot, image_size: int, cell_size: fl
 
 
 def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size, cell_size,
-                         images=None, wait: bool = True, **options) -> Tuple[List, List[Dict]]:
+                         images=None, wait: bool = True, plans: Optional[List[Dict]] = None,
+                         **options) -> Tuple[List, List[Dict]]:
     """Image several device-resident snapshots in one call (concurrent lanes); returns (images,
     plan dicts).  image_size / cell_size: one value for all, or one per snapshot (mixed sizes:
     svdi_image_snapshots).  `images`: optional list of (N_i, N_i) float32 device tensors.
     wait=False returns once the work is enqueued: the images are complete when the current
-    stream reaches this point, and the snapshots must stay alive until then."""
+    stream reaches this point, and the snapshots must stay alive until then.  plans=<the plan
+    dicts an earlier call returned for the same snapshots and sizes> calls
+    svdi_image_snapshots_planned: no planning and no host wait, so the batch can be captured in
+    one CUDA/HIP graph; it returns (images, plans) at once."""
     torch = _torch()
     n = len(dsnaps)
     if n == 0:
@@ -192,6 +196,14 @@ def image_snapshot_batch(ctx: Context, dsnaps: List[DeviceSnapshot], image_size,
     structs = (Snapshot * n)(*[d.struct() for d in dsnaps])
     ptrs = (ctypes.c_void_p * n)(*[_ptr(im) for im in images])
     _bind_stream(ctx, dev)
+    if plans is not None:
+        if len(plans) != n:
+            raise ValueError("one plan per snapshot")
+        pins = (PlanInfo * n)(*[PlanInfo.from_dict(p) for p in plans])
+        csz = (ctypes.c_size_t * n)(*sizes)
+        ccell = (ctypes.c_float * n)(*cells)
+        check(lib().svdi_image_snapshots_planned(ctx.handle, structs, n, ptrs, csz, ccell, ctypes.byref(opts), pins))
+        return images, list(plans)
     if mixed:
         csz = (ctypes.c_size_t * n)(*sizes)
         ccell = (ctypes.c_float * n)(*cells)

@@ -69,3 +69,51 @@ def test_graph_replays_with_new_visibilities():
             assert_parity(got, eager.cpu().numpy(), f"replay {rep} vs eager", whole=1e-5, central=1e-6)
         del graph
         torch.cuda.synchronize()
+
+
+def test_batch_graph_replays_mixed_sizes():
+    """svdi_image_snapshots_planned: three snapshots of mixed image sizes on known plans, captured
+    as one graph (the lanes fork from the capturing stream and it joins them), replayed with new
+    visibilities; every image of every replay is checked against an eager full call."""
+    snap = load_small_snapshot()
+    cell = snap["cell"]
+    sizes = [128, 64, 96]                      # G = 192, 96 and 145 (an odd grid: whole plane)
+    cells = [cell] * len(sizes)
+    with S.Context(0) as ctx:
+        ds = [S.DeviceSnapshot(snap, "cuda") for _ in sizes]
+        announce("planned batch, eager")
+        full, plans = S.image_snapshot_batch(ctx, ds, sizes, cells, output=S.OUTPUT_SIGNED)
+        imgs = [torch.zeros((N, N), dtype=torch.float32, device="cuda") for N in sizes]
+        # One eager planned batch sizes the lanes' workspaces for exactly this batch.
+        S.image_snapshot_batch(ctx, ds, sizes, cells, images=imgs, plans=plans, output=S.OUTPUT_SIGNED)
+        torch.cuda.synchronize()
+        for i, N in enumerate(sizes):
+            assert_parity(imgs[i].cpu().numpy(), full[i].cpu().numpy(), f"planned batch [{N}] vs full",
+                          whole=1e-5, central=1e-6)
+        announce("graph capture of a planned batch")
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            S.image_snapshot_batch(ctx, ds, sizes, cells, images=imgs, plans=plans, output=S.OUTPUT_SIGNED)
+        torch.cuda.synchronize()
+        base = [{k: d.tensors[k].clone() for k in VIS} for d in ds]
+        gen = torch.Generator(device="cuda").manual_seed(12)
+        for rep, scale in enumerate((1.0, 0.5, -1.5)):
+            for i, d in enumerate(ds):
+                for k in VIS:
+                    noise = torch.randn(base[i][k].shape, device="cuda", generator=gen)
+                    d.tensors[k].copy_(base[i][k] * (scale + 0.1 * i) + 0.05 * rep * noise)
+            torch.cuda.synchronize()
+            announce(f"batch graph replay {rep}")
+            for im in imgs:
+                im.fill_(float("nan"))
+            graph.replay()
+            torch.cuda.synchronize()
+            for i, (d, N) in enumerate(zip(ds, sizes)):
+                got = imgs[i].cpu().numpy()
+                eager, _ = S.image_snapshot(ctx, d, N, cell, output=S.OUTPUT_SIGNED)
+                torch.cuda.synchronize()
+                assert np.isfinite(got).all()
+                assert_parity(got, eager.cpu().numpy(), f"replay {rep} image {i} ({N}^2) vs eager",
+                              whole=1e-5, central=1e-6)
+        del graph
+        torch.cuda.synchronize()
