@@ -267,14 +267,29 @@ svdi_tuning tuning_for(const svdi_context* ctx, size_t G) {
     return t;
 }
 
-// Phase 2b, with the plan in cs: enqueue gridding, FFT and image pass per slab batch.  The image
-// (N x N floats, device) is overwritten.
-void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+// A call's buffers and parameters between the phases of image_enqueue.
+struct Enqueued {
+    GridParams gp;
+    ImageParams ip;
+    float2* planes = nullptr;
+    float2* rowfft = nullptr;
+    float2* colfft = nullptr;
+    float* conv = nullptr;
+    const float* vin_dev = nullptr;
+    int gridder = SVDI_GRIDDER_AUTO;
+    size_t total_slabs = 0;
+};
+
+// Phase 2b, with the plan in cs: the call's buffers, the image cleared, the grid-correction table,
+// the gridding and image-pass parameters, and the gridder resolved.
+Enqueued image_prepare(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+    Enqueued e;
     HostPlan& hp = cs.hp;
     const size_t N = cs.N, G = hp.grid_size;
     const float cell = cs.cell;
     const int K = hp.num_terms;
     const size_t total_slabs = static_cast<size_t>(hp.dev.num_slabs);
+    e.total_slabs = total_slabs;
 
     if (G % 2 == 0) {
         hp.row_lo = G / 2 - N / 2;
@@ -285,10 +300,11 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
         hp.rows = hp.cols = G;
     }
     const size_t P = hp.batch_slabs * static_cast<size_t>(K);
-    float2* planes = static_cast<float2*>(ctx->ws.get(Workspace::kPlanes, P * G * G * sizeof(float2)));
-    float2* rowfft = static_cast<float2*>(ctx->ws.get(Workspace::kRowFft, P * G * hp.cols * sizeof(float2)));
-    float2* colfft = static_cast<float2*>(ctx->ws.get(Workspace::kColFft, P * hp.rows * hp.cols * sizeof(float2)));
+    e.planes = static_cast<float2*>(ctx->ws.get(Workspace::kPlanes, P * G * G * sizeof(float2)));
+    e.rowfft = static_cast<float2*>(ctx->ws.get(Workspace::kRowFft, P * G * hp.cols * sizeof(float2)));
+    e.colfft = static_cast<float2*>(ctx->ws.get(Workspace::kColFft, P * hp.rows * hp.cols * sizeof(float2)));
     float* conv = static_cast<float*>(ctx->ws.get(Workspace::kConv, (N / 2 + 1) * sizeof(float)));
+    e.conv = conv;
 
     if (o.verbose) print_report(hp, std::cout);
 
@@ -300,7 +316,8 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
         ctx->conv_g = G;
     }
 
-    GridParams gp = {};
+    GridParams& gp = e.gp;
+    gp = GridParams();
     gp.G = static_cast<int>(G);
     gp.K = K;
     gp.total_slabs = static_cast<int>(total_slabs);
@@ -323,7 +340,8 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     gp.coop_cap = tune.coop_cap;
     gp.class_log2 = tune.class_log2;
 
-    ImageParams ip = {};
+    ImageParams& ip = e.ip;
+    ip = ImageParams();
     ip.N = static_cast<int>(N);
     ip.G = static_cast<int>(G);
     ip.K = K;
@@ -339,7 +357,7 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     ip.row_lo = static_cast<int>(hp.row_lo);
     ip.col_lo = static_cast<int>(hp.col_lo);
     ip.signed_out = o.output == SVDI_OUTPUT_SIGNED;
-    const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
+    e.vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
 
     // AUTO: the MFMA window gridder, or the FP32 tile gridder for the shapes it does not take
     // (judged at the largest slab batch).
@@ -348,29 +366,48 @@ void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options&
     const int gridder = o.gridder != SVDI_GRIDDER_AUTO ? o.gridder
                         : grid_windows_supported(in, probe) ? SVDI_GRIDDER_MFMA : SVDI_GRIDDER_TILED;
     cs.gridder = gridder;
-    for (size_t start = 0; start < total_slabs; start += hp.batch_slabs) {
-        const size_t nb = std::min(hp.batch_slabs, total_slabs - start);
-        gp.slab_start = static_cast<int>(start);
-        gp.nslabs = static_cast<int>(nb);
-        size_t tok = stage_begin(ctx, 2);
-        if (gridder == SVDI_GRIDDER_MFMA) {
-            cs.tile_totals = launch_grid_windows(ctx, in, gp, planes);
-        } else if (gridder == SVDI_GRIDDER_TILED) {
-            cs.tile_totals = launch_grid_tiled(ctx, in, gp, planes);
-        } else {
-            launch_grid_atomic(ctx, in, gp, planes);
-        }
-        stage_end(ctx, tok);
-        const size_t Pb = nb * static_cast<size_t>(K);
-        tok = stage_begin(ctx, 3);
-        launch_fft2_inverse(ctx, planes, Pb, G, hp.row_lo, hp.rows, hp.col_lo, hp.cols, rowfft, colfft);
-        stage_end(ctx, tok);
-        ip.P = static_cast<int>(Pb);
-        ip.slab_start = static_cast<int>(start);
-        ip.nslabs = static_cast<int>(nb);
-        tok = stage_begin(ctx, 4);
-        launch_image(ctx, colfft, vin_dev, conv, ip, image);
-        stage_end(ctx, tok);
+    e.gridder = gridder;
+    return e;
+}
+
+// Phase 2c: the slab batch [start, start + nb) of a prepared call gridded on `ctx`'s stream.
+void image_grid(svdi_context* ctx, const SampleInput& in, CallState& cs, Enqueued& e, size_t start, size_t nb) {
+    e.gp.slab_start = static_cast<int>(start);
+    e.gp.nslabs = static_cast<int>(nb);
+    const size_t tok = stage_begin(ctx, 2);
+    if (e.gridder == SVDI_GRIDDER_MFMA) {
+        cs.tile_totals = launch_grid_windows(ctx, in, e.gp, e.planes);
+    } else if (e.gridder == SVDI_GRIDDER_TILED) {
+        cs.tile_totals = launch_grid_tiled(ctx, in, e.gp, e.planes);
+    } else {
+        launch_grid_atomic(ctx, in, e.gp, e.planes);
+    }
+    stage_end(ctx, tok);
+}
+
+// Phase 2d: the FFT and image pass of a gridded slab batch [start, start + nb).
+void image_post(svdi_context* ctx, CallState& cs, Enqueued& e, size_t start, size_t nb, float* image) {
+    const HostPlan& hp = cs.hp;
+    const size_t Pb = nb * static_cast<size_t>(hp.num_terms);
+    size_t tok = stage_begin(ctx, 3);
+    launch_fft2_inverse(ctx, e.planes, Pb, hp.grid_size, hp.row_lo, hp.rows, hp.col_lo, hp.cols, e.rowfft, e.colfft);
+    stage_end(ctx, tok);
+    e.ip.P = static_cast<int>(Pb);
+    e.ip.slab_start = static_cast<int>(start);
+    e.ip.nslabs = static_cast<int>(nb);
+    tok = stage_begin(ctx, 4);
+    launch_image(ctx, e.colfft, e.vin_dev, e.conv, e.ip, image);
+    stage_end(ctx, tok);
+}
+
+// Phase 2 with the plan in cs: gridding, FFT and image pass per slab batch.  The image (N x N
+// floats, device) is overwritten.
+void image_enqueue(svdi_context* ctx, const SampleInput& in, const svdi_options& o, CallState& cs, float* image) {
+    Enqueued e = image_prepare(ctx, in, o, cs, image);
+    for (size_t start = 0; start < e.total_slabs; start += cs.hp.batch_slabs) {
+        const size_t nb = std::min(cs.hp.batch_slabs, e.total_slabs - start);
+        image_grid(ctx, in, cs, e, start, nb);
+        image_post(ctx, cs, e, start, nb, image);
     }
 }
 
@@ -820,9 +857,16 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
 }
 
 // The batched call on known plans: every snapshot's work is enqueued on the lanes with no wait
-// on the host (each lane takes its snapshots in turn), and the context stream waits for the
-// lanes.  Under stream capture on the context stream the lanes' streams join the capture
-// through the fork / join events, so the whole batch becomes one graph.
+// on the host, and the context stream waits for the lanes.  Under stream capture on the context
+// stream the lanes' streams join the capture through the fork / join events, so the whole batch
+// becomes one graph.
+//
+// When every snapshot's planes are gridded in one slab batch and the MFMA gridder takes them
+// (the common case), the snapshots go in groups of up to kMaxWindowJobs, one lane each: every
+// lane bins its snapshot, one window-contraction launch on the context stream takes the whole
+// group (persistent workgroups over all the group's windows), and the lanes then transform and
+// image their planes.  Otherwise each of up to four lanes takes its snapshots in turn, every
+// snapshot gridded on its lane.
 int image_batch_planned(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
                         const size_t* sizes, const float* cells, const svdi_options* opts,
                         const svdi_plan_info* plans) {
@@ -830,9 +874,11 @@ int image_batch_planned(svdi_context* ctx, const svdi_snapshot* snaps, size_t co
         if (!ctx || !snaps || !images || !sizes || !cells || !plans)
             throw ArgError("ctx / snapshots / images / sizes / cells / plans is null");
         std::vector<CallState> cs(count);
+        bool one_pass = true;
         for (size_t i = 0; i < count; ++i) {
             check_device_pointer(images[i], "image");
             cs[i] = planned_state(sizes[i], cells[i], plans[i]);
+            one_pass = one_pass && cs[i].hp.batch_slabs >= static_cast<size_t>(cs[i].hp.dev.num_slabs);
         }
         if (count == 0) return;
         const svdi_options o = resolve(opts, SVDI_SEMANTICS_INTEGRATED);
@@ -842,26 +888,67 @@ int image_batch_planned(svdi_context* ctx, const svdi_snapshot* snaps, size_t co
         hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
         SVDI_HIP(hipStreamIsCapturing(ctx->stream, &capture));
         const bool capturing = capture != hipStreamCaptureStatusNone;
-        const size_t L = std::min(count, kBatchLanes);
+        const bool grouped = one_pass && (o.gridder == SVDI_GRIDDER_AUTO || o.gridder == SVDI_GRIDDER_MFMA);
+        const size_t L = std::min(count, grouped ? static_cast<size_t>(kMaxWindowJobs) : kBatchLanes);
         if (ctx->lanes.size() < L) {
             if (capturing) throw ArgError("under capture the context needs its lanes from an earlier batched call");
             while (ctx->lanes.size() < L) ctx->lanes.push_back(create_context(ctx->device, false));
         }
         for (size_t j = 0; j < L; ++j) ctx->lanes[j]->tuning = ctx->tuning;
-        const std::vector<std::vector<size_t>> queue = deal_to_lanes(snaps, count, sizes, L);
-        // Fork: the lanes start after the caller's earlier work on the context stream (an event
-        // of the context's, so that nothing is created under capture).
+        PlannedScope ctx_scope(ctx, capturing);
+        std::vector<std::unique_ptr<PlannedScope>> scopes;
+        for (size_t j = 0; j < L; ++j) scopes.emplace_back(new PlannedScope(ctx->lanes[j].get(), capturing));
+        // Fork: the lanes start after the caller's earlier work on the context stream (events of
+        // the contexts', so that nothing is created under capture).
         SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
         for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ctx->ev_plan, 0));
+        if (grouped) {
+            std::vector<SampleInput> in(L);
+            std::vector<Enqueued> e(L);
+            for (size_t g0 = 0; g0 < count; g0 += L) {
+                const size_t n = std::min(L, count - g0);
+                std::vector<WindowJob> jobs;
+                std::vector<size_t> binned;                // lanes whose snapshot waits for the launch
+                for (size_t k = 0; k < n; ++k) {
+                    const size_t i = g0 + k;
+                    svdi_context* lane = ctx->lanes[k].get();
+                    begin_call(lane);
+                    in[k] = snapshot_input(lane, snaps[i]);
+                    e[k] = image_prepare(lane, in[k], o, cs[i], images[i]);
+                    if (e[k].gridder == SVDI_GRIDDER_MFMA) {
+                        e[k].gp.slab_start = 0;
+                        e[k].gp.nslabs = static_cast<int>(e[k].total_slabs);
+                        jobs.push_back(bin_windows(lane, in[k], e[k].gp, e[k].planes));
+                        SVDI_HIP(hipEventRecord(lane->ev_plan, lane->stream));
+                        binned.push_back(k);
+                    } else {
+                        image_grid(lane, in[k], cs[i], e[k], 0, e[k].total_slabs);
+                    }
+                }
+                if (!jobs.empty()) {
+                    for (size_t k : binned) SVDI_HIP(hipStreamWaitEvent(ctx->stream, ctx->lanes[k]->ev_plan, 0));
+                    contract_windows(ctx, jobs.data(), static_cast<int>(jobs.size()));
+                    for (const WindowJob& job : jobs) reduce_windows(ctx, job);
+                    SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
+                    for (size_t k : binned) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[k]->stream, ctx->ev_plan, 0));
+                }
+                for (size_t k = 0; k < n; ++k)
+                    image_post(ctx->lanes[k].get(), cs[g0 + k], e[k], 0, e[k].total_slabs, images[g0 + k]);
+            }
+        } else {
+            const std::vector<std::vector<size_t>> queue = deal_to_lanes(snaps, count, sizes, L);
+            for (size_t j = 0; j < L; ++j) {
+                svdi_context* lane = ctx->lanes[j].get();
+                for (size_t i : queue[j]) {
+                    begin_call(lane);
+                    SampleInput in = snapshot_input(lane, snaps[i]);
+                    image_enqueue(lane, in, o, cs[i], images[i]);
+                }
+            }
+        }
+        // Join: the context stream continues after every lane's last snapshot.
         for (size_t j = 0; j < L; ++j) {
             svdi_context* lane = ctx->lanes[j].get();
-            PlannedScope scope(lane, capturing);
-            for (size_t i : queue[j]) {
-                begin_call(lane);
-                SampleInput in = snapshot_input(lane, snaps[i]);
-                image_enqueue(lane, in, o, cs[i], images[i]);
-            }
-            // Join: the context stream continues after the lane's last snapshot.
             SVDI_HIP(hipEventRecord(lane->ev_plan, lane->stream));
             SVDI_HIP(hipStreamWaitEvent(ctx->stream, lane->ev_plan, 0));
         }

@@ -229,6 +229,29 @@ const uint32_t* launch_grid_tiled(svdi_context* ctx, const SampleInput& in, cons
                                   float2* planes);
 const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
                                     float2* planes);
+// The window gridder in three steps, so that one contraction launch (on the calling context's
+// stream) can take several slab batches, of several snapshots, binned on other streams:
+// bin_windows enqueues the binning of one slab batch, contract_windows the contraction of up to
+// kMaxWindowJobs binned batches of one MFMA precision, reduce_windows the sum of one batch's
+// split windows; launch_grid_windows is the three for one batch.
+constexpr int kMaxWindowJobs = 16;
+struct WindowJob {
+    const void* items = nullptr;
+    uint32_t* totals = nullptr;            // device: [0] records, [1] wavefront items, ...
+    void* records = nullptr;
+    float2* planes = nullptr;
+    float2* partial = nullptr;
+    float2* trash = nullptr;
+    const uint32_t* boff = nullptr;
+    const uint32_t* ioff = nullptr;
+    const uint32_t* multi = nullptr;
+    uint32_t max_slots = 0;
+    int K = 0, G = 0, prec = 0;
+    int geom[16] = {};                     // the window geometry (windows.hip)
+};
+WindowJob bin_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp, float2* planes);
+void contract_windows(svdi_context* ctx, const WindowJob* jobs, int count);
+void reduce_windows(svdi_context* ctx, const WindowJob& job);
 // Whether the window gridder takes this grid and sample count (else launch_grid_windows throws
 // UnsupportedError).
 bool grid_windows_supported(const SampleInput& in, const GridParams& gp);

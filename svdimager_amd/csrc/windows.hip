@@ -35,6 +35,8 @@
 #include "es_poly.hpp"
 #include "records.hpp"
 
+#include <cstring>
+
 namespace svdi {
 namespace {
 
@@ -894,22 +896,58 @@ __device__ __forceinline__ void window_store(const WinOut& o, const WItem& it, c
     *dst = make_float2(v.x * sg, v.y * sg);
 }
 
-// Persistent workgroups, one per CU.  First the workgroup items: all wavefronts of the workgroup
-// claim the item's chunks from a shared counter, then sum their accumulators in LDS (over the
-// staging area) and the workgroup writes the window once.  Then the wavefront items, each
-// wavefront on its own:
-// the next item is claimed while the current one is contracted and its first records are
-// requested before the current window is written, so the write-out overlaps the next item's
-// loads (vector memory operations complete in order).  The terms are done eight at a time
-// (one 16-row block of m = 2k + ri).
+// One job of the window kernel: one slab batch of one snapshot, binned (records, item lists and
+// totals in its call's workspace), with its own planes.  A launch takes up to kMaxJobs of them.
+constexpr int kMaxJobs = kMaxWindowJobs;
+struct WinJob {
+    const WItem* items;
+    uint32_t* totals;
+    WRecords rec;
+    float2* planes;
+    float2* partial;
+    float2* trash;
+    WinGeom wg;
+    int K, G;
+};
+struct WinJobs {
+    WinJob job[kMaxJobs];
+    int count;
+};
+
+// The c scale of a job from the largest |v| of its call: |c| <= 2^14 after scaling.
+__device__ __forceinline__ void job_scale(const WinJob& J, WinLane& L) {
+    const float vmax = __uint_as_float(J.totals[4]);
+    int e = 0;
+    if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
+    e = min(max(e, -100), 100);
+    L.vexp = 14 - e;
+    L.inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
+}
+
+// The job of global item index g of a queue whose per-job starts are off[0 .. count] (LDS):
+// the largest j with off[j] <= g (wave-uniform).
+__device__ __forceinline__ int job_of(const int* off, int count, int g) {
+    int j = 0;
+    while (j + 1 < count && __builtin_amdgcn_readfirstlane(off[j + 1]) <= g) ++j;
+    return j;
+}
+
+// Persistent workgroups, one per CU, over the items of all the launch's jobs.  First the
+// workgroup items of every job, from one queue: all wavefronts of the workgroup claim the item's
+// chunks from a shared counter, then sum their accumulators in LDS (over the staging area) and
+// the workgroup writes the window once.  Then the wavefront items of every job, each wavefront
+// on its own: the next item is claimed while the current one is contracted and its first records
+// are requested before the current window is written, so the write-out overlaps the next item's
+// loads (vector memory operations complete in order).  The terms are done eight at a time (one
+// 16-row block of m = 2k + ri).
 template <int P>
 __global__ void __launch_bounds__(WinTraits<P>::kWaves * 64)
-k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WRecords rec, GridParams gp,
-           WinGeom wg, float2* __restrict__ planes, float2* __restrict__ partial, float2* __restrict__ trash) {
+k_win_grid(WinJobs jobs) {
     using T = WinTraits<P>;
     constexpr int kWaves = T::kWaves;
     extern __shared__ __align__(16) float stage_all[];
     __shared__ int s_item, s_chunk, s_witem;
+    __shared__ int s_coop_off[kMaxJobs + 1], s_wave_off[kMaxJobs + 1];
     WinLane L;
     const int wave = threadIdx.x >> 6;
     L.lane = threadIdx.x & 63;
@@ -938,33 +976,32 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     L.s_kh = (L.axis ? L.s_kvh : L.s_kuh) + L.rl;
     L.s_kl = (L.axis ? L.s_kvl : L.s_kul) + L.rl;
     if constexpr (T::kBf16) L.s_kf = (L.axis ? L.s_kvf : L.s_kuf) + L.rl;
-    // Scale of c from the largest |v| of the call: |c| <= 2^14 after scaling.
-    {
-        const float vmax = __uint_as_float(totals[4]);
-        int e = 0;
-        if (vmax > 0.0f && vmax <= 3.0e38f) (void)frexpf(vmax, &e);
-        e = min(max(e, -100), 100);
-        L.vexp = 14 - e;
-        L.inv = ldexpf(1.0f, e - 14 - kEsScaleLog2Sq);
-    }
-    const int K = gp.K, G = gp.G;
     const int lane = L.lane;
-    const int nwave_items = static_cast<int>(totals[1]);
-    const int ncoop = static_cast<int>(totals[3]);
-    const WItem* coop_items = items + nwave_items;
+    const int njobs = jobs.count;
 
-    // Workgroup items.  The claimed index is read back into a scalar register, so the loop exit is
-    // wave-uniform by construction and every wavefront runs the same barriers: were it left in a
-    // vector register, the compiler could treat the exit as divergent and let lanes other than
-    // thread 0 iterate without the claim.  Within an item the wavefronts claim chunks from an LDS
-    // counter (the SIMDs issue their oldest wavefront first, so an even split would leave the
-    // item waiting for the youngest); each claims its next chunk before contracting the current
-    // one, and requests that chunk's records at once.
+    // The jobs' item counts as queue offsets (workgroup items first, then wavefront items, each
+    // job's list in its largest-first order), and the first workgroup item claimed from the one
+    // queue head (job 0's totals[5]).  The claimed index is read back into a scalar register, so
+    // the loop exit is wave-uniform by construction and every wavefront runs the same barriers:
+    // were it left in a vector register, the compiler could treat the exit as divergent and let
+    // lanes other than thread 0 iterate without the claim.  Within an item the wavefronts claim
+    // chunks from an LDS counter (the SIMDs issue their oldest wavefront first, so an even split
+    // would leave the item waiting for the youngest); each claims its next chunk before
+    // contracting the current one, and requests that chunk's records at once.
     const int wave_id = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
     if (threadIdx.x == 0) {
+        int c = 0, w = 0;
+        for (int j = 0; j < njobs; ++j) {
+            s_coop_off[j] = c;
+            s_wave_off[j] = w;
+            c += static_cast<int>(jobs.job[j].totals[3]);
+            w += static_cast<int>(jobs.job[j].totals[1]);
+        }
+        s_coop_off[njobs] = c;
+        s_wave_off[njobs] = w;
         s_chunk = 0;
         s_witem = 0;
-        s_item = static_cast<int>(atomicAdd(&totals[5], 1u));
+        s_item = static_cast<int>(atomicAdd(&jobs.job[0].totals[5], 1u));
     }
     auto claim_chunk = [&]() -> int {
         int v = 0;
@@ -974,24 +1011,30 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
     for (;;) {
         __syncthreads();
         const int gi = __builtin_amdgcn_readfirstlane(s_item);
+        const int ncoop = __builtin_amdgcn_readfirstlane(s_coop_off[njobs]);
         __syncthreads();
         if (gi >= ncoop) break;
         // The next item is claimed now; wavefront 0 publishes it after the last barrier of this one.
         uint32_t gnext = 0;
-        if (wave_id == 0 && lane == 0) gnext = atomicAdd(&totals[5], 1u);
-        const WItem it = coop_items[gi];
+        if (wave_id == 0 && lane == 0) gnext = atomicAdd(&jobs.job[0].totals[5], 1u);
+        const int jj = job_of(s_coop_off, njobs, gi);
+        const WinJob& J = jobs.job[jj];
+        job_scale(J, L);
+        const int K = J.K, G = J.G;
+        const int nwave_j = __builtin_amdgcn_readfirstlane(s_wave_off[jj + 1] - s_wave_off[jj]);
+        const WItem it = J.items[nwave_j + (gi - __builtin_amdgcn_readfirstlane(s_coop_off[jj]))];
         for (int kg = 0; kg < K; kg += 8) {
             floatx4 d[kWin];
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
             int c0 = claim_chunk();
             uint4 q_next = make_uint4(0u, 0u, 0u, 0u);
-            if (c0 < it.count) q_next = rec.get(it.start + min(c0 + L.rl, it.count - 1));
+            if (c0 < it.count) q_next = J.rec.get(it.start + min(c0 + L.rl, it.count - 1));
             while (c0 < it.count) {
                 const uint4 q = q_next;
                 const int c1 = claim_chunk();
-                if (c1 < it.count) q_next = rec.get(it.start + min(c1 + L.rl, it.count - 1));
-                window_chunk<P>(L, it, c0, kg, K, wg.cls_log2, q, d);
+                if (c1 < it.count) q_next = J.rec.get(it.start + min(c1 + L.rl, it.count - 1));
+                window_chunk<P>(L, it, c0, kg, K, J.wg.cls_log2, q, d);
                 c0 = c1;
             }
             // Sum the wavefronts' accumulators in LDS (over the staging area), in a fixed order:
@@ -1031,65 +1074,73 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
                     const float4 x = red4[(j * kWin + a) * 64 + sl];
                     acc = make_float4(acc.x + x.x, acc.y + x.y, acc.z + x.z, acc.w + x.w);
                 }
-                const WinOut os = window_out(it, wg, K, G, planes, partial, sl & 15);
-                window_store(os, it, wg, G, K, a, kg + 2 * (sl >> 4), sl & 15, L.inv, make_float2(acc.x, acc.y), trash, sl);
-                window_store(os, it, wg, G, K, a, kg + 2 * (sl >> 4) + 1, sl & 15, L.inv, make_float2(acc.z, acc.w), trash, sl);
+                const WinOut os = window_out(it, J.wg, K, G, J.planes, J.partial, sl & 15);
+                window_store(os, it, J.wg, G, K, a, kg + 2 * (sl >> 4), sl & 15, L.inv, make_float2(acc.x, acc.y), J.trash, sl);
+                window_store(os, it, J.wg, G, K, a, kg + 2 * (sl >> 4) + 1, sl & 15, L.inv, make_float2(acc.z, acc.w), J.trash, sl);
             }
             __syncthreads();
         }
         if (wave_id == 0 && lane == 0) s_item = static_cast<int>(gnext);
     }
 
-    // Wavefront items.  Workgroup g takes items g, g + G, g + 2G, ... of the (largest first) list,
+    // Wavefront items of all jobs, one list.  Workgroup g takes items g, g + G, g + 2G, ... of it,
     // G = the number of workgroups, so the CUs get equal shares without a global queue; its
     // wavefronts claim them from an LDS counter, each claiming its next item while it contracts
     // the current one.
+    const int nwave = __builtin_amdgcn_readfirstlane(s_wave_off[njobs]);
     const int nwg = static_cast<int>(gridDim.x);
-    const int wg_items = nwave_items > static_cast<int>(blockIdx.x)
-                             ? (nwave_items - static_cast<int>(blockIdx.x) + nwg - 1) / nwg : 0;
+    const int wg_items = nwave > static_cast<int>(blockIdx.x)
+                             ? (nwave - static_cast<int>(blockIdx.x) + nwg - 1) / nwg : 0;
     auto claim_item = [&]() -> int {
         int v = 0;
         if (lane == 0) v = atomicAdd(&s_witem, 1);
         return v;
     };
     int ticket = claim_item();
-    int j = __builtin_amdgcn_readfirstlane(ticket);
-    if (j >= wg_items) return;
-    int item = static_cast<int>(blockIdx.x) + j * nwg;
-    WItem it = items[item];
+    const int j0 = __builtin_amdgcn_readfirstlane(ticket);
+    if (j0 >= wg_items) return;
+    int item = static_cast<int>(blockIdx.x) + j0 * nwg;
+    int jj = job_of(s_wave_off, njobs, item);
+    WItem it = jobs.job[jj].items[item - __builtin_amdgcn_readfirstlane(s_wave_off[jj])];
     // Records of the next chunk, loaded while the current one is contracted.  Every lane loads
     // (lanes past the end re-read the last record and ignore it), so the loads land in place.
-    uint4 q_next = rec.get(it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0);
+    uint4 q_next = jobs.job[jj].rec.get(it.count > 0 ? it.start + min(L.rl, it.count - 1) : 0);
     for (;;) {
         ticket = claim_item();
-        int next = nwave_items;
+        const WinJob& J = jobs.job[jj];
+        job_scale(J, L);
+        const int K = J.K, G = J.G;
+        int next = nwave, jn = jj;
         WItem it_next = it;
-        const WinOut o = window_out(it, wg, K, G, planes, partial, L.n16);
+        const WinOut o = window_out(it, J.wg, K, G, J.planes, J.partial, L.n16);
         for (int kg = 0; kg < K; kg += 8) {
             floatx4 d[kWin];
 #pragma unroll
             for (int a = 0; a < kWin; ++a) d[a] = floatx4{0.f, 0.f, 0.f, 0.f};
             for (int c0 = 0; c0 < it.count; c0 += kChunk) {
                 const uint4 q = q_next;
-                if (c0 + kChunk < it.count) q_next = rec.get(it.start + min(c0 + kChunk + L.rl, it.count - 1));
-                window_chunk<P>(L, it, c0, kg, K, wg.cls_log2, q, d);
+                if (c0 + kChunk < it.count) q_next = J.rec.get(it.start + min(c0 + kChunk + L.rl, it.count - 1));
+                window_chunk<P>(L, it, c0, kg, K, J.wg.cls_log2, q, d);
             }
-            // What comes next: this item's next term block, or the next item.  Its first
-            // records are requested now, ahead of the stores below.
+            // What comes next: this item's next term block, or the next item (possibly of
+            // another job).  Its first records are requested now, ahead of the stores below.
             {
                 int pstart = it.start, pcount = it.count;
+                const WRecords* prec = &J.rec;
                 if (kg + 8 >= K) {
-                    const int jn = __builtin_amdgcn_readfirstlane(ticket);
-                    next = jn < wg_items ? static_cast<int>(blockIdx.x) + jn * nwg : nwave_items;
-                    if (next < nwave_items) {
-                        it_next = items[next];
+                    const int tn = __builtin_amdgcn_readfirstlane(ticket);
+                    next = tn < wg_items ? static_cast<int>(blockIdx.x) + tn * nwg : nwave;
+                    if (next < nwave) {
+                        jn = job_of(s_wave_off, njobs, next);
+                        it_next = jobs.job[jn].items[next - __builtin_amdgcn_readfirstlane(s_wave_off[jn])];
                         pstart = it_next.start;
                         pcount = it_next.count;
+                        prec = &jobs.job[jn].rec;
                     } else {
                         pcount = 0;
                     }
                 }
-                q_next = rec.get(pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0);
+                q_next = prec->get(pcount > 0 ? pstart + min(L.rl, pcount - 1) : 0);
             }
             // Rows m = 4 (lane / 16) + i are (re, im) of terms kg + 2 (lane / 16) and
             // kg + 2 (lane / 16) + 1; column n16.
@@ -1097,11 +1148,12 @@ k_win_grid(const WItem* __restrict__ items, uint32_t* __restrict__ totals, WReco
             for (int a = 0; a < kWin; ++a)
 #pragma unroll
                 for (int h = 0; h < 2; ++h)
-                    window_store(o, it, wg, G, K, a, kg + 2 * (lane >> 4) + h, L.n16, L.inv,
-                                 make_float2(d[a][2 * h], d[a][2 * h + 1]), trash, lane);
+                    window_store(o, it, J.wg, G, K, a, kg + 2 * (lane >> 4) + h, L.n16, L.inv,
+                                 make_float2(d[a][2 * h], d[a][2 * h + 1]), J.trash, lane);
         }
-        if (next >= nwave_items) break;
+        if (next >= nwave) break;
         item = next;
+        jj = jn;
         it = it_next;
     }
 }
@@ -1133,8 +1185,7 @@ bool grid_windows_supported(const SampleInput& in, const GridParams& gp) {
     return 2 * in.num_samples < (1ull << 31);             // records: at most two per sample
 }
 
-const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
-                                    float2* planes) {
+WindowJob bin_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp, float2* planes) {
     if (gp.K < 1 || gp.K > kMaxTerms) throw std::logic_error("term count out of range");
     if (!grid_windows_supported(in, gp)) throw UnsupportedError("grid or sample count too large for the window gridder");
     WinGeom wg;
@@ -1233,6 +1284,42 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
     }
     SVDI_LAUNCH_CHECK();
 
+    WindowJob job;
+    static_assert(sizeof(WinGeom) <= sizeof(job.geom), "window geometry does not fit the job");
+    std::memcpy(job.geom, &wg, sizeof wg);
+    job.items = items;
+    job.totals = totals;
+    job.records = rec.r;
+    job.planes = planes;
+    job.partial = partial;
+    job.trash = trash;
+    job.boff = boff;
+    job.ioff = ioff;
+    job.multi = multi;
+    job.max_slots = static_cast<uint32_t>(max_slots);
+    job.K = gp.K;
+    job.G = gp.G;
+    job.prec = gp.prec;
+    return job;
+}
+
+void contract_windows(svdi_context* ctx, const WindowJob* jobs, int count) {
+    if (count < 1 || count > kMaxJobs) throw std::logic_error("window job count out of range");
+    WinJobs wj = {};
+    wj.count = count;
+    for (int j = 0; j < count; ++j) {
+        if (jobs[j].prec != jobs[0].prec) throw std::logic_error("window jobs of one launch differ in precision");
+        WinJob& d = wj.job[j];
+        d.items = static_cast<const WItem*>(jobs[j].items);
+        d.totals = jobs[j].totals;
+        d.rec.r = static_cast<uint4*>(jobs[j].records);
+        d.planes = jobs[j].planes;
+        d.partial = jobs[j].partial;
+        d.trash = jobs[j].trash;
+        std::memcpy(&d.wg, jobs[j].geom, sizeof d.wg);
+        d.K = jobs[j].K;
+        d.G = jobs[j].G;
+    }
     const size_t tok = stage_begin(ctx, 5);
     const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
     auto launch = [&](auto prec) {
@@ -1240,18 +1327,31 @@ const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, co
         using T = WinTraits<P>;
         constexpr size_t lds = sizeof(float) * T::kStageFloats * T::kWaves;
         allow_dynamic_lds(reinterpret_cast<const void*>(&k_win_grid<P>), lds);
-        k_win_grid<P><<<grid, T::kWaves * 64, lds, ctx->stream>>>(items, totals, rec, gp, wg, planes, partial, trash);
+        k_win_grid<P><<<grid, T::kWaves * 64, lds, ctx->stream>>>(wj);
     };
-    if (gp.prec == kPrecBf16) launch(std::integral_constant<int, kPrecBf16>());
-    else if (gp.prec == kPrecSingle) launch(std::integral_constant<int, kPrecSingle>());
-    else if (gp.prec == kPrecSplitC) launch(std::integral_constant<int, kPrecSplitC>());
+    const int prec = jobs[0].prec;
+    if (prec == kPrecBf16) launch(std::integral_constant<int, kPrecBf16>());
+    else if (prec == kPrecSingle) launch(std::integral_constant<int, kPrecSingle>());
+    else if (prec == kPrecSplitC) launch(std::integral_constant<int, kPrecSplitC>());
     else launch(std::integral_constant<int, kPrecSplit3>());
     SVDI_LAUNCH_CHECK();
-    k_win_reduce<<<2048, kReduceGroups * 256, 0, ctx->stream>>>(boff, ioff, multi, totals, wg, gp.G, gp.K,
-                                                static_cast<uint32_t>(max_slots), partial, planes);
-    SVDI_LAUNCH_CHECK();
     stage_end(ctx, tok);
-    return totals;
+}
+
+void reduce_windows(svdi_context* ctx, const WindowJob& job) {
+    WinGeom wg;
+    std::memcpy(&wg, job.geom, sizeof wg);
+    k_win_reduce<<<2048, kReduceGroups * 256, 0, ctx->stream>>>(job.boff, job.ioff, job.multi, job.totals, wg, job.G,
+                                                                job.K, job.max_slots, job.partial, job.planes);
+    SVDI_LAUNCH_CHECK();
+}
+
+const uint32_t* launch_grid_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp,
+                                    float2* planes) {
+    const WindowJob job = bin_windows(ctx, in, gp, planes);
+    contract_windows(ctx, &job, 1);
+    reduce_windows(ctx, job);
+    return job.totals;
 }
 
 }  // namespace svdi

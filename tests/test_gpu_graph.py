@@ -117,3 +117,31 @@ def test_batch_graph_replays_mixed_sizes():
                               whole=1e-5, central=1e-6)
         del graph
         torch.cuda.synchronize()
+
+
+def test_grouped_planned_batch_matches_single_calls():
+    """svdi_image_snapshots_planned's grouped path: 18 snapshots (two groups, of 16 and 2) of three
+    image sizes, each group binned on its lanes and contracted by one window-kernel launch over
+    all its snapshots' windows; every image against the same snapshot imaged alone."""
+    snap = load_small_snapshot()
+    cell = snap["cell"]
+    sizes = [64, 96, 128] * 6
+    with S.Context(0) as ctx:
+        base = S.DeviceSnapshot(snap, "cuda")
+        ds = []
+        gen = torch.Generator(device="cuda").manual_seed(13)
+        for i, _ in enumerate(sizes):
+            d = S.DeviceSnapshot(snap, "cuda")
+            for k in VIS:
+                d.tensors[k].copy_(base.tensors[k] * (1.0 + 0.05 * i)
+                                   + 0.01 * torch.randn(base.tensors[k].shape, device="cuda", generator=gen))
+            ds.append(d)
+        _, plans = S.image_snapshot_batch(ctx, ds, sizes, [cell] * len(sizes), output=S.OUTPUT_SIGNED)
+        announce("grouped planned batch: 18 snapshots")
+        imgs, _ = S.image_snapshot_batch(ctx, ds, sizes, [cell] * len(sizes), plans=plans, output=S.OUTPUT_SIGNED)
+        torch.cuda.synchronize()
+        for i, (d, N) in enumerate(zip(ds, sizes)):
+            ref, _ = S.image_snapshot(ctx, d, N, cell, output=S.OUTPUT_SIGNED)
+            torch.cuda.synchronize()
+            assert_parity(imgs[i].cpu().numpy(), ref.cpu().numpy(), f"grouped [{i}] ({N}^2) vs single",
+                          whole=1e-5, central=1e-6)
