@@ -126,9 +126,10 @@ void svdi_default_options(svdi_options* opts);
 
 /* Context: one HIP device, one stream, a workspace reused across calls.  Calls on one context
  * are serialised; use one context per host thread for concurrency.  The context's own stream
- * (and each batch lane's) is a blocking stream, ordered with the null stream in both
- * directions: work the caller put on the null stream before a call is complete before the call
- * reads its inputs, and null-stream work after an asynchronous call sees its outputs. */
+ * is a blocking stream, ordered with the null stream in both directions: work the caller put on
+ * the null stream before a call is complete before the call reads its inputs, and null-stream
+ * work after an asynchronous call sees its outputs.  Batched calls run on lanes that fork from
+ * the context stream and that it joins (events), so the same holds for them. */
 int svdi_create(svdi_context** ctx, int device);
 int svdi_destroy(svdi_context* ctx);
 /* Run subsequent work on a caller stream (hipStream_t); NULL restores the context's stream. */
