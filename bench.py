@@ -419,22 +419,34 @@ def serve_main(args):
     budget = int(args.serve_budget_gib * (1 << 30))
     results = {}
     workspace = {}
-    for name, gridder in (("mfma", S.GRIDDER_MFMA), ("plain-fp32", S.GRIDDER_TILED)):
+    # The MFMA imager through the library's own request loop (svdi_service_*) and through the
+    # Python loop of serve.py, and the plain FP32 imager through the Python loop.
+    for name, gridder, native in (("mfma", S.GRIDDER_MFMA, True), ("mfma-python-loop", S.GRIDDER_MFMA, False),
+                                  ("plain-fp32", S.GRIDDER_TILED, False)):
         ctx = S.Context(0)
         opts = dict(gridder=gridder, output=S.OUTPUT_DIRTY)
-        svc = serve.SnapshotService(ctx, budget, max_batch=8, max_inflight=2, **opts)
+        make = serve.NativeService if native else serve.SnapshotService
+        svc = make(ctx, budget, max_batch=8, max_inflight=2, **opts)
         workspace[name] = svc.reserve([serve.Request(i, d, N, c, 0.0) for i, (d, N, c) in enumerate(pool)])
         log(f"serve[{name}]: workspace {workspace[name] / 2**30:.2f} GiB after the reserve pass")
         serve.run_poisson(svc, pool, rate=min(rates), duration=0.3, seed=12345)     # warm-up
+        if native:
+            svc.close()
         rows = []
         for rate in rates:
-            svc = serve.SnapshotService(ctx, budget, max_batch=8, max_inflight=2, **opts)
+            svc = make(ctx, budget, max_batch=8, max_inflight=2, **opts)
             out = serve.run_poisson(svc, pool, rate=rate, duration=args.serve_seconds,
                                     seed=int(rate), drain_limit=120.0)
             row = {k: out[k] for k in ("offered_rate", "count", "snapshots_per_s", "mvis_per_s",
                                        "latency_ms", "queue_ms", "mean_batch")}
-            row["peak_admitted_bytes"] = svc.peak_held
-            row["workspace_bytes"] = svc.workspace
+            if native:
+                row["request_loop"] = "native (svdi_service)"
+                svc.close()
+                row["workspace_bytes"] = ctx.workspace_bytes()
+            else:
+                row["request_loop"] = "python (serve.SnapshotService)"
+                row["peak_admitted_bytes"] = svc.peak_held
+                row["workspace_bytes"] = svc.workspace
             rows.append(row)
             lat = out["latency_ms"]
             log(f"serve[{name}] {rate:g} req/s: {out['count']} requests, p50 {lat['p50']:.2f} ms, "

@@ -282,6 +282,58 @@ int svdi_get_tuning(svdi_context* ctx, size_t grid_size, svdi_tuning* tuning);
 int svdi_autotune(svdi_context* ctx, const svdi_snapshot* snapshot, size_t image_size, float cell_size,
                   const svdi_options* opts, int repeats, svdi_tuning* best);
 
+/* Streaming request service (no counterpart in the reference, which images one snapshot per
+ * process; the streaming form): snapshot requests submitted over time are admitted against a
+ * device-memory budget, grouped by (image size, cell) into asynchronous batched calls and timed
+ * from submission to image-ready, by a request loop on a thread of the library's own.
+ *
+ * A request costs its snapshot's device bytes (as the caller states them) plus its image; it is
+ * admitted in submission order while the context's workspace plus the bytes already admitted
+ * plus its cost fit budget_bytes.  Size the workspace before starting (batched calls of every
+ * shape to be served, max_batch snapshots each), since it only grows: a request that cannot fit
+ * beside the workspace is refused at submission, and one that stops fitting because the
+ * workspace grew fails with SVDI_ERR_OUT_OF_MEMORY.  Admitted requests wait per key; while
+ * fewer than max_inflight calls are in flight, the key of the oldest waiting request is imaged
+ * with up to max_batch of its requests in one svdi_image_snapshots call on the context stream,
+ * and an event behind it tells the loop when the images are ready.  The service uses the
+ * context from its own thread: the caller must not image on the context meanwhile, and must
+ * keep each request's snapshot and image allocated until its result is collected. */
+typedef struct svdi_service svdi_service;
+typedef struct svdi_service_config {
+    size_t budget_bytes;        /* device bytes for the workspace and the admitted requests */
+    uint32_t max_batch;         /* requests per batched call, >= 1 */
+    uint32_t max_inflight;      /* batched calls in flight, >= 1 */
+    svdi_options options;       /* imaging options of every call */
+} svdi_service_config;
+typedef struct svdi_service_result {
+    uint64_t id;                /* the id given at submission */
+    int32_t status;             /* svdi_status of the request's batched call */
+    uint32_t batch;             /* requests in that call */
+    uint64_t call;              /* that call's sequence number (0, 1, ...) */
+    double queue_ms;            /* submission to launch */
+    double latency_ms;          /* submission to the loop seeing the image ready */
+} svdi_service_result;
+typedef struct svdi_service_stats_info {
+    size_t outstanding;         /* submitted, result not yet produced */
+    size_t waiting;             /* submitted, not admitted */
+    size_t admitted;            /* admitted, not launched */
+    size_t inflight_calls;      /* batched calls in flight */
+    size_t held_bytes;          /* bytes of admitted requests */
+    size_t workspace_bytes;     /* the context's workspace as last seen */
+} svdi_service_stats_info;
+
+int svdi_service_create(svdi_context* ctx, const svdi_service_config* config, svdi_service** service);
+/* Waits for every submitted request, then stops the loop and frees the service. */
+int svdi_service_destroy(svdi_service* service);
+/* image: device memory for image_size^2 floats, written by the call that images the request. */
+int svdi_service_submit(svdi_service* service, uint64_t id, const svdi_snapshot* snapshot,
+                        size_t snapshot_bytes, float* image, size_t image_size, float cell_size);
+/* Up to `capacity` finished requests' results, oldest first; wait_ms > 0 waits up to that long
+ * for one when none is ready and requests are outstanding. */
+int svdi_service_collect(svdi_service* service, svdi_service_result* results, size_t capacity, size_t* count,
+                         int wait_ms);
+int svdi_service_stats(svdi_service* service, svdi_service_stats_info* info);
+
 /* FIpipe (archive/FIPkernels.cu:679-911) on host buffers: Visreal/Visimag (S each), Bin
  * (S x 3), Vin (3 x 3), dirty_image (image_size^2, caller-allocated host memory). */
 int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im,

@@ -31,7 +31,8 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot_batch", "svdi_image_snapshot_batch_async", "svdi_workspace_bytes",
     "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
     "svdi_image_snapshot_planned", "svdi_set_tuning", "svdi_get_tuning", "svdi_autotune",
-    "svdi_image_snapshots_planned",
+    "svdi_image_snapshots_planned", "svdi_service_create", "svdi_service_destroy", "svdi_service_submit",
+    "svdi_service_collect", "svdi_service_stats",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -109,6 +110,22 @@ class Tuning(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServiceConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32), ("options", Options)]
+
+
+class ServiceResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("batch", ctypes.c_uint32),
+                ("call", ctypes.c_uint64), ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServiceStats(ctypes.Structure):
+    _fields_ = [("outstanding", ctypes.c_size_t), ("waiting", ctypes.c_size_t), ("admitted", ctypes.c_size_t),
+                ("inflight_calls", ctypes.c_size_t), ("held_bytes", ctypes.c_size_t),
+                ("workspace_bytes", ctypes.c_size_t)]
+
+
 _lib = None
 
 
@@ -180,6 +197,12 @@ def lib():
                                                             ctypes.POINTER(ctypes.c_size_t),
                                                             ctypes.POINTER(ctypes.c_float),
                                                             ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
+            "svdi_service_create": (ctypes.c_int, [vp, ctypes.POINTER(ServiceConfig), ctypes.POINTER(vp)]),
+            "svdi_service_destroy": (ctypes.c_int, [vp]),
+            "svdi_service_submit": (ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(Snapshot), sz, vp, sz, f]),
+            "svdi_service_collect": (ctypes.c_int, [vp, ctypes.POINTER(ServiceResult), sz,
+                                                    ctypes.POINTER(ctypes.c_size_t), ctypes.c_int]),
+            "svdi_service_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServiceStats)]),
             "svdi_set_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
             "svdi_get_tuning": (ctypes.c_int, [vp, sz, ctypes.POINTER(Tuning)]),
             "svdi_autotune": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), sz, f, ctypes.POINTER(Options),

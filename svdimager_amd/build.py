@@ -21,7 +21,8 @@ BIN = os.path.join(PKG, "bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SVDI_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["preprocess.hip", "plan.hip", "grid.hip", "tiles.hip", "windows.hip", "fft.hip", "image.hip", "api.cpp"]
+HIP_SOURCES = ["preprocess.hip", "plan.hip", "grid.hip", "tiles.hip", "windows.hip", "fft.hip", "image.hip", "api.cpp",
+               "service.cpp"]
 HOST_SOURCES = ["fits.cpp"]
 
 HIP_FLAGS = [

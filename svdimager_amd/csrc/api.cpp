@@ -773,6 +773,9 @@ std::vector<std::vector<size_t>> deal_to_lanes(const svdi_snapshot* snaps, size_
 // largest first (cost ~ samples + 4 grid cells, the least-loaded lane next), and each lane
 // always has its next snapshot's plan queued behind its current work; the host enqueues the
 // gridding of whichever lane's plan is back first, waiting only when none is.
+// svdi_last_error's message for a failure reported outside this file (the request service).
+void report_error(const char* msg) { set_error(msg); }
+
 int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
                 const size_t* sizes, const float* cells, const svdi_options* opts, svdi_plan_info* infos,
                 bool wait) {

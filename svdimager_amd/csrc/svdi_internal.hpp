@@ -252,6 +252,12 @@ struct WindowJob {
 WindowJob bin_windows(svdi_context* ctx, const SampleInput& in, const GridParams& gp, float2* planes);
 void contract_windows(svdi_context* ctx, const WindowJob* jobs, int count);
 void reduce_windows(svdi_context* ctx, const WindowJob& job);
+// api.cpp: the batched call (status), and svdi_last_error's message for failures reported
+// elsewhere.
+int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, float* const* images,
+                const size_t* sizes, const float* cells, const svdi_options* opts, svdi_plan_info* infos,
+                bool wait);
+void report_error(const char* msg);
 // Whether the window gridder takes this grid and sample count (else launch_grid_windows throws
 // UnsupportedError).
 bool grid_windows_supported(const SampleInput& in, const GridParams& gp);

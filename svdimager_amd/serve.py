@@ -208,6 +208,134 @@ class SnapshotService:
             self.workspace = int(self._ws_fn())
 
 
+class NativeService:
+    """The library's own request loop (svdi_service_*): admission, batching by key and completion
+    tracking run on a thread inside libsvdimager; this object submits requests, allocates their
+    images and collects the results.  Same interface as SnapshotService for run_poisson
+    (``submit``, ``poll``, ``busy``, ``completed``, ``batch_sizes``, ``clock``, ``reserve``).
+
+    The native loop starts at the first ``submit``, after ``reserve`` has sized the workspace;
+    from then on the context belongs to it until ``close``.  A request's ``launched`` and
+    ``done`` are its arrival plus the library's queue and latency times, each counted from the
+    submission, plus the delay between the request's arrival and its submission."""
+
+    def __init__(self, ctx, budget_bytes: int, max_batch: int = 8, max_inflight: int = 2,
+                 keep_images: bool = False, clock: Callable[[], float] = time.perf_counter, **options):
+        if max_batch < 1 or max_inflight < 1:
+            raise ValueError("max_batch and max_inflight must be at least 1")
+        self.ctx = ctx
+        self.budget = int(budget_bytes)
+        self.max_batch = int(max_batch)
+        self.max_inflight = int(max_inflight)
+        self.keep_images = keep_images
+        self.clock = clock
+        self.options = dict(options)
+        self.options.setdefault("gridder", GRIDDER_MFMA)
+        self.options.setdefault("output", OUTPUT_DIRTY)
+        self.completed: List[Request] = []
+        self.batch_sizes: List[int] = []
+        self._calls = set()
+        self._pending: Dict[int, Tuple[Request, float]] = {}
+        self._svc = None
+
+    def reserve(self, requests: Sequence[Request]) -> int:
+        """As SnapshotService.reserve: every (snapshot, key) imaged in a batch of max_batch copies
+        of itself, so that every lane holds every shape's workspace; returns its bytes."""
+        if self._svc is not None:
+            raise RuntimeError("reserve before the first submit")
+        import torch
+        seen = set()
+        for r in requests:
+            ident = (id(r.snapshot), r.key)
+            if ident in seen:
+                continue
+            seen.add(ident)
+            N, cell = r.key
+            imager.image_snapshot_batch(self.ctx, [r.snapshot] * self.max_batch, N, cell, wait=True, **self.options)
+        torch.cuda.synchronize()
+        return int(self.ctx.workspace_bytes())
+
+    def _start(self) -> None:
+        import ctypes
+        from ._lib import ServiceConfig, check, default_options, lib
+        cfg = ServiceConfig(budget_bytes=self.budget, max_batch=self.max_batch, max_inflight=self.max_inflight,
+                            options=default_options(**self.options))
+        h = ctypes.c_void_p()
+        check(lib().svdi_service_create(self.ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self._svc = h
+
+    def submit(self, req: Request) -> None:
+        import ctypes
+        import torch
+        from ._lib import check, lib
+        if self._svc is None:
+            self._start()
+        N = int(req.image_size)
+        dev = req.snapshot.tensors["uvw"].device
+        req.image = torch.empty((N, N), dtype=torch.float32, device=dev)
+        s = req.snapshot.struct()
+        t = self.clock()
+        self._pending[int(req.rid)] = (req, t)
+        try:
+            check(lib().svdi_service_submit(self._svc, int(req.rid), ctypes.byref(s), int(req.snapshot.nbytes),
+                                            imager._ptr(req.image), N, float(req.cell)))
+        except Exception:
+            del self._pending[int(req.rid)]
+            raise
+
+    def busy(self) -> bool:
+        return bool(self._pending)
+
+    def poll(self, wait_ms: int = 0) -> int:
+        """Collect the finished requests (waiting up to wait_ms for one); returns their number."""
+        import ctypes
+        from ._lib import SVDI_OK, ServiceResult, SvdiError, lib, check
+        if self._svc is None:
+            return 0
+        buf = (ServiceResult * 64)()
+        n = ctypes.c_size_t()
+        check(lib().svdi_service_collect(self._svc, buf, 64, ctypes.byref(n), int(wait_ms)))
+        for i in range(n.value):
+            res = buf[i]
+            req, t_sub = self._pending.pop(int(res.id))
+            if res.status != SVDI_OK:
+                raise SvdiError(res.status, f"request {res.id} failed in the service")
+            lag = t_sub - req.arrival
+            req.admitted = req.arrival + lag
+            req.launched = req.arrival + lag + res.queue_ms * 1e-3
+            req.done = req.arrival + lag + res.latency_ms * 1e-3
+            if not self.keep_images:
+                req.image = None
+            self.completed.append(req)
+            if res.call not in self._calls:
+                self._calls.add(res.call)
+                self.batch_sizes.append(int(res.batch))
+        return n.value
+
+    def stats(self) -> Dict[str, int]:
+        import ctypes
+        from ._lib import ServiceStats, check, lib
+        st = ServiceStats()
+        if self._svc is not None:
+            check(lib().svdi_service_stats(self._svc, ctypes.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in ServiceStats._fields_}
+
+    def close(self) -> None:
+        """Wait for every submitted request and stop the native loop."""
+        from ._lib import check, lib
+        if self._svc is not None:
+            while self._pending:
+                self.poll(wait_ms=100)
+            check(lib().svdi_service_destroy(self._svc))
+            self._svc = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def poisson_arrivals(rate: float, duration: float, rng: np.random.Generator) -> np.ndarray:
     """Arrival times in [0, duration) of a Poisson process of `rate` per second."""
     if rate <= 0 or duration <= 0:
@@ -219,7 +347,7 @@ def poisson_arrivals(rate: float, duration: float, rng: np.random.Generator) -> 
     return t[t < duration]
 
 
-def run_poisson(service: SnapshotService, pool: Sequence[Tuple[object, int, float]], rate: float,
+def run_poisson(service, pool: Sequence[Tuple[object, int, float]], rate: float,
                 duration: float, seed: int = 0, drain_limit: float = 60.0) -> Dict:
     """Requests drawn uniformly from `pool` ((snapshot, image size, cell) entries) arrive at
     `rate` per second for `duration` seconds of wall-clock time; the loop runs until every

@@ -71,6 +71,12 @@ def test_null_context_is_an_error(L):
     assert L.svdi_set_tuning(None, 0, ctypes.byref(t)) == 1 and L.svdi_last_error()
     assert L.svdi_get_tuning(None, 0, ctypes.byref(t)) == 1
     assert L.svdi_autotune(None, None, 1024, 1e-5, None, 1, ctypes.byref(t)) == 1
+    from svdimager_amd._lib import ServiceConfig, ServiceStats
+    h = ctypes.c_void_p()
+    cfg = ServiceConfig(budget_bytes=1 << 30, max_batch=4, max_inflight=2)
+    assert L.svdi_service_create(None, ctypes.byref(cfg), ctypes.byref(h)) == 1 and L.svdi_last_error()
+    assert L.svdi_service_destroy(None) == 1
+    assert L.svdi_service_stats(None, ctypes.byref(ServiceStats())) == 1
 
 
 def test_fits_round_trip(tmp_path):
