@@ -33,12 +33,17 @@ struct RecordWords {
     uint32_t w[4];
 };
 
-// Position p = base + q (base an integer shift, q in cells) -> (cell, fraction) bits in 31..8.
+// Position p = base + q (base an integer shift, q in cells) -> (cell, fraction) bits in 31..8:
+// the first footprint cell k = ceil(q - 4) = ceil(q) - 4 and F = ceil(q) - q to 2^-18, formed
+// exactly in FP32 and integers: rint(F 2^18) = ceil(q) 2^18 - rint(q 2^18), since ceil(q) 2^18 is
+// an even integer (round-half-even commutes with shifts by even integers) and q 2^18 is exact.
+// q is first clamped to [-8000, 8000] (NaN to -8000), which keeps ceil(q) 2^18 within int32; any
+// |q| beyond that is far outside the cell range and clamps there anyway.
 SVDI_HD uint32_t rec_encode_pos(int base, float q) {
-    const double qd = static_cast<double>(q);
-    const double k = ceil(qd - 4.0);
-    int cell = base + static_cast<int>(k);
-    int frac = static_cast<int>(rint((k + 4.0 - qd) * static_cast<double>(kRecFracScale)));
+    q = fminf(fmaxf(q, -8000.0f), 8000.0f);
+    const int c = static_cast<int>(ceilf(q));
+    int cell = base + c - 4;
+    int frac = c * (1 << kRecFracBits) - static_cast<int>(rintf(q * kRecFracScale));
     if (frac >= (1 << kRecFracBits)) {              // F rounded to 1: the same point as (cell - 1, 0)
         frac -= 1 << kRecFracBits;
         cell -= 1;
@@ -49,18 +54,19 @@ SVDI_HD uint32_t rec_encode_pos(int base, float q) {
 
 SVDI_HD RecordWords rec_encode(int base1, float q1, int base2, float q2, float s, float vr, float vi) {
     // Shared exponent: |vr|, |vi| < 2^E; mantissas rint(v 2^(23 - E)) in [-2^23, 2^23 - 1].
+    // |rint(v 2^(23 - E))| <= 2^24 (2^23 unless E was clamped), so the clamps and the
+    // conversions are done on integer-valued floats, all exact.
     const float m = fmaxf(fabsf(vr), fabsf(vi));
     int e = 0;
     if (m > 0.0f) (void)frexpf(m, &e);
     e = e < -127 ? -127 : (e > 127 ? 127 : e);
     auto mant = [&](float v) {
-        long r = static_cast<long>(rintf(ldexpf(v, 23 - e)));
-        r = r > 8388607 ? 8388607 : (r < -8388608 ? -8388608 : r);
+        const float r = fminf(fmaxf(rintf(ldexpf(v, 23 - e)), -8388608.0f), 8388607.0f);
         return static_cast<uint32_t>(static_cast<int32_t>(r)) << 8;
     };
     const float sc = s < -1.0f ? -1.0f : (s > 1.0f ? 1.0f : s);
-    long sq = static_cast<long>(rintf(sc * 8388608.0f)) + 8388608;
-    const uint32_t su = static_cast<uint32_t>(sq > 16777215 ? 16777215 : sq);
+    const float sq = fminf(fmaxf(rintf(sc * 8388608.0f) + 8388608.0f, 0.0f), 16777215.0f);
+    const uint32_t su = static_cast<uint32_t>(sq);
     RecordWords r;
     r.w[0] = mant(vr) | (su & 0xffu);
     r.w[1] = mant(vi) | ((su >> 8) & 0xffu);

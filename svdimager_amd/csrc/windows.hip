@@ -149,10 +149,12 @@ template <bool kSnap, bool kVis, class Fn>
 __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const GridParams& gp,
                                                 const WinGeom& wg, size_t blk, Fn&& fn) {
     auto bins_of = [&](const Footprint& f) {
+        // (a0, b0 >= 0: array coordinates of an in-grid footprint.)
         const int h = f.a0 >> wg.cls_log2;
         const int col0 = f.local_slab * wg.nw;
-        for (int wb = f.b0 / kWin; wb <= (f.b0 + f.nb - 1) / kWin; ++wb)
-            fn(f, (col0 + wb) * wg.nh + h, h, wb);
+        const int wb0 = static_cast<int>(static_cast<uint32_t>(f.b0) / kWin);
+        const int wb1 = static_cast<int>(static_cast<uint32_t>(f.b0 + f.nb - 1) / kWin);
+        for (int wb = wb0; wb <= wb1; ++wb) fn(f, (col0 + wb) * wg.nh + h, h, wb);
     };
     if constexpr (kSnap) {
         constexpr int kBatch = 4;

@@ -125,3 +125,45 @@ def test_visibility_pairs(lib, scale):
     assert (np.abs(gr - vr)[ok] <= 2.0 ** -23 * m[ok]).all()
     assert (np.abs(gi - vi)[ok] <= 2.0 ** -23 * m[ok]).all()
     assert (np.sign(gr) == np.sign(vr))[np.abs(vr) > 2.0 ** -23 * m].all()
+
+
+def test_bits_match_the_fp64_rules(lib):
+    """Cell and fraction bits equal the FP64 rule k = ceil(q - 4), F = rint((k + 4 - q) 2^18) (a
+    carry at F = 2^18), and the mantissas rint(v 2^(23 - E)), bit for bit, including exact ties
+    of the fraction rounding, tiny q and |q| up to 4100 cells."""
+    rng = np.random.default_rng(11)
+    n = 400000
+    q1 = np.concatenate([rng.uniform(-4100, 4100, n // 4), rng.uniform(2.9, 4.1, n // 4),
+                         (rng.integers(-2 ** 20, 2 ** 20, n // 4) + 0.5) / 2.0 ** 18,
+                         10.0 ** rng.uniform(-30, 0, n // 4) * rng.choice([-1, 1], n // 4)]).astype(np.float32)
+    q2 = rng.permutation(q1)
+    b1 = rng.integers(-8, 8, n)
+    b2 = rng.integers(-8, 8, n)
+    vr = (rng.standard_normal(n) * 10.0 ** rng.uniform(-20, 20, n)).astype(np.float32)
+    vi = (rng.standard_normal(n) * 10.0 ** rng.uniform(-20, 20, n)).astype(np.float32)
+    a = [np.ascontiguousarray(x, dtype=t) for x, t in ((b1, np.int32), (q1, np.float32), (b2, np.int32),
+                                                       (q2, np.float32), (np.zeros(n), np.float32),
+                                                       (vr, np.float32), (vi, np.float32))]
+    w = np.zeros(4 * n, np.uint32)
+    lib.enc(n, *a, w)
+    w = w.reshape(n, 4)
+
+    def pos_bits(base, q):
+        qd = q.astype(np.float64)
+        k = np.ceil(qd - 4.0)
+        cell = base + k.astype(np.int64)
+        frac = np.rint((k + 4.0 - qd) * 2.0 ** 18).astype(np.int64)
+        carry = frac >= 2 ** 18
+        frac[carry] -= 2 ** 18
+        cell[carry] -= 1
+        cell = np.clip(cell, -32, 31)
+        return ((cell.astype(np.int64) & 0x3f) << 26) | (frac << 8)
+
+    assert (w[:, 2] & 0xffffff00 == pos_bits(b1, q1)).all()
+    assert (w[:, 3] & 0xffffff00 == pos_bits(b2, q2)).all()
+    m = np.maximum(np.abs(vr), np.abs(vi)).astype(np.float64)
+    e = np.frexp(m)[1]
+    for v, word in ((vr, w[:, 0]), (vi, w[:, 1])):
+        mant = np.clip(np.rint(np.ldexp(v.astype(np.float64), 23 - e)), -2 ** 23, 2 ** 23 - 1).astype(np.int64)
+        assert ((word >> 8).astype(np.int64) == (mant & 0xffffff)).all()
+    assert ((w[:, 3] & 0xff).astype(np.int64) == (e & 0xff)).all()
