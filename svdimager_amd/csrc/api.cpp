@@ -533,6 +533,10 @@ SampleInput snapshot_input(svdi_context* ctx, const svdi_snapshot& s) {
     in.f3 = s.flag3;
     in.w0 = s.weight0;
     in.w3 = s.weight3;
+    auto aligned = [](const void* p, uintptr_t a) { return p == nullptr || reinterpret_cast<uintptr_t>(p) % a == 0; };
+    in.quad_rows = s.num_rows % 4 == 0 && aligned(s.vis0_re, 16) && aligned(s.vis0_im, 16) &&
+                   aligned(s.vis3_re, 16) && aligned(s.vis3_im, 16) && aligned(s.weight0, 16) &&
+                   aligned(s.weight3, 16) && aligned(s.flag0, 4) && aligned(s.flag3, 4);
     in.geo = geo;
     in.r3_partials = r3p;
     in.r3_blocks = nb;

@@ -79,6 +79,9 @@ struct SampleInput {
     const float *v0r = nullptr, *v0i = nullptr, *v3r = nullptr, *v3i = nullptr;
     const uint8_t *f0 = nullptr, *f3 = nullptr;
     const float *w0 = nullptr, *w3 = nullptr;
+    // Rows a multiple of 4 and the per-sample arrays aligned for 4-sample loads (16 bytes for the
+    // FP32 arrays, 4 for the flags): the binning passes then read four rows per load.
+    bool quad_rows = false;
     const DevGeometry* geo = nullptr;
     // r3 range partials (min, max pairs) already reduced per block, or null to reduce Bin
     const float* r3_partials = nullptr;

@@ -157,6 +157,69 @@ __device__ __forceinline__ void for_sample_bins(const SampleInput& in, const Gri
         for (int wb = wb0; wb <= wb1; ++wb) fn(f, (col0 + wb) * wg.nh + h, h, wb);
     };
     if constexpr (kSnap) {
+        if (in.quad_rows) {
+            // Four consecutive rows of one channel per thread (one 16-byte load per FP32 array,
+            // 4 bytes per flag array; a wavefront reads 256 consecutive rows), 128 row quads x 8
+            // channel groups; the next channel's loads are issued before this one is binned.
+            constexpr int kQuads = kBinRows / 4;
+            constexpr int kQuadGroups = kBinThreads / kQuads;
+            const size_t row0 = blk * kBinRows + 4 * (threadIdx.x % kQuads);
+            if (row0 >= in.num_rows) return;          // (num_rows % 4 == 0: quads are whole)
+            struct QuadLoad {
+                double nu;
+                float4 v0r, v0i, v3r, v3i, w0, w3;
+                uint32_t f0, f3;
+            };
+            const float4 one4 = make_float4(1.0f, 1.0f, 1.0f, 1.0f), zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            auto load = [&](size_t chan) {
+                QuadLoad q;
+                const size_t idx = row0 + chan * in.num_rows;
+                q.nu = in.freq[chan];
+                q.v0r = q.v0i = q.v3r = q.v3i = zero4;
+                q.w0 = q.w3 = one4;
+                if constexpr (kVis) {
+                    q.v0r = *reinterpret_cast<const float4*>(in.v0r + idx);
+                    q.v0i = *reinterpret_cast<const float4*>(in.v0i + idx);
+                    q.v3r = *reinterpret_cast<const float4*>(in.v3r + idx);
+                    q.v3i = *reinterpret_cast<const float4*>(in.v3i + idx);
+                    if (in.w0) q.w0 = *reinterpret_cast<const float4*>(in.w0 + idx);
+                    if (in.w3) q.w3 = *reinterpret_cast<const float4*>(in.w3 + idx);
+                }
+                q.f0 = *reinterpret_cast<const uint32_t*>(in.f0 + idx);
+                q.f3 = *reinterpret_cast<const uint32_t*>(in.f3 + idx);
+                return q;
+            };
+            auto lane4 = [](const float4& v, int r) { return r == 0 ? v.x : r == 1 ? v.y : r == 2 ? v.z : v.w; };
+            size_t chan = threadIdx.x / kQuads;
+            if (chan >= in.num_channels) return;
+            QuadLoad cur = load(chan);
+            double geo[4][3];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) geo[r][k] = in.rot[3 * (row0 + r) + k];
+            for (; chan < in.num_channels; chan += kQuadGroups) {
+                QuadLoad nxt = cur;
+                if (chan + kQuadGroups < in.num_channels) nxt = load(chan + kQuadGroups);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    SnapLoad s;
+                    s.nu = cur.nu;
+                    s.v0r = lane4(cur.v0r, r);
+                    s.v0i = lane4(cur.v0i, r);
+                    s.v3r = lane4(cur.v3r, r);
+                    s.v3i = lane4(cur.v3i, r);
+                    s.w0 = lane4(cur.w0, r);
+                    s.w3 = lane4(cur.w3, r);
+                    s.f0 = static_cast<uint8_t>(cur.f0 >> (8 * r));
+                    s.f3 = static_cast<uint8_t>(cur.f3 >> (8 * r));
+                    Footprint f;
+                    if (finish_snapshot_sample<true>(gp, geo[r][0], geo[r][1], geo[r][2], s, f)) bins_of(f);
+                }
+                cur = nxt;
+            }
+            return;
+        }
         constexpr int kBatch = 4;
         for (int set = 0; set < kBinRowSets; ++set) {
         const size_t row = blk * kBinRows + set * kBinRowsPerSet + (threadIdx.x % kBinRowsPerSet);
