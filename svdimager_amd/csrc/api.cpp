@@ -208,9 +208,7 @@ void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
     const size_t tok = stage_begin(ctx, 1);
     launch_plan(ctx, in, cs.hp.semantics, N, cell, target_alpha, dplan);
     stage_end(ctx, tok);
-    const float* vin_dev = in.from_snapshot ? in.geo->vin : in.vin_dev;
     SVDI_HIP(hipMemcpyAsync(&ctx->pinned->plan, dplan, sizeof(DevPlan), hipMemcpyDeviceToHost, ctx->stream));
-    SVDI_HIP(hipMemcpyAsync(ctx->pinned->vin, vin_dev, sizeof ctx->pinned->vin, hipMemcpyDeviceToHost, ctx->stream));
     SVDI_HIP(hipEventRecord(ctx->ev_plan, ctx->stream));
 }
 
@@ -219,7 +217,7 @@ void image_begin(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
 void image_take_plan(svdi_context* ctx, const svdi_options& o, CallState& cs) {
     HostPlan& hp = cs.hp;
     hp.dev = ctx->pinned->plan;
-    const float* v = ctx->pinned->vin;
+    const float* v = hp.dev.vin;
     cs.det = std::fabs(v[0] * v[4] - v[1] * v[3]);
 
     // K: bucket rule (SVDIntegratedImager.cu:1577-1603) or smallest passing K (FIPkernels.cu:301-318).
@@ -823,7 +821,9 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
             planned[j] = static_cast<long>(i);
             plan_seq[j] = order_of_plan++;
         };
-        for (size_t j = 0; j < L; ++j) begin_next(j);
+        // The lanes' first plans are queued one lane at a time, and between them the host looks
+        // for a plan already back, so the first gridding is enqueued as soon as its plan returns
+        // rather than after every lane's planning has been enqueued.
         for (;;) {
             long pick = -1;
             for (size_t j = 0; j < L && pick < 0; ++j) {
@@ -831,6 +831,15 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
                 const hipError_t q = hipEventQuery(ctx->lanes[j]->ev_plan);
                 if (q == hipSuccess) pick = static_cast<long>(j);
                 else if (q != hipErrorNotReady) SVDI_HIP(q);
+            }
+            if (pick < 0) {
+                long idle = -1;                                 // a lane with no plan queued yet
+                for (size_t j = 0; j < L && idle < 0; ++j)
+                    if (planned[j] < 0 && next[j] < queue[j].size()) idle = static_cast<long>(j);
+                if (idle >= 0) {
+                    begin_next(static_cast<size_t>(idle));
+                    continue;
+                }
             }
             if (pick < 0) {                                     // none back yet: the oldest plan
                 for (size_t j = 0; j < L; ++j)

@@ -80,13 +80,13 @@ k_t3_stats(const float* __restrict__ vin, size_t N, float cell, float* __restric
     }
 }
 
-// Finalise: r3 range, t3 stats, slab configuration.
+// Finalise: r3 range, t3 stats, slab configuration (and the rotation rows, read back with them).
 // Integrated: finish_host_chebyshev_selection + make_chebyshev_slab_config (SVDIntegratedImager.cu:941-991).
 // Archive: one slab centred on the r3 range (archive/FIPkernels.cu:336-358).
 __global__ void __launch_bounds__(kThreads)
 k_plan_finalize(const float* __restrict__ r3p, int r3_blocks, const float* __restrict__ t3max,
                 const uint32_t* __restrict__ t3valid, int t3_blocks, int archive,
-                float target_alpha, DevPlan* __restrict__ plan) {
+                float target_alpha, const float* __restrict__ vin, DevPlan* __restrict__ plan) {
     __shared__ float lo[kThreads], hi[kThreads], mx[kThreads];
     __shared__ uint32_t cnt[kThreads];
     float a = INFINITY, b = -INFINITY, m = 0.0f;
@@ -142,6 +142,7 @@ k_plan_finalize(const float* __restrict__ r3p, int r3_blocks, const float* __res
         p.inv_slab_half_range = 1.0f / p.slab_half_range;
         p.slab_alpha_max = static_cast<float>(2.0f * kPi * max_t3 * p.slab_half_range);
     }
+    for (int i = 0; i < 9; ++i) p.vin[i] = vin[i];
     *plan = p;
 }
 
@@ -264,7 +265,7 @@ void launch_plan(svdi_context* ctx, const SampleInput& in, Semantics sem, size_t
     k_t3_stats<<<t3_blocks, kThreads, 0, ctx->stream>>>(vin, image_size, cell, t3max, t3valid);
     SVDI_LAUNCH_CHECK();
     k_plan_finalize<<<1, kThreads, 0, ctx->stream>>>(r3p, r3_blocks, t3max, t3valid, t3_blocks,
-                                                    archive, target_alpha, plan);
+                                                    archive, target_alpha, vin, plan);
     SVDI_LAUNCH_CHECK();
     k_term_errors<<<archive ? 513 : 257, kTermThreads, 0, ctx->stream>>>(plan, archive);
     SVDI_LAUNCH_CHECK();

@@ -58,6 +58,7 @@ struct DevPlan {
     uint32_t bucket_err_bits[4];     // integrated-imager buckets K = 8, 16, 24, 32
     uint32_t term_err_bits[32];      // archive search, K = 1 .. 32
     uint32_t vis_absmax_bits;        // max |vis| / |det| over gridded samples (for MFMA scaling)
+    float vin[9];                    // the rotation rows the plan was made with (read back with it)
 };
 
 enum class Semantics { Integrated = SVDI_SEMANTICS_INTEGRATED, Archive = SVDI_SEMANTICS_ARCHIVE };
@@ -152,11 +153,11 @@ class Workspace {
 }  // namespace svdi
 
 namespace svdi {
-// What the host reads back, in one page-locked block: the plan and rotation rows between
-// planning and gridding, the gridder's totals and the consistency word at the end of a call.
+// What the host reads back, in one page-locked block: the plan (with its rotation rows, whose
+// first two give the vis scale |det|) between planning and gridding, the gridder's totals and the
+// consistency word at the end of a call.
 struct PinnedPlan {
     DevPlan plan;
-    float vin[9];           // the rotation rows in FP32 (|det| of the first two rows is the vis scale)
     uint32_t totals[8];     // the last slab batch's gridder totals ([0] = records)
     uint32_t error_word;    // copy of the context's device error word (nonzero: binning mismatch)
 };
