@@ -148,47 +148,50 @@ k_plan_finalize(const float* __restrict__ r3p, int r3_blocks, const float* __res
 
 // Sampled max |e^{i a s} - sum_{n<K} (n ? 2 : 1) i^n J_n(a) T_n(s)| over a in [0, alpha_max]
 // and s in [-1, 1], evaluated in FP64.  One block per alpha sample, two or four s samples per
-// thread: every thread forms its e^{i a s} while thread 0 forms the Bessel sequence.
+// thread: every thread forms its e^{i a s} while the Bessel sequence is formed (for a < 2 by the
+// power series, one order per lane of the first wavefront; otherwise by thread 0's recurrence).
 //   integrated: 257 x 1024 samples, K in {8, 16, 24, 32} (cheb_bucket_error_kernel,
 //               SVDIntegratedImager.cu:1125-1202; alpha and s formed in FP32 as there);
 //   archive:    513 x 2048 samples, every K in 1..32 (host_chebyshev_error_sample_range,
 //               archive/FIPkernels.cu:272-318; alpha in FP32, s in FP64 as there).
 constexpr int kTermThreads = 512;
+constexpr double kBesselLaneSeriesMax = 2.0;
 
+template <bool kArchive>
 __global__ void __launch_bounds__(kTermThreads)
-k_term_errors(DevPlan* __restrict__ plan, int archive) {
+k_term_errors(DevPlan* __restrict__ plan) {
+    constexpr int na = kArchive ? 513 : 257;
+    constexpr int ns = kArchive ? 2048 : 1024;
+    constexpr int kPer = ns / kTermThreads;     // s samples per thread
+    // Errors kept: every K (archive) or the buckets K = 8, 16, 24, 32 (terms 7, 15, 23, 31).
+    constexpr int kKept = kArchive ? kMaxTerms : kMaxTerms / 8;
     __shared__ double jn[kMaxTerms];
-    __shared__ double red[kMaxTerms][kTermThreads / 64];
-    const int na = archive ? 513 : 257;
-    const int ns = archive ? 2048 : 1024;
+    __shared__ double red[kKept][kTermThreads / 64];
     const float amax = plan->slab_alpha_max;
-    const float alpha_f = (na == 1) ? amax : amax * static_cast<float>(blockIdx.x) / static_cast<float>(na - 1);
+    const float alpha_f = amax * static_cast<float>(blockIdx.x) / static_cast<float>(na - 1);
     const double alpha = static_cast<double>(alpha_f);
-    constexpr int kPer = 2048 / kTermThreads; // s samples per thread at most
     double sv[kPer], er[kPer], ei[kPer];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
         const int si = threadIdx.x + u * kTermThreads;
-        if (archive) {
-            sv[u] = -1.0 + 2.0 * static_cast<double>(si) / static_cast<double>(ns - 1);
-        } else {
-            sv[u] = static_cast<double>(-1.0f + 2.0f * static_cast<float>(si) / static_cast<float>(ns - 1));
-        }
-        if (si < ns) sincos(alpha * sv[u], &ei[u], &er[u]);
+        if constexpr (kArchive) sv[u] = -1.0 + 2.0 * static_cast<double>(si) / static_cast<double>(ns - 1);
+        else sv[u] = static_cast<double>(-1.0f + 2.0f * static_cast<float>(si) / static_cast<float>(ns - 1));
+        sincos(alpha * sv[u], &ei[u], &er[u]);
     }
-    if (threadIdx.x == 0) {
+    if (fabs(alpha) < kBesselLaneSeriesMax) {
+        if (threadIdx.x < kMaxTerms) jn[threadIdx.x] = bessel_series(static_cast<int>(threadIdx.x), alpha);
+    } else if (threadIdx.x == 0) {
         double j[kMaxTerms];
         bessel_sequence_fixed<kMaxTerms>(alpha, j);
 #pragma unroll
         for (int n = 0; n < kMaxTerms; ++n) jn[n] = j[n];
     }
     __syncthreads();
-    double err2[kMaxTerms];                  // squared error; the root is taken once at the end
+    double err2[kKept];                      // squared error; the root is taken once at the end
 #pragma unroll
-    for (int n = 0; n < kMaxTerms; ++n) err2[n] = 0.0;
+    for (int i = 0; i < kKept; ++i) err2[i] = 0.0;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-        if (threadIdx.x + u * kTermThreads >= ns) break;
         const double s = sv[u];
         double ar = 0.0, ai = 0.0, tm2 = 1.0, tm1 = s;
 #pragma unroll
@@ -212,31 +215,28 @@ k_term_errors(DevPlan* __restrict__ plan, int archive) {
                     default: ai -= amp; break;
                 }
             }
-            const double dr = er[u] - ar, di = ei[u] - ai;
-            err2[n] = fmax(err2[n], dr * dr + di * di);
+            if (kArchive || (n & 7) == 7) {
+                const int i = kArchive ? n : n >> 3;
+                const double dr = er[u] - ar, di = ei[u] - ai;
+                err2[i] = fmax(err2[i], dr * dr + di * di);
+            }
         }
     }
-    // Maxima of the squared errors of the terms in use (all in archive mode, K = 8, 16, 24, 32
-    // otherwise): per wavefront, then across the block's wavefronts, then one root per term.
+    // Maxima of the squared errors: per wavefront, then across the block's wavefronts, then one
+    // root per kept term count.
 #pragma unroll
-    for (int n = 0; n < kMaxTerms; ++n) {
-        if (!archive && (n & 7) != 7) continue;
-        double v = err2[n];
+    for (int i = 0; i < kKept; ++i) {
+        double v = err2[i];
         for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-        if ((threadIdx.x & 63) == 0) red[n][threadIdx.x >> 6] = v;
+        if ((threadIdx.x & 63) == 0) red[i][threadIdx.x >> 6] = v;
     }
     __syncthreads();
-    if (threadIdx.x < kMaxTerms && (archive || (threadIdx.x & 7) == 7)) {
+    if (threadIdx.x < kKept) {
         double v2 = 0.0;
         for (int w = 0; w < kTermThreads / 64; ++w) v2 = fmax(v2, red[threadIdx.x][w]);
         const float v = static_cast<float>(sqrt(v2));
-        if (archive) {
-            atomic_max_nonneg(&plan->term_err_bits[threadIdx.x], v);
-        } else {
-            const int n = threadIdx.x;
-            const int bucket = (n == 7) ? 0 : (n == 15) ? 1 : (n == 23) ? 2 : (n == 31) ? 3 : -1;
-            if (bucket >= 0) atomic_max_nonneg(&plan->bucket_err_bits[bucket], v);
-        }
+        if constexpr (kArchive) atomic_max_nonneg(&plan->term_err_bits[threadIdx.x], v);
+        else atomic_max_nonneg(&plan->bucket_err_bits[threadIdx.x], v);
     }
 }
 
@@ -267,7 +267,8 @@ void launch_plan(svdi_context* ctx, const SampleInput& in, Semantics sem, size_t
     k_plan_finalize<<<1, kThreads, 0, ctx->stream>>>(r3p, r3_blocks, t3max, t3valid, t3_blocks,
                                                     archive, target_alpha, vin, plan);
     SVDI_LAUNCH_CHECK();
-    k_term_errors<<<archive ? 513 : 257, kTermThreads, 0, ctx->stream>>>(plan, archive);
+    if (archive) k_term_errors<true><<<513, kTermThreads, 0, ctx->stream>>>(plan);
+    else k_term_errors<false><<<257, kTermThreads, 0, ctx->stream>>>(plan);
     SVDI_LAUNCH_CHECK();
 }
 

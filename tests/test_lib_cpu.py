@@ -217,6 +217,7 @@ extern "C" {
 void p_bessel(double x, int n, double* j) { svdi::bessel_sequence_f64(x, n, j); }
 void p_bessel_fixed8(double x, double* j) { svdi::bessel_sequence_fixed<8>(x, j); }
 void p_bessel_fixed32(double x, double* j) { svdi::bessel_sequence_fixed<32>(x, j); }
+double p_bessel_series(int n, double x) { return svdi::bessel_series(n, x); }
 void p_eigen(const double* a, double* ev, double* vec) { svdi::eigen_sym3(a, ev, vec); }
 void p_basis(const double* vec, double acx, double* basis) { svdi::oriented_basis(vec, acx, basis); }
 float p_es(float x) { return svdi::es_kernel(x); }
@@ -242,6 +243,8 @@ def probe(tmp_path_factory):
     P.p_bessel.argtypes = [ctypes.c_double, ctypes.c_int, dp]
     P.p_bessel_fixed8.argtypes = [ctypes.c_double, dp]
     P.p_bessel_fixed32.argtypes = [ctypes.c_double, dp]
+    P.p_bessel_series.argtypes = [ctypes.c_int, ctypes.c_double]
+    P.p_bessel_series.restype = ctypes.c_double
     P.p_eigen.argtypes = [dp, dp, dp]
     P.p_basis.argtypes = [dp, ctypes.c_double, dp]
     P.p_es.argtypes = [ctypes.c_float]
@@ -269,6 +272,19 @@ def test_device_bessel_matches_scipy(probe, x):
         big = np.abs(r) > 1e-30
         if abs(x) < 1000:
             assert np.all(np.abs(j[big] - r[big]) <= 1e-10 * np.abs(r[big])), (x, n)
+
+
+def test_bessel_power_series_below_two(probe):
+    """The per-order power series that the term-error kernel uses for |a| < 2 (one order per
+    lane): J_0..J_31 against scipy to 5e-14 relative (orders above 1e-200) and 4e-16 absolute."""
+    xs = np.concatenate([[0.0, 1e-12, 1e-3, 0.712, 1.0, 1.999, -0.4, -1.999],
+                         np.random.default_rng(9).uniform(-2.0, 2.0, 40)])
+    for x in xs:
+        ref = special.jv(np.arange(32), x)
+        got = np.array([probe.p_bessel_series(n, float(x)) for n in range(32)])
+        assert np.abs(got - ref).max() <= 4e-16, x
+        big = np.abs(ref) > 1e-200
+        assert np.all(np.abs(got[big] - ref[big]) <= 5e-14 * np.abs(ref[big])), x
 
 
 def test_device_eigen_and_basis_match_oracle(probe):
