@@ -667,6 +667,10 @@ std::unique_ptr<svdi_context> create_context(int device, bool blocking = true) {
     hipDeviceProp_t prop;
     SVDI_HIP(hipGetDeviceProperties(&prop, device));
     c->cu_count = prop.multiProcessorCount;
+    // A lane's window kernel leaves an eighth of the CUs (four per XCD on MI355X) to the other
+    // lanes: 1,306-1,310 against 1,255-1,258 snapshots/s batched (SKA-Low 1024^2; a quarter less
+    // or more left free measured lower).
+    if (!blocking) c->grid_reserve = c->cu_count / 8;
     c->noalloc_scratch = noalloc_scratch_address(device);
     return c;
 }

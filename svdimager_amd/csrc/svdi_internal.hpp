@@ -187,6 +187,9 @@ struct svdi_context {
     size_t conv_n = 0, conv_g = 0;
     const float* conv_at = nullptr;
     int cu_count = 0;
+    // CUs the window kernel's persistent launch leaves free: nonzero on the lanes of a batched
+    // call, whose other lanes' kernels (binning, FFT, image) then run beside it.
+    int grid_reserve = 0;
     // stage timing: pairs of events recorded around stages of the current call
     bool profiling = false;
     std::vector<hipEvent_t> event_pool;

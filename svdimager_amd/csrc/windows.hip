@@ -1386,7 +1386,7 @@ void contract_windows(svdi_context* ctx, const WindowJob* jobs, int count) {
         d.G = jobs[j].G;
     }
     const size_t tok = stage_begin(ctx, 5);
-    const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count));
+    const unsigned grid = static_cast<unsigned>(std::max(1, ctx->cu_count - ctx->grid_reserve));
     auto launch = [&](auto prec) {
         constexpr int P = decltype(prec)::value;
         using T = WinTraits<P>;
