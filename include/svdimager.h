@@ -118,6 +118,10 @@ typedef struct svdi_plan_info {
     float r3_centre;        /* (r3_min + r3_max) / 2 */
     float inv_slab_half_range;
     float vis_det;          /* |V00 V11 - V01 V10| of the rotation rows (the visibility scale) */
+    int enqueued_early;     /* svdi_image_snapshot: 1 = enqueued whole before the plan was read back,
+                               on the K of the context's last one-slab call of this size, cell and
+                               options, and kept; 2 = that guess missed and the work was enqueued
+                               again with the plan's shape; 0 = enqueued after the read-back */
 } svdi_plan_info;
 
 int svdi_version(void);

@@ -77,7 +77,7 @@ class PlanInfo(ctypes.Structure):
         ("num_planes", ctypes.c_size_t), ("records", ctypes.c_size_t),
         ("elapsed_ms", ctypes.c_float), ("gridder", ctypes.c_int),
         ("r3_centre", ctypes.c_float), ("inv_slab_half_range", ctypes.c_float),
-        ("vis_det", ctypes.c_float),
+        ("vis_det", ctypes.c_float), ("enqueued_early", ctypes.c_int),
     ]
 
     @classmethod

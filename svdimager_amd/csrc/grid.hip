@@ -21,6 +21,7 @@ constexpr int kThreads = 256;
 template <bool kSnap>
 __global__ void __launch_bounds__(kThreads)
 k_grid_atomic(SampleInput in, GridParams gp, float2* __restrict__ planes) {
+    take_device_plan(gp);
     size_t idx, row = 0, chan = 0;
     if constexpr (kSnap) {
         row = blockIdx.x * (size_t)kThreads + threadIdx.x;

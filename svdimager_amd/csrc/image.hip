@@ -138,6 +138,7 @@ template <int KMAX>
 __global__ void __launch_bounds__(256)
 k_image(const float2* __restrict__ planes, ImageParams ip, const float* __restrict__ vin,
         const float* __restrict__ conv, float* __restrict__ image) {
+    take_device_plan(ip);
     const int x = blockIdx.x * 16 + threadIdx.x;
     const int y = blockIdx.y * 16 + threadIdx.y;
     float V[9];

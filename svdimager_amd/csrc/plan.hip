@@ -143,6 +143,10 @@ k_plan_finalize(const float* __restrict__ r3p, int r3_blocks, const float* __res
         p.slab_alpha_max = static_cast<float>(2.0f * kPi * max_t3 * p.slab_half_range);
     }
     for (int i = 0; i < 9; ++i) p.vin[i] = vin[i];
+    p.inv_slab_width = 1.0 / static_cast<double>(p.slab_width);
+    // |v0 v4 - v1 v3| with each operation rounded on its own, as the host forms it.
+    p.det = fabsf(__fsub_rn(__fmul_rn(vin[0], vin[4]), __fmul_rn(vin[1], vin[3])));
+    p.inv_det = 1.0f / p.det;
     *plan = p;
 }
 

@@ -11,6 +11,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "svdi_math.hpp"
@@ -59,6 +60,10 @@ struct DevPlan {
     uint32_t term_err_bits[32];      // archive search, K = 1 .. 32
     uint32_t vis_absmax_bits;        // max |vis| / |det| over gridded samples (for MFMA scaling)
     float vin[9];                    // the rotation rows the plan was made with (read back with it)
+    // Values the host derives from the plan, formed the same way on the device: 1 / slab width,
+    // |det| of the first two rotation rows (the vis scale) and its reciprocal.
+    double inv_slab_width;
+    float det, inv_det;
 };
 
 enum class Semantics { Integrated = SVDI_SEMANTICS_INTEGRATED, Archive = SVDI_SEMANTICS_ARCHIVE };
@@ -118,6 +123,9 @@ struct GridParams {
     // MFMA gridder work shape (svdi_tuning; 0 = the default).
     uint32_t coop_min, coop_cap;
     int class_log2;
+    // Non-null for a call enqueued before its plan was read back: the kernels then take the plan's
+    // values (r3 range, slab width, centre, vis scale) from this device record instead.
+    const DevPlan* dplan;
 };
 
 // Grown-on-demand device buffers owned by a context.  In arena mode (between begin_arena and
@@ -153,6 +161,19 @@ class Workspace {
 }  // namespace svdi
 
 namespace svdi {
+// The sizes and options that decide a call's plan shape (its K and slab count once the geometry
+// is known): a context keeps the K of the last one-slab call per key, the guess on which a
+// snapshot call is enqueued before its plan is read back.
+struct ShapeKey {
+    size_t N;
+    uint32_t cell_bits, alpha_bits;
+    int num_terms, gridder, semantics;
+    bool operator<(const ShapeKey& o) const {
+        return std::tie(N, cell_bits, alpha_bits, num_terms, gridder, semantics) <
+               std::tie(o.N, o.cell_bits, o.alpha_bits, o.num_terms, o.gridder, o.semantics);
+    }
+};
+
 // What the host reads back, in one page-locked block: the plan (with its rotation rows, whose
 // first two give the vis scale |det|) between planning and gridding, the gridder's totals and the
 // consistency word at the end of a call.
@@ -187,6 +208,8 @@ struct svdi_context {
     size_t conv_n = 0, conv_g = 0;
     const float* conv_at = nullptr;
     int cu_count = 0;
+    // K of the last one-slab snapshot call per shape key (the guess of a speculative call).
+    std::map<svdi::ShapeKey, int> shapes;
     // CUs the window kernel's persistent launch leaves free: nonzero on the lanes of a batched
     // call, whose other lanes' kernels (binning, FFT, image) then run beside it.
     int grid_reserve = 0;
@@ -290,7 +313,30 @@ struct ImageParams {
     int row_lo, col_lo;
     int signed_out;
     int lin_q0, lin_r0;     // (G G / 2 + G / 2) = lin_q0 G + lin_r0 (set by launch_image)
+    const DevPlan* dplan;   // as GridParams::dplan
 };
+
+// The plan's values from the device record, for a call enqueued before its plan was read back
+// (the same values, formed the same way, as the host copies into the parameters otherwise).
+__device__ inline void take_device_plan(GridParams& gp) {
+    const DevPlan* p = gp.dplan;
+    if (p == nullptr) return;
+    gp.r3_min = p->r3_min;
+    gp.slab_width = p->slab_width;
+    gp.inv_slab_width = p->inv_slab_width;
+    gp.inv_half = p->inv_slab_half_range;
+    gp.r3_centre = p->r3_centre;
+    gp.det = p->det;
+    gp.inv_det = p->inv_det;
+}
+__device__ inline void take_device_plan(ImageParams& ip) {
+    const DevPlan* p = ip.dplan;
+    if (p == nullptr) return;
+    ip.r3_min = p->r3_min;
+    ip.slab_width = p->slab_width;
+    ip.slab_half = p->slab_half_range;
+    ip.r3_centre = p->r3_centre;
+}
 void launch_image(svdi_context* ctx, const float2* planes, const float* vin_dev,
                   const float* conv, const ImageParams& ip, float* image);
 

@@ -118,6 +118,7 @@ template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(SampleInput in, GridParams gp, TileGeom tg, uint32_t* __restrict__ bcount,
             uint16_t* __restrict__ runpos, uint32_t* __restrict__ runoff) {
+    take_device_plan(gp);
     if constexpr (kSnap) {
         const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
         if (row >= in.num_rows) return;
@@ -162,6 +163,7 @@ __global__ void __launch_bounds__(kBinThreads)
 k_bin_scatter(SampleInput in, GridParams gp, TileGeom tg, const uint32_t* __restrict__ boff,
               uint32_t* __restrict__ bcursor, const uint16_t* __restrict__ runpos,
               const uint32_t* __restrict__ runoff, Records rec, uint32_t limit) {
+    take_device_plan(gp);
     if constexpr (kSnap) {
         const size_t row = blockIdx.x * (size_t)kBinThreads + threadIdx.x;
         if (row >= in.num_rows) return;

@@ -289,6 +289,7 @@ template <bool kSnap>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
             uint32_t* __restrict__ blockhist, uint32_t* __restrict__ blocktot) {
+    take_device_plan(gp);
     extern __shared__ uint32_t hist[];
     const size_t blk = blockIdx.x;
     if (!wg.use_lds) {
@@ -392,6 +393,7 @@ __global__ void __launch_bounds__(kBinThreads)
 k_win_scatter(SampleInput in, GridParams gp, WinGeom wg, const uint32_t* __restrict__ blockhist,
               const uint32_t* __restrict__ blocktot, uint32_t* __restrict__ cursor, WRecords rec,
               uint32_t limit, uint32_t* __restrict__ vmax_bits, uint32_t* __restrict__ err) {
+    take_device_plan(gp);
     extern __shared__ uint32_t hist[];
     const int off = gp.G / 2;
     float m = 0.0f;
