@@ -483,48 +483,70 @@ void image_complete(svdi_context* ctx, const svdi_options& o, CallState& cs, svd
     fill_info(cs, ms, info);
 }
 
-// One imaging call on one context: the three phases back to back.
-//
-// With `speculate`, a snapshot whose image size, cell and options were imaged before with one
-// slab and K terms is enqueued whole before its plan is read back, on the guess of one slab and
-// the same K: the gridding and image kernels take the plan's values from the device record.  The
-// host then reads the plan; if its shape (K, one slab) differs from the guess, the call's work is
-// enqueued again with the real plan behind the guessed one on the same stream, which overwrites
-// every buffer and the image.  Either way the image is the one the plan calls for.
+// Enqueueing a snapshot call before its plan is read back.  A context records, per image size,
+// cell and plan-shaping options, the K of its last call that had one slab.  A snapshot call whose
+// key has a record is enqueued whole right after its planning kernels, on the guess of one slab
+// and that K, with the gridding and image kernels taking the plan's values from the device record.
+// When the host has read the plan, it keeps the guessed work if the plan's shape is the guess, and
+// otherwise enqueues the call's work again with the real plan behind it on the same stream, which
+// overwrites every buffer and the image.  Either way the image is the one the plan calls for.
+struct Guess {
+    ShapeKey key{};
+    int K = 0;                                             // 0: no guess
+    CallState sp;                                          // the guessed call's state
+};
+
+// The guess for a call (K = 0 when there is none) and, with one, its work enqueued on `lane`.
+Guess guess_and_enqueue(const std::map<ShapeKey, int>& shapes, svdi_context* lane, const SampleInput& in,
+                        const svdi_options& o, const CallState& cs, float* image) {
+    Guess g;
+    g.key = ShapeKey{cs.N, bits_of(cs.cell), bits_of(o.target_alpha), o.num_terms, o.gridder, o.semantics};
+    if (!in.from_snapshot || o.semantics != SVDI_SEMANTICS_INTEGRATED || o.verbose) return g;
+    const auto it = shapes.find(g.key);
+    if (it == shapes.end()) return g;
+    g.K = it->second;
+    g.sp = cs;
+    g.sp.hp.num_terms = g.K;
+    g.sp.hp.dev.num_slabs = 1;
+    g.sp.hp.batch_slabs = 1;
+    g.sp.det = 1.0f;                                       // (the kernels take the device plan's)
+    const DevPlan* dplan = static_cast<const DevPlan*>(lane->ws.get(Workspace::kPlan, sizeof(DevPlan)));
+    image_enqueue(lane, in, o, g.sp, image, dplan);
+    return g;
+}
+
+// With the plan taken into cs: keep the guessed work or enqueue the call's work again, and record
+// the call's shape.
+void settle_guess(std::map<ShapeKey, int>& shapes, svdi_context* lane, const SampleInput& in, const svdi_options& o,
+                  const Guess& g, CallState& cs, float* image) {
+    const bool one_slab = cs.hp.dev.num_slabs == 1 && cs.hp.batch_slabs == 1;
+    if (g.K > 0 && one_slab && cs.hp.num_terms == g.K) {
+        cs.gridder = g.sp.gridder;
+        cs.tile_totals = g.sp.tile_totals;
+        cs.early = 1;
+    } else {
+        image_enqueue(lane, in, o, cs, image);
+        cs.early = g.K > 0 ? 2 : 0;
+    }
+    if (one_slab) shapes[g.key] = cs.hp.num_terms;
+    else shapes.erase(g.key);
+}
+
+// One imaging call on one context: the three phases back to back (with `speculate`, enqueued
+// before the plan read-back when the context has a guess for it).
 void image_core(svdi_context* ctx, const SampleInput& in, size_t N, float cell,
                 const svdi_options& o, svdi_plan_info* info, float* image, bool speculate = false) {
     CallState cs;
     image_begin(ctx, in, N, cell, o, cs);
-    const ShapeKey key{N, bits_of(cell), bits_of(o.target_alpha), o.num_terms, o.gridder, o.semantics};
-    int guess = 0;
-    if (speculate && in.from_snapshot && o.semantics == SVDI_SEMANTICS_INTEGRATED && !o.verbose) {
-        const auto it = ctx->shapes.find(key);
-        if (it != ctx->shapes.end()) guess = it->second;
-    }
-    CallState sp;
-    if (guess > 0) {
-        sp = cs;
-        sp.hp.num_terms = guess;
-        sp.hp.dev.num_slabs = 1;
-        sp.hp.batch_slabs = 1;
-        sp.det = 1.0f;                                     // (the kernels take the device plan's)
-        const DevPlan* dplan = static_cast<const DevPlan*>(ctx->ws.get(Workspace::kPlan, sizeof(DevPlan)));
-        image_enqueue(ctx, in, o, sp, image, dplan);
-    }
-    SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
-    image_take_plan(ctx, o, cs);
-    const bool one_slab = cs.hp.dev.num_slabs == 1 && cs.hp.batch_slabs == 1;
-    if (guess > 0 && one_slab && cs.hp.num_terms == guess) {
-        cs.gridder = sp.gridder;
-        cs.tile_totals = sp.tile_totals;
-        cs.early = 1;
-    } else {
-        image_enqueue(ctx, in, o, cs, image);
-        cs.early = guess > 0 ? 2 : 0;
-    }
     if (speculate) {
-        if (one_slab) ctx->shapes[key] = cs.hp.num_terms;
-        else ctx->shapes.erase(key);
+        const Guess g = guess_and_enqueue(ctx->shapes, ctx, in, o, cs, image);
+        SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
+        image_take_plan(ctx, o, cs);
+        settle_guess(ctx->shapes, ctx, in, o, g, cs, image);
+    } else {
+        SVDI_HIP(hipEventSynchronize(ctx->ev_plan));
+        image_take_plan(ctx, o, cs);
+        image_enqueue(ctx, in, o, cs, image);
     }
     image_finish_enqueue(ctx, cs.tile_totals, true);
     image_complete(ctx, o, cs, info);
@@ -856,6 +878,7 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
         SVDI_HIP(hipEventRecord(ready, ctx->stream));
         for (size_t j = 0; j < L; ++j) SVDI_HIP(hipStreamWaitEvent(ctx->lanes[j]->stream, ready, 0));
         std::vector<CallState> cs(count);
+        std::vector<Guess> guesses(count);
         std::vector<SampleInput> in(count);
         std::vector<size_t> next(L, 0);                        // position in the lane's queue
         std::vector<long> planned(L, -1);                      // snapshot whose plan is queued
@@ -868,6 +891,7 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
             begin_call(lane);
             in[i] = snapshot_input(lane, snaps[i]);
             image_begin(lane, in[i], sizes[i], cells[i], o, cs[i]);
+            guesses[i] = guess_and_enqueue(ctx->shapes, lane, in[i], o, cs[i], images[i]);
             planned[j] = static_cast<long>(i);
             plan_seq[j] = order_of_plan++;
         };
@@ -901,7 +925,7 @@ int image_batch(svdi_context* ctx, const svdi_snapshot* snaps, size_t count, flo
             const size_t j = static_cast<size_t>(pick);
             const size_t i = static_cast<size_t>(planned[j]);
             image_take_plan(ctx->lanes[j].get(), o, cs[i]);
-            image_enqueue(ctx->lanes[j].get(), in[i], o, cs[i], images[i]);
+            settle_guess(ctx->shapes, ctx->lanes[j].get(), in[i], o, guesses[i], cs[i], images[i]);
             planned[j] = -1;
             begin_next(j);
         }

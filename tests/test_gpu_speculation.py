@@ -63,3 +63,25 @@ def test_guessed_calls_match_fresh_contexts(output):
             assert_parity(img, ref, f"call {i}", whole=1e-5, central=1e-6)
     finally:
         ctx.close()
+
+
+def test_guessed_batched_calls_match_fresh_contexts():
+    """A batched call on a context with a record (K = 8, one slab): every lane's snapshot is
+    enqueued on that guess; the K = 16 and the two-slab snapshots miss and are enqueued again."""
+    snaps = [_snap(5.0, 0.0), _snap(5.0, 1.5), _snap(250.0, 0.0), _snap(200.0, 1.5)]
+    want_early = [1, 1, 2, 2]
+    N, cell = synth.SMALL.image_size, synth.SMALL.cell
+    ctx = S.Context(0)
+    try:
+        _image(ctx, snaps[0], S.OUTPUT_SIGNED)             # the record: K = 8, one slab
+        announce("guessed batched call of 4 snapshots (2 hits, 2 misses)")
+        imgs, plans = S.image_snapshot_batch(ctx, snaps, N, cell, output=S.OUTPUT_SIGNED)
+        for i, d in enumerate(snaps):
+            ref, ref_plan = _fresh(d, S.OUTPUT_SIGNED)
+            assert plans[i]["enqueued_early"] == want_early[i], (i, plans[i]["enqueued_early"])
+            assert plans[i]["num_terms"] == ref_plan["num_terms"] and plans[i]["num_slabs"] == ref_plan["num_slabs"]
+            img = imgs[i].cpu().numpy()
+            assert np.isfinite(img).all()
+            assert_parity(img, ref, f"batched {i}", whole=1e-5, central=1e-6)
+    finally:
+        ctx.close()
