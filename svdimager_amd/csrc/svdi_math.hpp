@@ -99,9 +99,42 @@ constexpr double kBesselSeriesMax = 1e-2;
 // J_0..J_{COUNT-1}(x) with every index known at compile time, so the sequence stays in
 // registers on the device: Miller's backward recurrence runs from m down to COUNT + 1 without
 // stores, then through the COUNT stored terms unrolled.  As the runtime-count form below, except
-// that for COUNT <= 16 and 1e-2 <= |x| < 2 the recurrence starts at the fixed m = COUNT + 12 (even)
-// and runs fully unrolled: there J_m(x) / J_{COUNT-1}(x) < 1e-13, and the recurrence grows by
-// less than (2m / |x|)^m < 1e110, so it needs no rescaling.
+// that for COUNT <= 16 the recurrence starts at a fixed m and runs fully unrolled: for
+// 1e-2 <= |x| < 2 at m = max(COUNT, 8) + 12 (even), where J_m(x) / J_{COUNT-1}(x) < 1e-15 and the
+// recurrence grows by less than (2m / |x|)^m < 1e110; for 2 <= |x| < 8 at m = 36, where
+// J_37(x) < 1e-21, J_37(x) / J_15(x) < 4e-18 and the growth is at most J_0(x) / J_36(x) < 1e41.
+// Neither needs rescaling.
+constexpr double kBesselFixedMax = 8.0;
+constexpr int kBesselFixedStart = 36;
+
+// Miller's backward recurrence from J_M = 1, J_{M+1} = 0 (M even), unrolled, normalised by
+// J_0 + 2 sum_k J_{2k} = 1: J_0..J_{COUNT-1}(ax), ax > 0.
+template <int COUNT, int M>
+SVDI_HD void bessel_miller_fixed(double ax, double* j) {
+    static_assert(M % 2 == 0 && M > COUNT, "start order");
+    const double two_over_x = 2.0 / ax;
+    double jp = 0.0, jc = 1.0, norm = 0.0;
+#pragma unroll
+    for (int k = M; k > COUNT; --k) {
+        const double jm = (k * two_over_x) * jc - jp;   // J_{k-1}
+        jp = jc;
+        jc = jm;
+        if (((k - 1) & 1) == 0) norm += 2.0 * jc;
+    }
+#pragma unroll
+    for (int k = COUNT; k > 0; --k) {
+        const double jm = (k * two_over_x) * jc - jp;
+        jp = jc;
+        jc = jm;
+        j[k - 1] = jc;
+        if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2.0 * jc;
+    }
+    norm += jc;                                        // + J_0
+    const double inv = 1.0 / norm;
+#pragma unroll
+    for (int n = 0; n < COUNT; ++n) j[n] *= inv;
+}
+
 template <int COUNT>
 SVDI_HD void bessel_sequence_fixed(double x, double* j) {
     const double ax = fabs(x);
@@ -120,28 +153,9 @@ SVDI_HD void bessel_sequence_fixed(double x, double* j) {
             p *= h * (1.0 / (n + 1));
         }
     } else if (COUNT <= 16 && ax < 2.0) {
-        constexpr int m = 2 * ((COUNT + 13) / 2);
-        const double two_over_x = 2.0 / ax;
-        double jp = 0.0, jc = 1.0, norm = 0.0;
-#pragma unroll
-        for (int k = m; k > COUNT; --k) {
-            const double jm = (k * two_over_x) * jc - jp;   // J_{k-1}
-            jp = jc;
-            jc = jm;
-            if (((k - 1) & 1) == 0) norm += 2.0 * jc;
-        }
-#pragma unroll
-        for (int k = COUNT; k > 0; --k) {
-            const double jm = (k * two_over_x) * jc - jp;
-            jp = jc;
-            jc = jm;
-            j[k - 1] = jc;
-            if (k - 1 > 0 && ((k - 1) & 1) == 0) norm += 2.0 * jc;
-        }
-        norm += jc;                                        // + J_0
-        const double inv = 1.0 / norm;
-#pragma unroll
-        for (int n = 0; n < COUNT; ++n) j[n] *= inv;
+        bessel_miller_fixed<COUNT, 2 * (((COUNT > 8 ? COUNT : 8) + 13) / 2)>(ax, j);
+    } else if (COUNT <= 16 && ax < kBesselFixedMax) {
+        bessel_miller_fixed<COUNT, kBesselFixedStart>(ax, j);
     } else if (ax >= 1000.0) {
         double j0, j1;
         bessel_j01_asymptotic(ax, &j0, &j1);

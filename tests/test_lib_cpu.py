@@ -215,7 +215,9 @@ PROBE = r"""
 #include "svdi_math.hpp"
 extern "C" {
 void p_bessel(double x, int n, double* j) { svdi::bessel_sequence_f64(x, n, j); }
+void p_bessel_fixed4(double x, double* j) { svdi::bessel_sequence_fixed<4>(x, j); }
 void p_bessel_fixed8(double x, double* j) { svdi::bessel_sequence_fixed<8>(x, j); }
+void p_bessel_fixed16(double x, double* j) { svdi::bessel_sequence_fixed<16>(x, j); }
 void p_bessel_fixed32(double x, double* j) { svdi::bessel_sequence_fixed<32>(x, j); }
 double p_bessel_series(int n, double x) { return svdi::bessel_series(n, x); }
 void p_eigen(const double* a, double* ev, double* vec) { svdi::eigen_sym3(a, ev, vec); }
@@ -241,7 +243,9 @@ def probe(tmp_path_factory):
     dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
     fp = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     P.p_bessel.argtypes = [ctypes.c_double, ctypes.c_int, dp]
+    P.p_bessel_fixed4.argtypes = [ctypes.c_double, dp]
     P.p_bessel_fixed8.argtypes = [ctypes.c_double, dp]
+    P.p_bessel_fixed16.argtypes = [ctypes.c_double, dp]
     P.p_bessel_fixed32.argtypes = [ctypes.c_double, dp]
     P.p_bessel_series.argtypes = [ctypes.c_int, ctypes.c_double]
     P.p_bessel_series.restype = ctypes.c_double
@@ -272,6 +276,24 @@ def test_device_bessel_matches_scipy(probe, x):
         big = np.abs(r) > 1e-30
         if abs(x) < 1000:
             assert np.all(np.abs(j[big] - r[big]) <= 1e-10 * np.abs(r[big])), (x, n)
+
+
+def test_bessel_fixed_start_range(probe):
+    """The unrolled recurrences from fixed start orders (orders up to 16: m = max(COUNT, 8) + 12
+    for 1e-2 <= |x| < 2, m = 36 for 2 <= |x| < 8), on a dense sweep through both ranges and their
+    ends: J_0..J_{COUNT-1} against scipy to 1e-15 absolute, and 1e-10 relative for the orders above
+    1e-30 (near a zero of J_n the relative error of a correct value grows).  A start order too low
+    for its range (m = 16 at COUNT = 4, measured 5e-14 absolute near x = 2) fails the first bound."""
+    xs = np.concatenate([np.linspace(0.01, 7.999, 401), [0.0100001, 1.999, 2.0, 2.001, 7.9999999, 8.0, 8.5],
+                         -np.linspace(0.01, 7.999, 37)])
+    for count, fn in ((4, probe.p_bessel_fixed4), (8, probe.p_bessel_fixed8), (16, probe.p_bessel_fixed16)):
+        for x in xs:
+            ref = special.jv(np.arange(count), x)
+            j = np.zeros(count)
+            fn(float(x), j)
+            assert np.abs(j - ref).max() <= 1e-15, (count, x)
+            big = np.abs(ref) > 1e-30
+            assert np.all(np.abs(j[big] - ref[big]) <= 1e-10 * np.abs(ref[big])), (count, x)
 
 
 def test_bessel_power_series_below_two(probe):
