@@ -355,6 +355,38 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
 int svdi_fft2_inverse(svdi_context* ctx, const void* planes, size_t num_planes, size_t grid_size,
                       size_t row_lo, size_t rows, size_t col_lo, size_t cols, void* out);
 
+/* One TIME snapshot read on the host from a MeasurementSet main table whose columns were
+ * exported as NumPy .npy files (python-casacore's getcol arrays saved with numpy.save): the
+ * integrated CLI's reader (read_measurement_set_snapshot, group_rows_by_time and
+ * read_measurement_set_rows, SVDIntegratedImager.cu:180-378) with casacore's table layer
+ * replaced by one file per column.  column_dir holds TIME.npy, UVW.npy, DATA.npy, FLAG.npy,
+ * optionally WEIGHT_SPECTRUM.npy, SPECTRAL_WINDOW/CHAN_FREQ.npy and optionally
+ * OBSERVATION/ARRAY_CENTER.npy.  Rows are grouped by exact TIME value in ascending order;
+ * polarisations 0 and 3 are taken on the reference's row-axis layout rule.  `data` holds host
+ * pointers owned by the library (weight0 / weight3 NULL without WEIGHT_SPECTRUM); free with
+ * svdi_ms_free_snapshot.  Fails with SVDI_ERR_IO, svdi_last_error() giving the reference's
+ * message where it has one ("Snapshot_Index is outside available TIME snapshots", "DATA has
+ * fewer than 4 polarisations in ..."). */
+typedef struct svdi_ms_snapshot {
+    svdi_snapshot data;         /* host pointers */
+    size_t index;               /* the snapshot's place in TIME order */
+    size_t total_snapshots;     /* distinct TIME values of the table */
+    double time;                /* its TIME value */
+    void* internal;             /* the library's storage; do not touch */
+} svdi_ms_snapshot;
+int svdi_ms_read_snapshot(const char* column_dir, size_t snapshot_index, svdi_ms_snapshot* out);
+void svdi_ms_free_snapshot(svdi_ms_snapshot* snapshot);
+
+/* svdi_image_snapshot on a snapshot in host memory (svdi_snapshot with host pointers, e.g.
+ * from svdi_ms_read_snapshot): the arrays are copied into the context's workspace and the image
+ * (image_size^2 floats, caller-allocated host memory) is copied back.  This is
+ * preprocess_measurement_set_gpu + FIpipeDevice of the integrated CLI
+ * (SVDIntegratedImager.cu:635-853,1498-1761, main :1773-1853), integrated semantics by
+ * default. */
+int svdi_image_host_snapshot(svdi_context* ctx, const svdi_snapshot* host_snapshot, float* image,
+                             size_t image_size, float cell_size, const svdi_options* opts,
+                             svdi_plan_info* info);
+
 /* FITS primary-HDU images, BITPIX -32 (write_fits_2d, SVDIntegratedImager.cu:153-176; the
  * archive's read_fits_image / write_fits_image, pipe_interface.cpp:12-137).  write replaces
  * an existing file.  read returns 1-D (naxes[1] = 1) or 2-D images as float; free with

@@ -371,3 +371,73 @@ def fipipe_archive_ref32(bin_, vin, vre, vim, N: int, cell: float,
             "valid_pixels": int(stats[3]), "alpha_max": float(alpha_max), "num_terms": K,
             "term_errors": errs, "grid_size": G}
     return {"dirty": dirty.reshape(N, N), "signed": signed.reshape(N, N), "plan": plan}
+
+
+def ms_read_snapshot(column_dir: str, snapshot_index: int) -> Dict:
+    """One TIME snapshot of a MeasurementSet whose main-table columns were saved as .npy files
+    (TIME, UVW, DATA, FLAG, optional WEIGHT_SPECTRUM; SPECTRAL_WINDOW/CHAN_FREQ; optional
+    OBSERVATION/ARRAY_CENTER; the arrays as python-casacore's getcol returns them), restating
+    group_rows_by_time, read_measurement_set_snapshot and read_measurement_set_rows
+    (SVDIntegratedImager.cu:180-378) with detect_row_axis_layout (:99-134).  uvw and the
+    frequencies are kept in FP64 (the reference narrows them to float).  Raises ValueError with
+    the reference's message where it has one."""
+    p = lambda *a: os.path.join(column_dir, *a)
+    time = np.load(p("TIME.npy"))
+    times = sorted(set(time.tolist()))                      # std::map<double, rows>, ascending
+    if snapshot_index >= len(times):
+        raise ValueError("Snapshot_Index is outside available TIME snapshots")
+    t = times[snapshot_index]
+    rows = [r for r in range(time.shape[0]) if time[r] == t]
+    uvw = np.load(p("UVW.npy"))
+    data = np.load(p("DATA.npy"))
+    flag = np.load(p("FLAG.npy"))
+    weight = np.load(p("WEIGHT_SPECTRUM.npy")) if os.path.exists(p("WEIGHT_SPECTRUM.npy")) else None
+    freq = [float(f) for spw in np.atleast_2d(np.load(p("SPECTRAL_WINDOW", "CHAN_FREQ.npy"))) for f in spw]
+    if not freq:
+        raise ValueError("No frequencies found in SPECTRAL_WINDOW")
+    centre = 0.0
+    if os.path.exists(p("OBSERVATION", "ARRAY_CENTER.npy")):
+        ac = np.atleast_2d(np.load(p("OBSERVATION", "ARRAY_CENTER.npy")))
+        if ac.shape[0] > 0 and ac.shape[1] > 0:
+            centre = float(ac[0][0])
+    # Row shape in casacore's IPosition order: the reverse of the numpy row axes.
+    d0, d1 = data.shape[2], data.shape[1]
+    if d1 in (4, 2, 1):
+        chan_axis, nchan, npol = 0, d0, d1
+    elif d0 in (4, 2, 1):
+        chan_axis, nchan, npol = 1, d1, d0
+    elif d0 >= d1:
+        chan_axis, nchan, npol = 0, d0, d1
+    else:
+        chan_axis, nchan, npol = 1, d1, d0
+    if npol < 4:
+        raise ValueError("DATA has fewer than 4 polarisations")
+
+    def at(arr, row, chan, pol):
+        ip = [0, 0]
+        ip[chan_axis] = chan
+        ip[1 - chan_axis] = pol
+        return arr[row, ip[1], ip[0]]                      # numpy axes: (row, IPosition[1], IPosition[0])
+
+    R, C = len(rows), len(freq)
+    out = {k: np.zeros(R * C, np.float32) for k in ("vis0_re", "vis0_im", "vis3_re", "vis3_im")}
+    out["flag0"] = np.zeros(R * C, np.uint8)
+    out["flag3"] = np.zeros(R * C, np.uint8)
+    out["weight0"] = None if weight is None else np.zeros(R * C, np.float32)
+    out["weight3"] = None if weight is None else np.zeros(R * C, np.float32)
+    out["uvw"] = np.zeros((R, 3), np.float64)
+    for lr, row in enumerate(rows):
+        out["uvw"][lr] = uvw[row]
+        for c in range(C):
+            i, cr = lr + c * R, c % nchan
+            v0, v3 = np.complex64(at(data, row, cr, 0)), np.complex64(at(data, row, cr, 3))
+            out["vis0_re"][i], out["vis0_im"][i] = v0.real, v0.imag
+            out["vis3_re"][i], out["vis3_im"][i] = v3.real, v3.imag
+            out["flag0"][i] = 1 if at(flag, row, cr, 0) else 0
+            out["flag3"][i] = 1 if at(flag, row, cr, 3) else 0
+            if weight is not None:
+                out["weight0"][i] = at(weight, row, cr, 0)
+                out["weight3"][i] = at(weight, row, cr, 3)
+    out.update(num_rows=R, num_channels=C, freq_hz=np.array(freq, np.float64), array_center_x=centre,
+               time=t, index=snapshot_index, total_snapshots=len(times))
+    return out

@@ -32,7 +32,8 @@ EXPORTED_SYMBOLS = [
     "svdi_image_snapshot_noalloc", "svdi_image_snapshots", "svdi_fft2_inverse",
     "svdi_image_snapshot_planned", "svdi_set_tuning", "svdi_get_tuning", "svdi_autotune",
     "svdi_image_snapshots_planned", "svdi_service_create", "svdi_service_destroy", "svdi_service_submit",
-    "svdi_service_collect", "svdi_service_stats",
+    "svdi_service_collect", "svdi_service_stats", "svdi_ms_read_snapshot", "svdi_ms_free_snapshot",
+    "svdi_image_host_snapshot",
     "_Z6FIpipePfS_S_S_S_mmf",
 ]
 STAGES = ("preprocess", "plan", "grid", "fft", "image", "grid_accumulate")
@@ -54,6 +55,11 @@ class Snapshot(ctypes.Structure):
         ("weight0", ctypes.c_void_p), ("weight3", ctypes.c_void_p),
         ("array_center_x", ctypes.c_double),
     ]
+
+
+class MsSnapshot(ctypes.Structure):
+    _fields_ = [("data", Snapshot), ("index", ctypes.c_size_t), ("total_snapshots", ctypes.c_size_t),
+                ("time", ctypes.c_double), ("internal", ctypes.c_void_p)]
 
 
 class Options(ctypes.Structure):
@@ -189,6 +195,10 @@ def lib():
             "svdi_fits_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p),
                                               ctypes.POINTER(ctypes.c_long)]),
             "svdi_free": (None, [vp]),
+            "svdi_ms_read_snapshot": (ctypes.c_int, [ctypes.c_char_p, sz, ctypes.POINTER(MsSnapshot)]),
+            "svdi_ms_free_snapshot": (None, [ctypes.POINTER(MsSnapshot)]),
+            "svdi_image_host_snapshot": (ctypes.c_int, [vp, ctypes.POINTER(Snapshot), vp, sz, f,
+                                                        ctypes.POINTER(Options), ctypes.POINTER(PlanInfo)]),
             "svdi_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
             "svdi_last_stage_times": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_float),
                                                      ctypes.c_int]),

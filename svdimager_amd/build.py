@@ -1,8 +1,9 @@
 """Build libsvdimager.so (HIP, gfx950) and the sharedlibrary_gpu CLI in-tree.
 
 Every .hip / .cpp under csrc/ is compiled with hipcc for gfx950 with packed-FP32 instructions
-disabled; fits.cpp and the CLI are plain host C++.  Outputs: svdimager_amd/libsvdimager.so and
-svdimager_amd/bin/sharedlibrary_gpu.
+disabled; fits.cpp, msreader.cpp and the two CLIs are plain host C++.  Outputs:
+svdimager_amd/libsvdimager.so, svdimager_amd/bin/sharedlibrary_gpu and
+svdimager_amd/bin/svd_integrated_imager_gpu.
 """
 from __future__ import annotations
 
@@ -23,7 +24,7 @@ ARCH = os.environ.get("SVDI_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["preprocess.hip", "plan.hip", "grid.hip", "tiles.hip", "windows.hip", "fft.hip", "image.hip", "api.cpp",
                "service.cpp"]
-HOST_SOURCES = ["fits.cpp"]
+HOST_SOURCES = ["fits.cpp", "msreader.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -60,17 +61,21 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    cli = os.path.join(BIN, "sharedlibrary_gpu")
-    _run(["g++", *HOST_FLAGS, os.path.join(CSRC, "cli_sharedlibrary_gpu.cpp"), "-o", cli,
-          f"-L{PKG}", "-lsvdimager", "-Wl,-rpath,$ORIGIN/.."])
+    clis = []
+    for name, src in (("sharedlibrary_gpu", "cli_sharedlibrary_gpu.cpp"),
+                      ("svd_integrated_imager_gpu", "cli_integrated_imager_gpu.cpp")):
+        cli = os.path.join(BIN, name)
+        _run(["g++", *HOST_FLAGS, os.path.join(CSRC, src), "-o", cli, f"-L{PKG}", "-lsvdimager",
+              "-Wl,-rpath,$ORIGIN/.."])
+        clis.append(cli)
     if verbose:
-        print(f"built {LIB} and {cli}")
+        print(f"built {LIB} and {' '.join(clis)}")
     return LIB
 
 
 def clean() -> None:
     shutil.rmtree(OBJ, ignore_errors=True)
-    for p in (LIB, os.path.join(BIN, "sharedlibrary_gpu")):
+    for p in (LIB, os.path.join(BIN, "sharedlibrary_gpu"), os.path.join(BIN, "svd_integrated_imager_gpu")):
         if os.path.exists(p):
             os.remove(p)
 

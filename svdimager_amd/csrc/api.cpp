@@ -18,6 +18,7 @@
 #include <new>
 #include <string>
 
+#include "msreader.hpp"
 #include "svdi_internal.hpp"
 
 // Device scratch of the no-allocation entry point (svdi_image_snapshot_noalloc): a file-scope
@@ -1315,6 +1316,101 @@ int svdi_fipipe_host(svdi_context* ctx, const float* vis_re, const float* vis_im
         image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_ARCHIVE), info, d_img);
         SVDI_HIP(hipMemcpyAsync(dirty_image, d_img, image_size * image_size * sizeof(float),
                                 hipMemcpyDeviceToHost, ctx->stream));
+        SVDI_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int svdi_ms_read_snapshot(const char* column_dir, size_t snapshot_index, svdi_ms_snapshot* out) {
+    if (!column_dir || !out) {
+        svdi::set_error("column_dir / out is null");
+        return SVDI_ERR_INVALID_ARGUMENT;
+    }
+    *out = svdi_ms_snapshot{};
+    std::unique_ptr<svdi::MsSnapshot> s;
+    try {
+        s.reset(new svdi::MsSnapshot(svdi::read_ms_snapshot(column_dir, snapshot_index)));
+    } catch (const std::bad_alloc&) {
+        svdi::set_error("host allocation failed");
+        return SVDI_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        svdi::set_error(e.what());
+        return SVDI_ERR_IO;
+    }
+    svdi_snapshot& d = out->data;
+    d.num_rows = s->num_rows;
+    d.num_channels = s->num_channels;
+    d.uvw = s->uvw.data();
+    d.freq_hz = s->freq_hz.data();
+    d.vis0_re = s->vis0_re.data();
+    d.vis0_im = s->vis0_im.data();
+    d.vis3_re = s->vis3_re.data();
+    d.vis3_im = s->vis3_im.data();
+    d.flag0 = s->flag0.data();
+    d.flag3 = s->flag3.data();
+    d.weight0 = s->has_weights ? s->weight0.data() : nullptr;
+    d.weight3 = s->has_weights ? s->weight3.data() : nullptr;
+    d.array_center_x = s->array_center_x;
+    out->index = s->index;
+    out->total_snapshots = s->total_snapshots;
+    out->time = s->time;
+    out->internal = s.release();
+    return SVDI_OK;
+}
+
+void svdi_ms_free_snapshot(svdi_ms_snapshot* snapshot) {
+    if (!snapshot) return;
+    delete static_cast<svdi::MsSnapshot*>(snapshot->internal);
+    *snapshot = svdi_ms_snapshot{};
+}
+
+int svdi_image_host_snapshot(svdi_context* ctx, const svdi_snapshot* hs, float* image, size_t image_size,
+                             float cell_size, const svdi_options* opts, svdi_plan_info* info) {
+    return guarded([&] {
+        if (!ctx || !hs || !image) throw ArgError("ctx / snapshot / image is null");
+        if (hs->num_rows == 0 || hs->num_channels == 0) throw ArgError("Measurement set is empty");
+        if (!hs->uvw || !hs->freq_hz || !hs->vis0_re || !hs->vis0_im || !hs->vis3_re || !hs->vis3_im ||
+            !hs->flag0 || !hs->flag3)
+            throw ArgError("null host array in the snapshot");
+        if (image_size == 0) throw ArgError("Image_Size must be greater than zero");
+        if (image_size > 46340) throw ArgError("Image_Size too large");
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        DeviceGuard g(ctx->device);
+        begin_call(ctx);
+        const size_t R = hs->num_rows, C = hs->num_channels, S = R * C;
+        const bool weights = hs->weight0 != nullptr && hs->weight3 != nullptr;
+        // The arrays in one workspace slot, each at a 256-byte boundary.
+        auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+        const size_t sizes[11] = {3 * R * sizeof(double), C * sizeof(double), S * sizeof(float), S * sizeof(float),
+                                  S * sizeof(float), S * sizeof(float), S, S,
+                                  weights ? S * sizeof(float) : 0, weights ? S * sizeof(float) : 0, 0};
+        const void* srcs[10] = {hs->uvw, hs->freq_hz, hs->vis0_re, hs->vis0_im, hs->vis3_re, hs->vis3_im,
+                                hs->flag0, hs->flag3, hs->weight0, hs->weight3};
+        size_t total = 0;
+        for (int i = 0; i < 10; ++i) total += up(sizes[i]);
+        char* base = static_cast<char*>(ctx->ws.get(Workspace::kInSnap, total));
+        void* dst[10];
+        size_t off = 0;
+        for (int i = 0; i < 10; ++i) {
+            dst[i] = sizes[i] ? base + off : nullptr;
+            if (sizes[i]) SVDI_HIP(hipMemcpyAsync(dst[i], srcs[i], sizes[i], hipMemcpyHostToDevice, ctx->stream));
+            off += up(sizes[i]);
+        }
+        float* d_img = static_cast<float*>(ctx->ws.get(Workspace::kImage, image_size * image_size * sizeof(float)));
+        svdi_snapshot ds = *hs;
+        ds.uvw = static_cast<const double*>(dst[0]);
+        ds.freq_hz = static_cast<const double*>(dst[1]);
+        ds.vis0_re = static_cast<const float*>(dst[2]);
+        ds.vis0_im = static_cast<const float*>(dst[3]);
+        ds.vis3_re = static_cast<const float*>(dst[4]);
+        ds.vis3_im = static_cast<const float*>(dst[5]);
+        ds.flag0 = static_cast<const uint8_t*>(dst[6]);
+        ds.flag3 = static_cast<const uint8_t*>(dst[7]);
+        ds.weight0 = static_cast<const float*>(dst[8]);
+        ds.weight3 = static_cast<const float*>(dst[9]);
+        SampleInput in = snapshot_input(ctx, ds);
+        image_core(ctx, in, image_size, cell_size, resolve(opts, SVDI_SEMANTICS_INTEGRATED), info, d_img);
+        SVDI_HIP(hipMemcpyAsync(image, d_img, image_size * image_size * sizeof(float), hipMemcpyDeviceToHost,
+                                ctx->stream));
         SVDI_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -136,7 +136,7 @@ class Workspace {
     enum Slot {
         kPartials, kPlan, kGeometry, kRot, kConv, kPlanes, kRowFft, kColFft, kTwiddles,
         kBinCounts, kBinOffsets, kBinCursor, kRecPos, kRecVim, kItems, kMisc, kImage,
-        kInBin, kInVre, kInVim, kInVin, kR3Partials, kScanSums, kRunPos, kRunOff, kItemCounts, kItemOffsets, kWinPartial, kSlots
+        kInBin, kInVre, kInVim, kInVin, kInSnap, kR3Partials, kScanSums, kRunPos, kRunOff, kItemCounts, kItemOffsets, kWinPartial, kSlots
     };
     void* get(Slot s, size_t bytes);
     void release();

@@ -279,3 +279,62 @@ def read_fits_image(filename: str) -> np.ndarray:
     finally:
         lib().svdi_free(data)
     return arr if n2 == 1 else arr.reshape(n2, n1)
+
+
+def read_measurement_set_snapshot(column_dir: str, snapshot_index: int) -> Dict:
+    """read_measurement_set_snapshot (SVDIntegratedImager.cu:359-378) on a MeasurementSet whose
+    main-table columns were saved as .npy files (svdi_ms_read_snapshot): TIME.npy, UVW.npy,
+    DATA.npy, FLAG.npy, optional WEIGHT_SPECTRUM.npy, SPECTRAL_WINDOW/CHAN_FREQ.npy and optional
+    OBSERVATION/ARRAY_CENTER.npy, each as python-casacore's getcol returns it.  Returns the
+    snapshot dict DeviceSnapshot takes (uvw, freq, vis0_re ... weight3, num_rows, num_channels,
+    array_center_x) plus index, time and total_snapshots."""
+    ms = _lib.MsSnapshot()
+    check(lib().svdi_ms_read_snapshot(column_dir.encode(), int(snapshot_index), ctypes.byref(ms)))
+    try:
+        d = ms.data
+        R, C = int(d.num_rows), int(d.num_channels)
+        S = R * C
+
+        def take(ptr, ctype, n, dtype):
+            if not ptr:
+                return None
+            a = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,))
+            return a.astype(dtype, copy=True)
+        out = {"uvw": take(d.uvw, ctypes.c_double, 3 * R, np.float64).reshape(R, 3),
+               "freq": take(d.freq_hz, ctypes.c_double, C, np.float64)}
+        for k in ("vis0_re", "vis0_im", "vis3_re", "vis3_im", "weight0", "weight3"):
+            out[k] = take(getattr(d, k), ctypes.c_float, S, np.float32)
+        for k in ("flag0", "flag3"):
+            out[k] = take(getattr(d, k), ctypes.c_uint8, S, np.uint8)
+        out.update(num_rows=R, num_channels=C, array_center_x=float(d.array_center_x), index=int(ms.index),
+                   time=float(ms.time), total_snapshots=int(ms.total_snapshots))
+        return out
+    finally:
+        lib().svdi_ms_free_snapshot(ctypes.byref(ms))
+
+
+def image_host_snapshot(ctx: Context, snap: Dict, image_size: int, cell_size: float, **options):
+    """svdi_image_host_snapshot: a snapshot dict of host numpy arrays (read_measurement_set_snapshot,
+    synth.make_snapshot) imaged through the context; returns (image, plan)."""
+    keep = []
+
+    def ptr(k, dtype):
+        if snap.get(k) is None:
+            return None
+        a = np.ascontiguousarray(snap[k], dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data
+    s = Snapshot()
+    s.num_rows, s.num_channels = int(snap["num_rows"]), int(snap["num_channels"])
+    s.uvw, s.freq_hz = ptr("uvw", np.float64), ptr("freq", np.float64)
+    for k in ("vis0_re", "vis0_im", "vis3_re", "vis3_im", "weight0", "weight3"):
+        setattr(s, k, ptr(k, np.float32))
+    s.flag0, s.flag3 = ptr("flag0", np.uint8), ptr("flag3", np.uint8)
+    s.array_center_x = float(snap["array_center_x"])
+    N = int(image_size)
+    out = np.zeros(N * N, np.float32)
+    opts = default_options(**options)
+    info = PlanInfo()
+    check(lib().svdi_image_host_snapshot(ctx.handle, ctypes.byref(s), out.ctypes.data, N, float(cell_size),
+                                         ctypes.byref(opts), ctypes.byref(info)))
+    return out.reshape(N, N), info.as_dict()

@@ -127,3 +127,27 @@ def measurement_set_columns(snap: Dict):
         wgt[:, :, 0] = rc(snap["weight0"])
         wgt[:, :, 3] = rc(snap["weight3"])
     return vis, np.asarray(snap["uvw"], np.float64), flag, np.asarray(snap["freq"], np.float64), wgt
+
+
+def save_measurement_set_columns(out_dir: str, snaps, times) -> str:
+    """Snapshot dicts (svdi_snapshot fields, one channel list shared by all) saved as the column
+    directory that svd_integrated_imager_gpu and svdi_ms_read_snapshot read: the rows of
+    snapshot i carry TIME times[i]; DATA / FLAG / WEIGHT_SPECTRUM as (rows, channels, 4) with XX
+    in polarisation 0 and YY in 3, as python-casacore's getcol returns them; CHAN_FREQ one
+    spectral window; ARRAY_CENTER (x, 0, 0) from the first snapshot.  Returns out_dir."""
+    cols = [measurement_set_columns(s) for s in snaps]
+    freq = cols[0][3]
+    if any(not np.array_equal(c[3], freq) for c in cols):
+        raise ValueError("the snapshots must share one channel list")
+    os.makedirs(os.path.join(out_dir, "SPECTRAL_WINDOW"), exist_ok=True)
+    os.makedirs(os.path.join(out_dir, "OBSERVATION"), exist_ok=True)
+    time = np.concatenate([np.full(c[1].shape[0], float(t)) for c, t in zip(cols, times)])
+    np.save(os.path.join(out_dir, "TIME.npy"), time)
+    np.save(os.path.join(out_dir, "UVW.npy"), np.concatenate([c[1] for c in cols]))
+    np.save(os.path.join(out_dir, "DATA.npy"), np.concatenate([c[0] for c in cols]))
+    np.save(os.path.join(out_dir, "FLAG.npy"), np.concatenate([c[2] for c in cols]))
+    np.save(os.path.join(out_dir, "WEIGHT_SPECTRUM.npy"), np.concatenate([c[4] for c in cols]))
+    np.save(os.path.join(out_dir, "SPECTRAL_WINDOW", "CHAN_FREQ.npy"), freq.reshape(1, -1))
+    np.save(os.path.join(out_dir, "OBSERVATION", "ARRAY_CENTER.npy"),
+            np.array([[float(snaps[0]["array_center_x"]), 0.0, 0.0]]))
+    return out_dir

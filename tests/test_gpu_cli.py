@@ -14,7 +14,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ARCHIVE_DIRTY, ROOT, announce
+from conftest import ARCHIVE_DIRTY, ROOT, announce, assert_parity
 from oracle import oracle as O
 from svdimager_amd import prepare, read_fits_image
 
@@ -48,3 +48,52 @@ def test_sharedlibrary_gpu_end_to_end(tmp_path, small_snap):
     assert err <= ARCHIVE_DIRTY
     assert abs(float(dirty.sum()) - float(ref["dirty"].sum())) <= 1e-4 * float(np.abs(ref["dirty"]).sum())
     assert np.all(dirty >= 0.0)
+
+
+INTEGRATED_CLI = os.path.join(ROOT, "svdimager_amd", "bin", "svd_integrated_imager_gpu")
+
+
+def test_svd_integrated_imager_gpu_end_to_end(tmp_path, small_snap):
+    """The integrated CLI (SVDIntegratedImager.cu:1763-1854) on a column directory of two TIME
+    snapshots: the small snapshot at TIME 100 and a copy with half its visibilities at TIME 50.
+    Snapshot 1 must be the small snapshot's dirty image as the library's device path makes it
+    (the same kernels; the order in which records reach a bin and float atomics reach a pixel
+    varies from run to run, so the bound is the one between two runs of the device path: 1e-5 of
+    peak over the image, 1e-6 over its central half), snapshot 0 half of it (the dirty image is
+    linear in |v|), and the stdout / stderr lines the reference's."""
+    import torch
+    import svdimager_amd as S
+    from svdimager_amd import prepare
+    assert os.path.exists(INTEGRATED_CLI), "the CLI is built with the library"
+    half = dict(small_snap)
+    for k in ("vis0_re", "vis0_im", "vis3_re", "vis3_im"):
+        half[k] = (0.5 * small_snap[k]).astype(np.float32)
+    ms = prepare.save_measurement_set_columns(str(tmp_path / "ms"), [small_snap, half], [100.0, 50.0])
+    N = small_snap["image_size"]
+    cell = float(np.float32(small_snap["cell"]))
+    images = {}
+    for idx in (1, 0):
+        path = str(tmp_path / f"snap{idx}.fits")
+        announce(f"svd_integrated_imager_gpu small snapshot index {idx} N={N}")
+        r = subprocess.run([INTEGRATED_CLI, ms, str(N), repr(cell), str(idx), path], capture_output=True, text=True,
+                           timeout=180)
+        print(r.stderr[-1500:], r.stdout[-1500:])
+        assert r.returncode == 0, r.stderr
+        R, C = small_snap["num_rows"], small_snap["num_channels"]
+        assert "Found 2 TIME snapshot(s)" in r.stderr
+        assert f"Selected snapshot {idx} time {100 if idx else 50} rows {R} samples {R * C}" in r.stderr
+        assert f"Wrote image FITS: {path}" in r.stderr and "Total pipeline time:" in r.stdout
+        images[idx] = read_fits_image(path).reshape(N, N)
+    ctx = S.Context(0)
+    try:
+        announce("device-path reference image of the small snapshot")
+        ref, _ = S.image_snapshot(ctx, S.DeviceSnapshot(small_snap, "cuda"), N, cell, output=S.OUTPUT_DIRTY)
+        ref = ref.cpu().numpy()
+        announce("host-snapshot entry point on the read-back snapshot")
+        host, _ = S.image_host_snapshot(ctx, S.read_measurement_set_snapshot(ms, 1), N, cell, output=S.OUTPUT_DIRTY)
+        torch.cuda.synchronize()
+    finally:
+        ctx.close()
+    assert float(np.abs(ref).max()) > 0
+    for name, img in (("CLI", images[1]), ("host entry", host), ("CLI half / 0.5", images[0] / 0.5)):
+        assert_parity(img, ref, f"{name} vs device path", whole=1e-5, central=1e-6)
