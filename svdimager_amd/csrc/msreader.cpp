@@ -28,19 +28,25 @@
 #include <cstring>
 #include <iterator>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <sys/types.h>
 
 namespace svdi {
 namespace {
 
 enum class Kind { F64, F32, C64, C128, B1, U1 };
 
-// One .npy file read whole: element kind, shape and the raw little-endian data.
+// A .npy file's element kind, shape (file_rows: the length of its first axis in the file) and
+// raw little-endian data: the whole array, or only some of its rows (the first axis), in the
+// order asked, shape[0] then being their count.
 struct NpyArray {
     Kind kind = Kind::F64;
     std::vector<size_t> shape;
+    size_t file_rows = 0;
     std::vector<char> data;
     size_t count() const {
         size_t n = 1;
@@ -71,35 +77,40 @@ std::string header_field(const std::string& header, const std::string& key, cons
     return header.substr(p);
 }
 
-NpyArray read_npy(const std::string& path) {
-    std::FILE* f = std::fopen(path.c_str(), "rb");
+struct FileCloser {
+    void operator()(std::FILE* f) const { std::fclose(f); }
+};
+
+void read_exact(std::FILE* f, void* dst, size_t bytes, off_t at, const std::string& path) {
+    if (bytes == 0) return;
+    if (fseeko(f, at, SEEK_SET) != 0 || std::fread(dst, 1, bytes, f) != bytes)
+        throw std::runtime_error("cannot read " + path + " (truncated?)");
+}
+
+// `rows`: the rows (first-axis indices) to read, or null for the whole array.  A table column
+// can be far larger than one snapshot's rows, which are read run by run of consecutive rows.
+NpyArray read_npy(const std::string& path, const std::vector<size_t>* rows = nullptr) {
+    std::unique_ptr<std::FILE, FileCloser> f(std::fopen(path.c_str(), "rb"));
     if (!f) throw std::runtime_error("cannot open " + path + ": " + std::strerror(errno));
-    std::vector<char> file;
-    {
-        char buf[1 << 16];
-        size_t got;
-        while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) file.insert(file.end(), buf, buf + got);
-        const bool err = std::ferror(f) != 0;
-        std::fclose(f);
-        if (err) throw std::runtime_error("cannot read " + path);
-    }
-    if (file.size() < 10 || std::memcmp(file.data(), "\x93NUMPY", 6) != 0)
-        throw std::runtime_error(path + " is not a .npy file");
-    const unsigned major = static_cast<unsigned char>(file[6]);
+    unsigned char pre[12] = {};
+    const size_t got = std::fread(pre, 1, sizeof pre, f.get());
+    if (got < 10 || std::memcmp(pre, "\x93NUMPY", 6) != 0) throw std::runtime_error(path + " is not a .npy file");
+    const unsigned major = pre[6];
     size_t hlen, hoff;
     if (major == 1) {
-        hlen = static_cast<unsigned char>(file[8]) | (static_cast<size_t>(static_cast<unsigned char>(file[9])) << 8);
+        hlen = pre[8] | (static_cast<size_t>(pre[9]) << 8);
         hoff = 10;
     } else if (major == 2 || major == 3) {
-        if (file.size() < 12) throw std::runtime_error(path + " is truncated");
+        if (got < 12) throw std::runtime_error(path + " is truncated");
         hlen = 0;
-        for (int i = 0; i < 4; ++i) hlen |= static_cast<size_t>(static_cast<unsigned char>(file[8 + i])) << (8 * i);
+        for (int i = 0; i < 4; ++i) hlen |= static_cast<size_t>(pre[8 + i]) << (8 * i);
         hoff = 12;
     } else {
         throw std::runtime_error(path + ": unsupported .npy version " + std::to_string(major));
     }
-    if (hoff + hlen > file.size()) throw std::runtime_error(path + " is truncated");
-    const std::string header(file.data() + hoff, hlen);
+    if (hlen > (1u << 20)) throw std::runtime_error(path + ": .npy header too long");
+    std::string header(hlen, ' ');
+    read_exact(f.get(), &header[0], hlen, static_cast<off_t>(hoff), path);
 
     NpyArray a;
     const std::string descr = header_field(header, "descr", path);
@@ -130,11 +141,31 @@ NpyArray read_npy(const std::string& path) {
         a.shape.push_back(static_cast<size_t>(v));
         p += used;
     }
-    const size_t bytes = a.count() * kind_size(a.kind);
-    const size_t doff = hoff + hlen;
-    if (file.size() - doff < bytes) throw std::runtime_error(path + " holds fewer elements than its shape");
-    a.data.assign(file.begin() + static_cast<std::ptrdiff_t>(doff),
-                  file.begin() + static_cast<std::ptrdiff_t>(doff + bytes));
+    const size_t esize = kind_size(a.kind);
+    const off_t doff = static_cast<off_t>(hoff + hlen);
+    if (fseeko(f.get(), 0, SEEK_END) != 0) throw std::runtime_error("cannot read " + path);
+    const off_t fsize = ftello(f.get());
+    if (fsize < doff || static_cast<size_t>(fsize - doff) < a.count() * esize)
+        throw std::runtime_error(path + " holds fewer elements than its shape");
+    if (rows == nullptr) {
+        a.file_rows = a.shape.empty() ? 1 : a.shape[0];
+        a.data.resize(a.count() * esize);
+        read_exact(f.get(), a.data.data(), a.data.size(), doff, path);
+        return a;
+    }
+    if (a.shape.empty()) throw std::runtime_error(path + ": expected one entry per row");
+    a.file_rows = a.shape[0];
+    const size_t row_bytes = (a.file_rows ? a.count() / a.file_rows : 0) * esize;
+    a.shape[0] = rows->size();
+    a.data.resize(rows->size() * row_bytes);
+    for (size_t i = 0; i < rows->size();) {
+        size_t j = i + 1;
+        while (j < rows->size() && (*rows)[j] == (*rows)[j - 1] + 1) ++j;
+        if ((*rows)[j - 1] >= a.file_rows) throw std::runtime_error(path + " has fewer rows than TIME");
+        read_exact(f.get(), a.data.data() + i * row_bytes, (j - i) * row_bytes,
+                   doff + static_cast<off_t>((*rows)[i] * row_bytes), path);
+        i = j;
+    }
     return a;
 }
 
@@ -206,23 +237,25 @@ MsSnapshot read_ms_snapshot(const std::string& dir, size_t snapshot_index) {
     const std::vector<size_t>& rows = g->second;
 
     // read_measurement_set_rows (SVDIntegratedImager.cu:180-325)
-    const NpyArray uvw = read_npy(base + "UVW.npy");
-    const NpyArray data = read_npy(base + "DATA.npy");
-    const NpyArray flag = read_npy(base + "FLAG.npy");
+    // Only the snapshot's rows of the per-row columns are read.
+    const NpyArray uvw = read_npy(base + "UVW.npy", &rows);
+    const NpyArray data = read_npy(base + "DATA.npy", &rows);
+    const NpyArray flag = read_npy(base + "FLAG.npy", &rows);
     const bool has_weight = file_exists(base + "WEIGHT_SPECTRUM.npy");
     NpyArray weight;
-    if (has_weight) weight = read_npy(base + "WEIGHT_SPECTRUM.npy");
+    if (has_weight) weight = read_npy(base + "WEIGHT_SPECTRUM.npy", &rows);
     require_real(uvw, "UVW");
-    if (uvw.shape.size() != 2 || uvw.shape[0] != table_rows || uvw.shape[1] != 3)
+    if (uvw.shape.size() != 2 || uvw.file_rows != table_rows || uvw.shape[1] != 3)
         throw std::runtime_error("UVW does not have 3 entries per row of TIME");
     if (data.kind != Kind::C64 && data.kind != Kind::C128) throw std::runtime_error("DATA must hold complex values");
     if (data.shape.size() != 3) throw std::runtime_error("DATA does not have 2 dimensions per row");
-    if (data.shape[0] != table_rows) throw std::runtime_error("DATA does not have one row per row of TIME");
+    if (data.file_rows != table_rows) throw std::runtime_error("DATA does not have one row per row of TIME");
     if (flag.kind != Kind::B1 && flag.kind != Kind::U1) throw std::runtime_error("FLAG must hold booleans");
-    if (flag.shape != data.shape) throw std::runtime_error("FLAG's shape differs from DATA's");
+    if (flag.shape != data.shape || flag.file_rows != table_rows) throw std::runtime_error("FLAG's shape differs from DATA's");
     if (has_weight) {
         require_real(weight, "WEIGHT_SPECTRUM");
-        if (weight.shape != data.shape) throw std::runtime_error("WEIGHT_SPECTRUM's shape differs from DATA's");
+        if (weight.shape != data.shape || weight.file_rows != table_rows)
+            throw std::runtime_error("WEIGHT_SPECTRUM's shape differs from DATA's");
     }
 
     const NpyArray chan_freq = read_npy(base + "SPECTRAL_WINDOW/CHAN_FREQ.npy");
@@ -276,13 +309,12 @@ MsSnapshot read_ms_snapshot(const std::string& dir, size_t snapshot_index) {
         return ip[1] * b + ip[0];
     };
     const size_t per_row = a * b;
-    for (size_t lr = 0; lr < R; ++lr) {
-        const size_t row = rows[lr];
-        for (int k = 0; k < 3; ++k) s.uvw[3 * lr + k] = real_at(uvw, 3 * row + k);
+    for (size_t lr = 0; lr < R; ++lr) {                    // (the arrays hold the snapshot's rows)
+        for (int k = 0; k < 3; ++k) s.uvw[3 * lr + k] = real_at(uvw, 3 * lr + k);
         for (size_t c = 0; c < C; ++c) {
             const size_t dst = lr + c * R;
             const size_t cr = c % L.num_channels;
-            const size_t e0 = row * per_row + element(cr, 0), e3 = row * per_row + element(cr, 3);
+            const size_t e0 = lr * per_row + element(cr, 0), e3 = lr * per_row + element(cr, 3);
             const std::complex<float> v0 = complex_at(data, e0), v3 = complex_at(data, e3);
             s.vis0_re[dst] = v0.real();
             s.vis0_im[dst] = v0.imag();

@@ -137,6 +137,17 @@ def test_reader_errors(tmp_path):
     np.save(os.path.join(d5, "DATA.npy"), np.zeros((len(TIMES), 5, 4), np.int32))
     with pytest.raises(S.SvdiError, match="unsupported element type"):
         S.read_measurement_set_snapshot(d5, 0)
+    d6 = write_ms(str(tmp_path / "f"), TIMES)
+    np.save(os.path.join(d6, "FLAG.npy"), np.zeros((len(TIMES) - 1, 5, 4), bool))
+    with pytest.raises(S.SvdiError, match="fewer rows than TIME"):
+        S.read_measurement_set_snapshot(d6, 0)             # TIME 2.0 holds the last row
+    with pytest.raises(S.SvdiError, match="FLAG's shape differs"):
+        S.read_measurement_set_snapshot(d6, 2)             # TIME 9.5: rows 3 and 6 only
+    d7 = write_ms(str(tmp_path / "g"), TIMES)
+    path = os.path.join(d7, "DATA.npy")
+    os.truncate(path, os.path.getsize(path) - 8)
+    with pytest.raises(S.SvdiError, match="fewer elements than its shape"):
+        S.read_measurement_set_snapshot(d7, 0)
 
 
 def test_cli_exit_codes_before_the_gpu(tmp_path):
