@@ -1377,12 +1377,12 @@ int svdi_image_host_snapshot(svdi_context* ctx, const svdi_snapshot* hs, float* 
         DeviceGuard g(ctx->device);
         begin_call(ctx);
         const size_t R = hs->num_rows, C = hs->num_channels, S = R * C;
-        const bool weights = hs->weight0 != nullptr && hs->weight3 != nullptr;
-        // The arrays in one workspace slot, each at a 256-byte boundary.
+        // The arrays in one workspace slot, each at a 256-byte boundary (a NULL weight array
+        // stays NULL: weights 1, as on the device path).
         auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
-        const size_t sizes[11] = {3 * R * sizeof(double), C * sizeof(double), S * sizeof(float), S * sizeof(float),
+        const size_t sizes[10] = {3 * R * sizeof(double), C * sizeof(double), S * sizeof(float), S * sizeof(float),
                                   S * sizeof(float), S * sizeof(float), S, S,
-                                  weights ? S * sizeof(float) : 0, weights ? S * sizeof(float) : 0, 0};
+                                  hs->weight0 ? S * sizeof(float) : 0, hs->weight3 ? S * sizeof(float) : 0};
         const void* srcs[10] = {hs->uvw, hs->freq_hz, hs->vis0_re, hs->vis0_im, hs->vis3_re, hs->vis3_im,
                                 hs->flag0, hs->flag3, hs->weight0, hs->weight3};
         size_t total = 0;

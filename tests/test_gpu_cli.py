@@ -97,3 +97,28 @@ def test_svd_integrated_imager_gpu_end_to_end(tmp_path, small_snap):
     assert float(np.abs(ref).max()) > 0
     for name, img in (("CLI", images[1]), ("host entry", host), ("CLI half / 0.5", images[0] / 0.5)):
         assert_parity(img, ref, f"{name} vs device path", whole=1e-5, central=1e-6)
+
+
+@pytest.mark.parametrize("weights", ["both", "none", "xx_only"])
+def test_host_snapshot_entry_matches_device_path(small_snap, weights):
+    """svdi_image_host_snapshot (host arrays copied in by the library) against
+    svdi_image_snapshot on the same arrays already on the device, signed image: with both weight
+    arrays, without WEIGHT_SPECTRUM (weights 1), and with only polarisation 0's weights."""
+    import svdimager_amd as S
+    snap = dict(small_snap)
+    if weights == "none":
+        snap["weight0"] = snap["weight3"] = None
+    elif weights == "xx_only":
+        snap["weight3"] = None
+    N = small_snap["image_size"]
+    cell = float(np.float32(small_snap["cell"]))
+    ctx = S.Context(0)
+    try:
+        announce(f"host-snapshot entry, weights {weights}")
+        host, hplan = S.image_host_snapshot(ctx, snap, N, cell, output=S.OUTPUT_SIGNED)
+        dev, dplan = S.image_snapshot(ctx, S.DeviceSnapshot(snap, "cuda"), N, cell, output=S.OUTPUT_SIGNED)
+        dev = dev.cpu().numpy()
+    finally:
+        ctx.close()
+    assert hplan["num_terms"] == dplan["num_terms"] and hplan["num_slabs"] == dplan["num_slabs"]
+    assert_parity(host, dev, f"host entry vs device path ({weights})", whole=1e-5, central=1e-6)
