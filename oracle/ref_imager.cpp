@@ -215,19 +215,24 @@ void fft1d_inverse_f64(std::complex<double>* data, size_t n, std::complex<double
     std::complex<double>* in = data;
     std::complex<double>* out = work;
     size_t ns = 1;
-    std::vector<std::complex<double>> v, y, root;
+    std::vector<std::complex<double>> v, y, root, tw;
     for (size_t r : radices) {
         v.resize(r);
         y.resize(r);
         root.resize(r);
         for (size_t q = 0; q < r; ++q)
             root[q] = std::polar(1.0, 2.0 * kPi * static_cast<double>(q) / static_cast<double>(r));
+        // The stage's twiddles e^{i ang(jm) q}, ang(jm) = 2 pi jm / (ns r), once per (jm, q).
+        tw.resize(ns * r);
+        for (size_t jm = 0; jm < ns; ++jm) {
+            const double ang = 2.0 * kPi * static_cast<double>(jm) / static_cast<double>(ns * r);
+            for (size_t q = 0; q < r; ++q) tw[jm * r + q] = std::polar(1.0, ang * static_cast<double>(q));
+        }
         const size_t stride = n / r;
         for (size_t j = 0; j < stride; ++j) {
             const size_t jm = j % ns;
-            const double ang = 2.0 * kPi * static_cast<double>(jm) / static_cast<double>(ns * r);
             for (size_t q = 0; q < r; ++q)
-                v[q] = in[j + q * stride] * std::polar(1.0, ang * static_cast<double>(q));
+                v[q] = in[j + q * stride] * tw[jm * r + q];
             for (size_t k = 0; k < r; ++k) {
                 std::complex<double> acc(0.0, 0.0);
                 for (size_t q = 0; q < r; ++q) acc += v[q] * root[(q * k) % r];
@@ -754,7 +759,10 @@ void orc_conv_corr(size_t N, size_t G, int variant, float* out) {
 
 // fused_gridding restated (SVDI.cu:1218-1293).  grid: planes x G x G complex interleaved,
 // accumulated (caller zeroes).  Rows of the grid are split across threads; each thread scans
-// every sample in order, so each cell sums its samples in sample order (deterministic).
+// every sample in order, so each cell sums its samples in sample order (deterministic).  A
+// thread accumulates its rows in a band laid out cell-major ([row][column][plane], the planes of
+// a cell adjacent) and copies it back at the end: the same additions in the same order as
+// adding into the planes directly, without touching nslabs * K pages per cell.
 void orc_grid_svdi(const float* bin, const float* vre, const float* vim, size_t S,
                    const float* vin, float r1r2_scale, size_t G, float r3_min, float slab_width,
                    float inv_half, int K, int slab_start, int nslabs, int total_slabs,
@@ -774,6 +782,20 @@ void orc_grid_svdi(const float* bin, const float* vre, const float* vim, size_t 
 #endif
         const long row_lo = gmin + static_cast<long>(G) * tid / nth;
         const long row_hi = gmin + static_cast<long>(G) * (tid + 1) / nth - 1;
+        const size_t P = static_cast<size_t>(nslabs) * K;
+        const size_t band_rows = row_hi >= row_lo ? static_cast<size_t>(row_hi - row_lo + 1) : 0;
+        std::vector<float> band(band_rows * G * P * 2);
+        auto band_at = [&](long g1, long col, size_t plane) {   // (row g1, array column col)
+            return band.data() + 2 * ((static_cast<size_t>(g1 - row_lo) * G + static_cast<size_t>(col)) * P + plane);
+        };
+        for (long g1 = row_lo; g1 <= row_hi; ++g1)
+            for (size_t col = 0; col < G; ++col)
+                for (size_t plane = 0; plane < P; ++plane) {
+                    const float* src = grid + 2 * (plane * G * G + static_cast<size_t>(g1 + off) * G + col);
+                    float* dst = band_at(g1, static_cast<long>(col), plane);
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                }
         float cheb[kChebMax];
         float k1[16], k2[16];
         for (size_t idx = 0; idx < S; ++idx) {
@@ -800,16 +822,23 @@ void orc_grid_svdi(const float* bin, const float* vre, const float* vim, size_t 
                 for (long g2 = b_lo; g2 <= b_hi; ++g2) {
                     float kv = k1[g1 - a_lo] * k2[g2 - b_lo];
                     if (((g1 + g2) & 1) != 0) kv = -kv;
-                    const long cell = (g1 + off) * static_cast<long>(G) + g2 + off;
                     for (int t = 0; t < K; ++t) {
                         const size_t plane = static_cast<size_t>(slab - slab_start) * K + t;
                         const float mw = (cheb[t] / weight) * kv;
-                        float* c = grid + 2 * (plane * G * G + cell);
+                        float* c = band_at(g1, g2 + off, plane);
                         c[0] += vre[idx] * mw;
                         c[1] += vim[idx] * mw;
                     }
                 }
         }
+        for (long g1 = row_lo; g1 <= row_hi; ++g1)
+            for (size_t col = 0; col < G; ++col)
+                for (size_t plane = 0; plane < P; ++plane) {
+                    const float* src = band_at(g1, static_cast<long>(col), plane);
+                    float* dst = grid + 2 * (plane * G * G + static_cast<size_t>(g1 + off) * G + col);
+                    dst[0] = src[0];
+                    dst[1] = src[1];
+                }
     }
 }
 
