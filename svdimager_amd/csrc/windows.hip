@@ -289,10 +289,10 @@ __device__ __forceinline__ void lds_band(const WinGeom& wg, int& lo, int& n) {
 // to by a 32-bit atomic of 1 << 16 (b & 1)), for blocks of fewer than 65536 samples: a sample
 // adds at most one record to a bin, so no half carries into the other.  Half the LDS lets two
 // workgroups share a CU.
-template <bool kSnap, bool kHist16>
+template <bool kSnap, bool kHist16, typename CountT>
 __global__ void __launch_bounds__(kBinThreads)
 k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ counts,
-            uint32_t* __restrict__ blockhist, uint32_t* __restrict__ blocktot) {
+            CountT* __restrict__ blockhist, uint32_t* __restrict__ blocktot) {
     take_device_plan(gp);
     extern __shared__ uint32_t hist[];
     const size_t blk = blockIdx.x;
@@ -318,16 +318,18 @@ k_win_count(SampleInput in, GridParams gp, WinGeom wg, uint32_t* __restrict__ co
     });
     const uint32_t tot = block_sum(cnt);          // (its barrier also ends the histogram's atomics)
     if (threadIdx.x == 0) blocktot[static_cast<size_t>(blockIdx.y) * gridDim.x + blk] = tot;
-    uint32_t* row = blockhist + blk * wg.nbins + lo;
+    CountT* row = blockhist + blk * wg.nbins + lo;
     for (int b = threadIdx.x; b < n; b += kBinThreads)
-        row[b] = kHist16 ? (hist[b >> 1] >> ((b & 1) << 4)) & 0xffffu : hist[b];
+        row[b] = static_cast<CountT>(kHist16 ? (hist[b >> 1] >> ((b & 1) << 4)) & 0xffffu : hist[b]);
 }
 
 // The per-block histograms turned into per-block record offsets without global atomics per
 // (block, bin): the blocks are split into kBinSegs segments; k_bin_seg_sums forms each segment's
 // count of every bin (and the bin totals), and after the scan of the totals k_bin_block_offsets
-// replaces blockhist[block][bin] by boff[bin] + the records of the bin in earlier blocks.  Each
-// block's records of a bin therefore follow those of the blocks before it.
+// writes offs[block][bin] = boff[bin] + the records of the bin in earlier blocks.  Each block's
+// records of a bin therefore follow those of the blocks before it.  The counts are 16-bit when a
+// block has fewer than 65536 samples (half the histogram traffic), and otherwise 32-bit words
+// that the offsets then replace in place.
 constexpr int kBinSegs = 8;
 
 __device__ __forceinline__ void bin_segment(int nblocks, int seg, int& b0, int& b1) {
@@ -336,15 +338,16 @@ __device__ __forceinline__ void bin_segment(int nblocks, int seg, int& b0, int& 
     b1 = min(b0 + per, nblocks);
 }
 
+template <typename CountT>
 __global__ void __launch_bounds__(256)
-k_bin_seg_sums(const uint32_t* __restrict__ blockhist, int nblocks, int nbins, uint32_t* __restrict__ segsum,
+k_bin_seg_sums(const CountT* __restrict__ blockhist, int nblocks, int nbins, uint32_t* __restrict__ segsum,
                uint32_t* __restrict__ counts) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= kBinSegs * nbins) return;
     const int seg = t / nbins, bin = t - seg * nbins;
     int b0, b1;
     bin_segment(nblocks, seg, b0, b1);
-    const uint32_t* p = blockhist + bin;
+    const CountT* p = blockhist + bin;
     uint32_t sum = 0;
     int b = b0;
     for (; b + 8 <= b1; b += 8) {
@@ -359,8 +362,10 @@ k_bin_seg_sums(const uint32_t* __restrict__ blockhist, int nblocks, int nbins, u
     if (sum != 0) atomicAdd(&counts[bin], sum);
 }
 
+// cnt and offs may be the same words (32-bit counts): each is read before it is written.
+template <typename CountT>
 __global__ void __launch_bounds__(256)
-k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, const uint32_t* __restrict__ segsum,
+k_bin_block_offsets(const CountT* cnt, uint32_t* offs, int nblocks, int nbins, const uint32_t* __restrict__ segsum,
                     const uint32_t* __restrict__ boff) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (t >= kBinSegs * nbins) return;
@@ -369,7 +374,8 @@ k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, co
     bin_segment(nblocks, seg, b0, b1);
     uint32_t run = boff[bin];
     for (int s = 0; s < seg; ++s) run += segsum[s * nbins + bin];
-    uint32_t* p = blockhist + bin;
+    const CountT* p = cnt + bin;
+    uint32_t* o = offs + bin;
     int b = b0;
     for (; b + 8 <= b1; b += 8) {
         uint32_t v[8];
@@ -377,13 +383,13 @@ k_bin_block_offsets(uint32_t* __restrict__ blockhist, int nblocks, int nbins, co
         for (int j = 0; j < 8; ++j) v[j] = p[static_cast<size_t>(b + j) * nbins];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            p[static_cast<size_t>(b + j) * nbins] = run;
+            o[static_cast<size_t>(b + j) * nbins] = run;
             run += v[j];
         }
     }
     for (; b < b1; ++b) {
         const uint32_t v = p[static_cast<size_t>(b) * nbins];
-        p[static_cast<size_t>(b) * nbins] = run;
+        o[static_cast<size_t>(b) * nbins] = run;
         run += v;
     }
 }
@@ -1298,10 +1304,17 @@ WindowJob bin_windows(svdi_context* ctx, const SampleInput& in, const GridParams
 
     // counts[0, nbins) | counts[nbins + 2, +8) = totals.
     uint32_t* counts = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinCounts, (nbins + 12) * sizeof(uint32_t)));
-    // Per-block histograms, then the (block, band) record totals.
+    // Per-block record offsets (the counts themselves when 32-bit), the (block, band) record
+    // totals, then the 16-bit per-block counts when a block has fewer than 65536 samples (a
+    // bin's count per block is at most its sample count).
+    const size_t block_samples = in.from_snapshot ? static_cast<size_t>(kBinRows) * in.num_channels
+                                                  : static_cast<size_t>(kBinThreads) * kBinPerThread;
+    const bool count16 = wg.use_lds && block_samples < 65536;
     const size_t ntot = static_cast<size_t>(blocks) * bands;
-    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(Workspace::kRunOff, (nhist + ntot) * sizeof(uint32_t)));
+    uint32_t* blockhist = static_cast<uint32_t*>(ctx->ws.get(
+        Workspace::kRunOff, (nhist + ntot) * sizeof(uint32_t) + (count16 ? nhist * sizeof(uint16_t) : 0)));
     uint32_t* blocktot = blockhist + nhist;
+    uint16_t* blockcount16 = count16 ? reinterpret_cast<uint16_t*>(blocktot + ntot) : nullptr;
     uint32_t* boff = static_cast<uint32_t*>(ctx->ws.get(Workspace::kBinOffsets, (nbins + 8) * sizeof(uint32_t)));
     uint32_t* cursor = static_cast<uint32_t*>(
         ctx->ws.get(Workspace::kBinCursor, (nbins + 8 + nwt + (wg.use_lds ? kBinSegs * nbins : 0)) * sizeof(uint32_t)));
@@ -1322,34 +1335,44 @@ WindowJob bin_windows(svdi_context* ctx, const SampleInput& in, const GridParams
     rec.r = static_cast<uint4*>(ctx->ws.get(Workspace::kRecPos, max_records * sizeof(uint4)));
 
     launch_fill(ctx, reinterpret_cast<float*>(counts), nbins + 12, 0.0f);
-    // 16-bit count histograms when there are more blocks than CUs, so that two can share a CU,
-    // and a block has fewer than 65536 samples (a bin's count per block is at most its sample
-    // count).  With one block per CU the halved LDS gains nothing and the packing costs a little
-    // (snapshot rows at 64 channels: 35.9 against 33.9 us; preprocessed samples: 50 against 64).
-    const size_t block_samples = in.from_snapshot ? static_cast<size_t>(kBinRows) * in.num_channels
-                                                  : static_cast<size_t>(kBinThreads) * kBinPerThread;
-    const bool hist16 = wg.use_lds && block_samples < 65536 && blocks > static_cast<unsigned>(ctx->cu_count);
+    // 16-bit LDS histograms (two bins per word) when there are also more blocks than CUs, so that
+    // two can share a CU.  With one block per CU the halved LDS gains nothing and the packing
+    // costs a little (snapshot rows at 64 channels: 35.9 against 33.9 us; preprocessed samples:
+    // 50 against 64).
+    const bool hist16 = count16 && blocks > static_cast<unsigned>(ctx->cu_count);
     const size_t count_lds = hist16 ? static_cast<size_t>((wg.band_bins + 1) / 2) * sizeof(uint32_t) : bin_lds;
-    auto count = [&](auto kernel) {
+    auto count = [&](auto kernel, auto* out) {
         allow_dynamic_lds(reinterpret_cast<const void*>(kernel), count_lds);
-        kernel<<<bin_grid, kBinThreads, count_lds, ctx->stream>>>(in, gp, wg, counts, blockhist, blocktot);
+        kernel<<<bin_grid, kBinThreads, count_lds, ctx->stream>>>(in, gp, wg, counts, out, blocktot);
     };
     if (in.from_snapshot) {
-        if (hist16) count(&k_win_count<true, true>);
-        else count(&k_win_count<true, false>);
+        if (hist16) count(&k_win_count<true, true, uint16_t>, blockcount16);
+        else if (count16) count(&k_win_count<true, false, uint16_t>, blockcount16);
+        else count(&k_win_count<true, false, uint32_t>, blockhist);
     } else {
-        if (hist16) count(&k_win_count<false, true>);
-        else count(&k_win_count<false, false>);
+        if (hist16) count(&k_win_count<false, true, uint16_t>, blockcount16);
+        else if (count16) count(&k_win_count<false, false, uint16_t>, blockcount16);
+        else count(&k_win_count<false, false, uint32_t>, blockhist);
     }
     SVDI_LAUNCH_CHECK();
     const unsigned seg_grid = static_cast<unsigned>((kBinSegs * nbins + 255) / 256);
     if (wg.use_lds) {
-        k_bin_seg_sums<<<seg_grid, 256, 0, ctx->stream>>>(blockhist, static_cast<int>(blocks), wg.nbins, segsum, counts);
+        if (count16)
+            k_bin_seg_sums<uint16_t><<<seg_grid, 256, 0, ctx->stream>>>(blockcount16, static_cast<int>(blocks),
+                                                                       wg.nbins, segsum, counts);
+        else
+            k_bin_seg_sums<uint32_t><<<seg_grid, 256, 0, ctx->stream>>>(blockhist, static_cast<int>(blocks), wg.nbins,
+                                                                       segsum, counts);
         SVDI_LAUNCH_CHECK();
     }
     launch_exclusive_scan(ctx, counts, boff, wg.use_lds ? nullptr : cursor, sums, nbins);
     if (wg.use_lds) {
-        k_bin_block_offsets<<<seg_grid, 256, 0, ctx->stream>>>(blockhist, static_cast<int>(blocks), wg.nbins, segsum, boff);
+        if (count16)
+            k_bin_block_offsets<uint16_t><<<seg_grid, 256, 0, ctx->stream>>>(
+                blockcount16, blockhist, static_cast<int>(blocks), wg.nbins, segsum, boff);
+        else
+            k_bin_block_offsets<uint32_t><<<seg_grid, 256, 0, ctx->stream>>>(
+                blockhist, blockhist, static_cast<int>(blocks), wg.nbins, segsum, boff);
         SVDI_LAUNCH_CHECK();
     }
     k_win_item_counts<<<static_cast<unsigned>((nwt + 255) / 256), 256, 0, ctx->stream>>>(boff, wg, icount, multi, totals);
